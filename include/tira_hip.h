@@ -1,0 +1,133 @@
+/*
+ * tira_hip.h -- C ABI of the MI355X (gfx950) tensor-voting library
+ * (tiralib_amd/libtira_hip.so).
+ *
+ * Every compute entry point replaces one TIRA CUDA entry point; the comment
+ * above it names the reference declaration (paths relative to the TIRA tree).
+ * Plain pointers and sizes only. Rules shared by all entry points:
+ *
+ *  - Pointers may be host or device memory (detected with
+ *    hipPointerGetAttributes, as the reference does with
+ *    cudaPointerGetAttributes, tensorvote.h:1880-1890). Host buffers are staged
+ *    through device memory owned by the library.
+ *  - Outputs are caller-allocated (the reference returns new[]/cudaMalloc
+ *    buffers; the C++ header include/tira/hip/tira.hpp restores that
+ *    convention on top of this ABI).
+ *  - Every call returns a tira_status; TIRA_OK is 0. tira_hip_last_error()
+ *    returns a thread-local message for the last failure.
+ *  - Calls are blocking (they synchronise the library stream before
+ *    returning) unless named *_async. The library stream is the null stream
+ *    unless tira_hip_set_stream() installs another one.
+ *  - No global mutable device symbols: calls are re-entrant across host
+ *    threads that use different streams.
+ */
+#ifndef TIRA_HIP_H
+#define TIRA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TIRA_HIP_ABI_VERSION 1
+
+typedef enum tira_status {
+    TIRA_OK = 0,
+    TIRA_ERR_INVALID_ARG = 1,   /* bad shape / sigma / window (pybind checks, tensorvote_tk.cu:73-83, 162-178) */
+    TIRA_ERR_HIP = 2,           /* a HIP runtime call failed */
+    TIRA_ERR_OUT_OF_MEMORY = 3,
+    TIRA_ERR_UNSUPPORTED = 4,   /* valid request this build does not serve (ATV) */
+    TIRA_ERR_NO_DEVICE = 5
+} tira_status;
+
+/* Tensor layouts for the native entry points. */
+#define TIRA_LAYOUT_MAT9    0   /* 9 floats, column-major glm::mat3 (reference 3D layout) */
+#define TIRA_LAYOUT_PACKED6 1   /* 6 floats: xx, xy, xz, yy, yz, zz */
+
+/* Plate-vote evaluation for the 3D vote (samples > 0 is the reference's
+ * numerically integrated plate, tensorvote.h:1651-1695 / 2457-2485). */
+#define TIRA_PLATE_AUTO    0   /* closed form when samples >= power+2 (the Riemann sum is then exact), else literal sum */
+#define TIRA_PLATE_SUM     1   /* literal N-sample Riemann sum, the reference's algorithm, any N */
+#define TIRA_PLATE_CLOSED  2   /* closed form of the same integral (exact for N >= p+2 and for samples == 0) */
+#define TIRA_PLATE_MLP     3   /* learned surrogate with per-voxel integrator fallback (tira_mlp_*) */
+
+/* eig3 flags */
+#define TIRA_EIG_REFINE_FP64   1u  /* FP64 re-solve of voxels with relative eigengap < 1e-4 */
+#define TIRA_EIG_SCALE_INVARIANT 2u /* thresholds relative to the tensor norm (reference: absolute 1e-6) */
+
+/* ------------------------------------------------------------------ runtime */
+int tira_hip_abi_version(void);
+const char* tira_hip_last_error(void);
+int tira_hip_device_count(int* count);
+int tira_hip_set_device(int device);
+int tira_hip_malloc(void** ptr, size_t bytes);
+int tira_hip_free(void* ptr);
+/* kind: 0 = infer from pointer attributes */
+int tira_hip_memcpy(void* dst, const void* src, size_t bytes);
+int tira_hip_memset(void* dst, int value, size_t bytes);
+int tira_hip_synchronize(void);
+int tira_hip_stream_create(void** stream);
+int tira_hip_stream_destroy(void* stream);
+/* Install the stream every later call on this host thread launches on (NULL = null stream). */
+int tira_hip_set_stream(void* stream);
+int tira_hip_event_create(void** event);
+int tira_hip_event_destroy(void* event);
+int tira_hip_event_record(void* event, void* stream);
+int tira_hip_event_elapsed_ms(void* start, void* stop, float* ms);
+int tira_hip_stream_synchronize(void* stream);
+/* Kernel timing: between begin and end, every library kernel launch on this
+ * host thread is bracketed by HIP events on its own stream; end() waits for
+ * them and aggregates per kernel name (e.g. "vote3_closed", "eig3"). */
+int tira_hip_profile_begin(void);
+int tira_hip_profile_end(void);
+int tira_hip_profile_get(const char* kernel_name, double* total_ms, long* count);
+/* Release every cached device buffer held by the library on the current device. */
+int tira_hip_release_workspace(void);
+
+/* ------------------------------------------------------------------ eigen */
+/* tira::cuda::evals2_symmetric(const T* mats, size_t n, int device), eigen.h:768-812.
+ * mats n x 4 [a b b c]; evals n x 2 ascending. device < 0 is rejected (the
+ * reference falls back to its CPU path there; use tira::cpu for that). */
+int tira_evals2_symmetric(const float* mats, size_t n, float* evals, int device);
+/* tira::cuda::evecs2polar_symmetric(mats, evals, n, device), eigen.h:827-893. */
+int tira_evecs2polar_symmetric(const float* mats, const float* evals, size_t n, float* thetas, int device);
+/* tira::cuda::evals3_symmetric(const T* mats, size_t n, int device), eigen.h:895-942.
+ * mats n x 9 column-major (slots 0,1,2,4,5,8 read); evals n x 3 ascending.
+ * FP32 with FP64 re-solve of near-degenerate tensors. */
+int tira_evals3_symmetric(const float* mats, size_t n, float* evals, int device);
+/* tira::cuda::evecs3spherical_symmetric(mats, lambda, n, device), eigen.h:944-1012.
+ * evecs n x 6 = (theta, phi) of e0, e1, e2. */
+int tira_evecs3spherical_symmetric(const float* mats, const float* lambdas, size_t n, float* evecs, int device);
+/* Native batched eig3 (no reference counterpart beyond eigen.h:895-1012):
+ * layout TIRA_LAYOUT_*, evals n x 3 ascending; evec0 / evec2 (n x 3,
+ * cartesian, canonical sign) may be NULL. n_refined (may be NULL) receives
+ * the number of tensors re-solved in FP64. Device pointers only. */
+int tira_eig3_async(const float* tensors, size_t n, int layout, float* evals, float* evec0, float* evec2,
+                    unsigned flags, unsigned long long* n_refined_dev);
+
+/* ------------------------------------------------------------------ 3D voting */
+/* tira::cuda::tensorvote3(input_field, output_field, s0, s1, s2, sigma, sigma2, w,
+ * power, device, stick, plate, samples, is_atv), tensorvote.h:2816-2873.
+ * in / out: s0*s1*s2 x 9 floats (glm::mat3, s2 fastest). is_atv != 0 returns
+ * TIRA_ERR_UNSUPPORTED. */
+int tira_tensorvote3(const float* input_field, float* output_field, unsigned s0, unsigned s1, unsigned s2,
+                     float sigma, float sigma2, unsigned w, unsigned power, int device, int stick, int plate,
+                     unsigned samples, int is_atv);
+/* tira::cuda::_tensorvote3_tk(out, lambdas, evec2, evec0, s0, s1, s2, sigma1,
+ * sigma2, power, w, STICK, PLATE, samples), tensorvote.h:2741-2809.
+ * lambdas n x 3, evec2 / evec0 n x 3 cartesian. */
+int tira_tensorvote3_tk(float* output_field, const float* lambdas, const float* evec2, const float* evec0,
+                        unsigned s0, unsigned s1, unsigned s2, float sigma1, float sigma2, unsigned power,
+                        unsigned w, int stick, int plate, unsigned samples);
+/* Native 3D vote on device buffers, non-blocking on the library stream.
+ * in_layout / out_layout: TIRA_LAYOUT_*; plate_mode: TIRA_PLATE_*. */
+int tira_tensorvote3_async(const float* input_field, int in_layout, float* output_field, int out_layout,
+                           unsigned s0, unsigned s1, unsigned s2, float sigma1, float sigma2, unsigned w,
+                           unsigned power, int stick, int plate, unsigned samples, int plate_mode);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TIRA_HIP_H */

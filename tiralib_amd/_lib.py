@@ -1,0 +1,211 @@
+"""ctypes loader for libtira_hip.so (the C ABI declared in include/tira_hip.h).
+
+The library is the product path: there is no CPU fallback. If the shared
+object is missing or fails to load, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtira_hip.so")
+
+TIRA_OK = 0
+TIRA_ERR_INVALID_ARG = 1
+TIRA_ERR_HIP = 2
+TIRA_ERR_OUT_OF_MEMORY = 3
+TIRA_ERR_UNSUPPORTED = 4
+TIRA_ERR_NO_DEVICE = 5
+
+LAYOUT_MAT9 = 0
+LAYOUT_PACKED6 = 1
+
+PLATE_AUTO = 0
+PLATE_SUM = 1
+PLATE_CLOSED = 2
+PLATE_MLP = 3
+
+EIG_REFINE_FP64 = 1
+EIG_SCALE_INVARIANT = 2
+
+
+class TiraError(RuntimeError):
+    def __init__(self, status, fn, msg):
+        super().__init__(f"{fn} failed with status {status}: {msg}")
+        self.status = status
+
+
+class TiraInvalidArgument(TiraError, ValueError):
+    pass
+
+
+class TiraUnsupported(TiraError, NotImplementedError):
+    pass
+
+
+_vp = C.c_void_p
+_sz = C.c_size_t
+_u = C.c_uint
+_i = C.c_int
+_f = C.c_float
+
+# name -> argtypes (all return int status)
+SIGNATURES = {
+    "tira_hip_device_count": [C.POINTER(_i)],
+    "tira_hip_set_device": [_i],
+    "tira_hip_malloc": [C.POINTER(_vp), _sz],
+    "tira_hip_free": [_vp],
+    "tira_hip_memcpy": [_vp, _vp, _sz],
+    "tira_hip_memset": [_vp, _i, _sz],
+    "tira_hip_synchronize": [],
+    "tira_hip_stream_create": [C.POINTER(_vp)],
+    "tira_hip_stream_destroy": [_vp],
+    "tira_hip_set_stream": [_vp],
+    "tira_hip_stream_synchronize": [_vp],
+    "tira_hip_event_create": [C.POINTER(_vp)],
+    "tira_hip_event_destroy": [_vp],
+    "tira_hip_event_record": [_vp, _vp],
+    "tira_hip_event_elapsed_ms": [_vp, _vp, C.POINTER(_f)],
+    "tira_hip_release_workspace": [],
+    "tira_evals2_symmetric": [_vp, _sz, _vp, _i],
+    "tira_evecs2polar_symmetric": [_vp, _vp, _sz, _vp, _i],
+    "tira_evals3_symmetric": [_vp, _sz, _vp, _i],
+    "tira_evecs3spherical_symmetric": [_vp, _vp, _sz, _vp, _i],
+    "tira_eig3_async": [_vp, _sz, _i, _vp, _vp, _vp, _u, _vp],
+    "tira_tensorvote3": [_vp, _vp, _u, _u, _u, _f, _f, _u, _u, _i, _i, _i, _u, _i],
+    "tira_tensorvote3_tk": [_vp, _vp, _vp, _vp, _u, _u, _u, _f, _f, _u, _u, _i, _i, _u],
+    "tira_tensorvote3_async": [_vp, _i, _vp, _i, _u, _u, _u, _f, _f, _u, _u, _i, _i, _u, _i],
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load():
+    """Load libtira_hip.so (raises if it is absent: no CPU fallback exists)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(f"libtira_hip.so not built at {LIB_PATH}; run `python -m tiralib_amd.build`")
+            L = C.CDLL(LIB_PATH)
+            L.tira_hip_last_error.restype = C.c_char_p
+            L.tira_hip_last_error.argtypes = []
+            L.tira_hip_abi_version.restype = _i
+            for name, args in SIGNATURES.items():
+                fn = getattr(L, name)
+                fn.argtypes = args
+                fn.restype = _i
+            _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return ["tira_hip_abi_version", "tira_hip_last_error", *SIGNATURES.keys()]
+
+
+def call(name, *args):
+    L = load()
+    st = getattr(L, name)(*args)
+    if st != TIRA_OK:
+        msg = (L.tira_hip_last_error() or b"").decode(errors="replace")
+        if st == TIRA_ERR_INVALID_ARG:
+            raise TiraInvalidArgument(st, name, msg)
+        if st == TIRA_ERR_UNSUPPORTED:
+            raise TiraUnsupported(st, name, msg)
+        raise TiraError(st, name, msg)
+    return st
+
+
+def device_count() -> int:
+    L = load()
+    n = C.c_int(0)
+    L.tira_hip_device_count(C.byref(n))
+    return n.value
+
+
+def ptr(a) -> int:
+    """Address of a numpy array or DeviceArray."""
+    if isinstance(a, DeviceArray):
+        return a.ptr
+    return a.ctypes.data
+
+
+class DeviceArray:
+    """A device buffer with a numpy-like shape/dtype, owned by this object."""
+
+    def __init__(self, shape, dtype=np.float32):
+        self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        call("tira_hip_malloc", C.byref(p), max(self.nbytes, 1))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        if a.nbytes:
+            call("tira_hip_memcpy", d.ptr, a.ctypes.data, a.nbytes)
+        return d
+
+    def to_host(self):
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            call("tira_hip_memcpy", out.ctypes.data, self.ptr, self.nbytes)
+        return out
+
+    def zero(self):
+        call("tira_hip_memset", self.ptr, 0, self.nbytes)
+
+    def free(self):
+        if self.ptr:
+            call("tira_hip_free", self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and _lib is not None:
+                _lib.tira_hip_free(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        p = C.c_void_p()
+        call("tira_hip_stream_create", C.byref(p))
+        self.handle = p.value
+
+    def use(self):
+        call("tira_hip_set_stream", self.handle)
+        return self
+
+    def synchronize(self):
+        call("tira_hip_stream_synchronize", self.handle)
+
+    def destroy(self):
+        if self.handle:
+            call("tira_hip_stream_destroy", self.handle)
+            self.handle = None
+
+
+class Event:
+    def __init__(self):
+        p = C.c_void_p()
+        call("tira_hip_event_create", C.byref(p))
+        self.handle = p.value
+
+    def record(self, stream=None):
+        call("tira_hip_event_record", self.handle, stream.handle if stream else None)
+
+    def elapsed_ms(self, end) -> float:
+        ms = C.c_float()
+        call("tira_hip_event_elapsed_ms", self.handle, end.handle, C.byref(ms))
+        return ms.value

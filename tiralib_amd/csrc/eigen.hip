@@ -1,0 +1,302 @@
+// eigen.hip -- batched eigen-solvers for symmetric 2x2 / 3x3 tensor fields
+// on gfx950, and the C-ABI entry points that replace tira::cuda::evals2/3 and
+// evecs2polar / evecs3spherical (eigen.h:700-1012).
+//
+// eig3 kernel design (HBM-bound: 36 B per tensor for packed-6 in / 3 floats
+// out, 48 B in the reference 9-float layout):
+//  * a 256-thread block owns 1024 consecutive tensors; the AoS input is read
+//    with fully coalesced 16-byte loads into LDS and each lane then picks its
+//    tensors out of LDS (the 9-float stride is odd, so no bank conflicts);
+//  * FP32 trigonometric solve for every tensor;
+//  * tensors whose relative eigengap is below 1e-4 are appended to a
+//    block-local list in LDS and re-solved in FP64 after a barrier, so the
+//    FP64 path runs on ~ceil(flagged/64) waves of the block instead of on
+//    every wave (10% scattered near-degenerate tensors would otherwise put
+//    the FP64 path in 99.9% of all waves);
+//  * results are staged in LDS and written back with 16-byte stores.
+#include "common.hpp"
+#include "eig_device.hpp"
+
+#include <cfloat>
+#include <cmath>
+
+namespace th {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kEigThreads = 256;
+constexpr int kEigPerThread = 4;
+constexpr int kEigTile = kEigThreads * kEigPerThread;  // tensors per block
+
+// Power-of-two prescale so that the largest |entry| lies in [0.5, 1): exact
+// in floating point, and it turns the reference's absolute thresholds into
+// relative ones. Returns the exponent to scale eigenvalues back with.
+__device__ __forceinline__ int prescale6(float& a, float& b, float& c, float& d, float& e, float& f) {
+    float m = fmaxf(fmaxf(fmaxf(fabsf(a), fabsf(b)), fmaxf(fabsf(c), fabsf(d))), fmaxf(fabsf(e), fabsf(f)));
+    if (!(m > 0.0f) || !isfinite(m)) return 0;
+    int ex;
+    frexpf(m, &ex);
+    a = ldexpf(a, -ex); b = ldexpf(b, -ex); c = ldexpf(c, -ex);
+    d = ldexpf(d, -ex); e = ldexpf(e, -ex); f = ldexpf(f, -ex);
+    return ex;
+}
+
+// Reads tensor `i` of a staged LDS tile as (a, b, c, d, e, f) = (xx, xy, yy, xz, yz, zz).
+template <int LAYOUT>
+__device__ __forceinline__ void lds_tensor(const float* s, int i, float& a, float& b, float& c, float& d, float& e,
+                                           float& f) {
+    if (LAYOUT == TIRA_LAYOUT_MAT9) {
+        const float* m = s + i * 9;
+        a = m[0]; b = m[1]; d = m[2]; c = m[4]; e = m[5]; f = m[8];
+    } else {
+        const float* m = s + i * 6;
+        a = m[0]; b = m[1]; d = m[2]; c = m[3]; e = m[4]; f = m[5];
+    }
+}
+
+template <int LAYOUT>
+__device__ __forceinline__ void global_tensor(const float* g, size_t i, float& a, float& b, float& c, float& d,
+                                              float& e, float& f) {
+    if (LAYOUT == TIRA_LAYOUT_MAT9) {
+        const float* m = g + i * 9;
+        a = m[0]; b = m[1]; d = m[2]; c = m[4]; e = m[5]; f = m[8];
+    } else {
+        const float* m = g + i * 6;
+        a = m[0]; b = m[1]; d = m[2]; c = m[3]; e = m[4]; f = m[5];
+    }
+}
+
+// evals: n x 3. evec0 / evec2 (n x 3) written when VECS. flags: TIRA_EIG_*.
+template <int LAYOUT, bool VECS>
+__global__ __launch_bounds__(kEigThreads) void k_eig3(const float* __restrict__ in, size_t n,
+                                                      float* __restrict__ evals, float* __restrict__ evec0,
+                                                      float* __restrict__ evec2, unsigned flags,
+                                                      unsigned long long* __restrict__ n_refined) {
+    constexpr int W = (LAYOUT == TIRA_LAYOUT_MAT9) ? 9 : 6;
+    __shared__ float s_in[kEigTile * W];
+    __shared__ float s_out[kEigTile * 3];
+    __shared__ short s_list[kEigTile];
+    __shared__ int s_count;
+
+    const int tid = threadIdx.x;
+    const size_t base = (size_t)blockIdx.x * kEigTile;
+    const int cnt = (int)((n - base) < (size_t)kEigTile ? (n - base) : (size_t)kEigTile);
+    if (tid == 0) s_count = 0;
+
+    // ---- stage the tile: 16-byte loads when full and aligned, else scalar
+    const float* gin = in + base * W;
+    const bool vec_ok = (cnt == kEigTile) && ((((uintptr_t)gin) & 15u) == 0);
+    if (vec_ok) {
+        const f32x4* g4 = reinterpret_cast<const f32x4*>(gin);
+        f32x4* s4 = reinterpret_cast<f32x4*>(s_in);
+#pragma unroll
+        for (int k = 0; k < (kEigTile * W) / (4 * kEigThreads); k++)
+            s4[k * kEigThreads + tid] = __builtin_nontemporal_load(&g4[k * kEigThreads + tid]);
+    } else {
+        for (int k = tid; k < cnt * W; k += kEigThreads) s_in[k] = gin[k];
+    }
+    __syncthreads();
+
+    const bool scale_inv = (flags & TIRA_EIG_SCALE_INVARIANT) != 0;
+    const bool refine = (flags & TIRA_EIG_REFINE_FP64) != 0;
+
+    // ---- FP32 pass
+#pragma unroll
+    for (int j = 0; j < kEigPerThread; j++) {
+        const int i = j * kEigThreads + tid;
+        if (i >= cnt) break;
+        float a, b, c, d, e, f;
+        lds_tensor<LAYOUT>(s_in, i, a, b, c, d, e, f);
+        int ex = 0;
+        if (scale_inv) ex = prescale6(a, b, c, d, e, f);
+        float l0, l1, l2;
+        eig3_vals<float>(a, b, c, d, e, f, 1e-6f, l0, l1, l2);
+        if (refine && eig3_near_degenerate(l0, l1, l2)) {
+            const int slot = atomicAdd(&s_count, 1);
+            s_list[slot] = (short)i;
+        }
+        if (VECS) {
+            float v0[3], v1[3], v2[3];
+            eig3_vecs<float>(a, b, c, d, e, f, l0, l1, l2, 1e-6f, v0, v1, v2);
+            const size_t g = base + i;
+            if (evec0) { evec0[g * 3 + 0] = v0[0]; evec0[g * 3 + 1] = v0[1]; evec0[g * 3 + 2] = v0[2]; }
+            if (evec2) { evec2[g * 3 + 0] = v2[0]; evec2[g * 3 + 1] = v2[1]; evec2[g * 3 + 2] = v2[2]; }
+        }
+        s_out[i * 3 + 0] = ldexpf(l0, ex);
+        s_out[i * 3 + 1] = ldexpf(l1, ex);
+        s_out[i * 3 + 2] = ldexpf(l2, ex);
+    }
+    __syncthreads();
+
+    // ---- FP64 re-solve of the compacted near-degenerate list
+    const int nflag = s_count;
+    for (int k = tid; k < nflag; k += kEigThreads) {
+        const int i = s_list[k];
+        float af, bf, cf, df, ef, ff;
+        lds_tensor<LAYOUT>(s_in, i, af, bf, cf, df, ef, ff);
+        int ex = 0;
+        if (scale_inv) ex = prescale6(af, bf, cf, df, ef, ff);
+        double l0, l1, l2;
+        eig3_vals<double>(af, bf, cf, df, ef, ff, 1e-6, l0, l1, l2);
+        s_out[i * 3 + 0] = ldexpf((float)l0, ex);
+        s_out[i * 3 + 1] = ldexpf((float)l1, ex);
+        s_out[i * 3 + 2] = ldexpf((float)l2, ex);
+        if (VECS) {
+            double v0[3], v1[3], v2[3];
+            eig3_vecs<double>(af, bf, cf, df, ef, ff, l0, l1, l2, 1e-6, v0, v1, v2);
+            const size_t g = base + i;
+            if (evec0) { evec0[g * 3 + 0] = (float)v0[0]; evec0[g * 3 + 1] = (float)v0[1]; evec0[g * 3 + 2] = (float)v0[2]; }
+            if (evec2) { evec2[g * 3 + 0] = (float)v2[0]; evec2[g * 3 + 1] = (float)v2[1]; evec2[g * 3 + 2] = (float)v2[2]; }
+        }
+    }
+    if (n_refined && tid == 0 && nflag) atomicAdd(n_refined, (unsigned long long)nflag);
+    __syncthreads();
+
+    // ---- write back
+    float* gout = evals + base * 3;
+    if (cnt == kEigTile && ((((uintptr_t)gout) & 15u) == 0)) {
+        f32x4* g4 = reinterpret_cast<f32x4*>(gout);
+        const f32x4* s4 = reinterpret_cast<const f32x4*>(s_out);
+#pragma unroll
+        for (int k = 0; k < (kEigTile * 3) / (4 * kEigThreads); k++)
+            __builtin_nontemporal_store(s4[k * kEigThreads + tid], &g4[k * kEigThreads + tid]);
+    } else {
+        for (int k = tid; k < cnt * 3; k += kEigThreads) gout[k] = s_out[k];
+    }
+}
+
+// evecs3spherical_symmetric (eigen.h:744-755): (theta, phi) of e0, e1, e2.
+__global__ __launch_bounds__(256) void k_evecs3spherical(const float* __restrict__ mats,
+                                                         const float* __restrict__ lam, size_t n,
+                                                         float* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float a, b, c, d, e, f;
+    global_tensor<TIRA_LAYOUT_MAT9>(mats, i, a, b, c, d, e, f);
+    float l0 = lam[i * 3], l1 = lam[i * 3 + 1], l2 = lam[i * 3 + 2];
+    float v[3][3];
+    if (eig3_near_degenerate(l0, l1, l2)) {
+        double dv[3][3];
+        eig3_vecs<double>(a, b, c, d, e, f, l0, l1, l2, 1e-6, dv[0], dv[1], dv[2]);
+        for (int k = 0; k < 3; k++)
+            for (int r = 0; r < 3; r++) v[k][r] = (float)dv[k][r];
+    } else {
+        eig3_vecs<float>(a, b, c, d, e, f, l0, l1, l2, 1e-6f, v[0], v[1], v[2]);
+    }
+    for (int k = 0; k < 3; k++) {
+        const float z = fminf(1.0f, fmaxf(-1.0f, v[k][2]));
+        out[i * 6 + 2 * k + 0] = atan2f(v[k][1], v[k][0]);
+        out[i * 6 + 2 * k + 1] = acosf(z);
+    }
+}
+
+// kernel_eval2_symmetric (eigen.h:700-712)
+__global__ __launch_bounds__(256) void k_evals2(const float* __restrict__ m, size_t n, float* __restrict__ ev) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 t = reinterpret_cast<const float4*>(m)[i];
+    float e0, e1;
+    eig2_vals(t.x, t.y, t.w, e0, e1);
+    reinterpret_cast<float2*>(ev)[i] = make_float2(e0, e1);
+}
+
+// kernel_evec2polar_symmetric (eigen.h:714-722)
+__global__ __launch_bounds__(256) void k_evecs2polar(const float* __restrict__ m, const float* __restrict__ ev,
+                                                     size_t n, float* __restrict__ th) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 t = reinterpret_cast<const float4*>(m)[i];
+    float vx, vy, ux, uy;
+    eig2_vecs(t.x, t.y, t.w, ev[i * 2], vx, vy, ux, uy);
+    float t0 = atan2f(vy, vx), t1 = atan2f(uy, ux);
+    const float pi = 3.14159265358979323846f;
+    if (t0 <= -pi) t0 += 2.0f * pi;
+    if (t1 <= -pi) t1 += 2.0f * pi;
+    if (t0 > pi) t0 -= 2.0f * pi;
+    if (t1 > pi) t1 -= 2.0f * pi;
+    reinterpret_cast<float2*>(th)[i] = make_float2(t0, t1);
+}
+
+int launch_eig3(const float* in, size_t n, int layout, float* evals, float* evec0, float* evec2, unsigned flags,
+                unsigned long long* n_refined, hipStream_t st) {
+    if (n == 0) return TIRA_OK;
+    const unsigned grid = ceil_div(n, kEigTile);
+    const bool vecs = evec0 || evec2;
+    ProfScope prof("eig3", st);
+    if (layout == TIRA_LAYOUT_MAT9) {
+        if (vecs) k_eig3<TIRA_LAYOUT_MAT9, true><<<grid, kEigThreads, 0, st>>>(in, n, evals, evec0, evec2, flags, n_refined);
+        else k_eig3<TIRA_LAYOUT_MAT9, false><<<grid, kEigThreads, 0, st>>>(in, n, evals, evec0, evec2, flags, n_refined);
+    } else if (layout == TIRA_LAYOUT_PACKED6) {
+        if (vecs) k_eig3<TIRA_LAYOUT_PACKED6, true><<<grid, kEigThreads, 0, st>>>(in, n, evals, evec0, evec2, flags, n_refined);
+        else k_eig3<TIRA_LAYOUT_PACKED6, false><<<grid, kEigThreads, 0, st>>>(in, n, evals, evec0, evec2, flags, n_refined);
+    } else {
+        set_error("unknown tensor layout %d", layout);
+        return TIRA_ERR_INVALID_ARG;
+    }
+    TH_HIP(hipGetLastError());
+    return TIRA_OK;
+}
+
+}  // namespace th
+
+using namespace th;
+
+extern "C" {
+
+int tira_evals2_symmetric(const float* mats, size_t n, float* evals, int device) {
+    TH_TRY(use_device(device));
+    TH_ARG(mats && evals, "NULL pointer");
+    if (!n) return TIRA_OK;
+    const float* dm; void* de;
+    TH_TRY(stage_in(mats, n * 16, WS_IN0, (const void**)&dm));
+    TH_TRY(stage_out(evals, n * 8, WS_OUT0, &de));
+    k_evals2<<<ceil_div(n, 256), 256, 0, stream()>>>(dm, n, (float*)de);
+    TH_HIP(hipGetLastError());
+    return finish_out(evals, de, n * 8);
+}
+
+int tira_evecs2polar_symmetric(const float* mats, const float* evals, size_t n, float* thetas, int device) {
+    TH_TRY(use_device(device));
+    TH_ARG(mats && evals && thetas, "NULL pointer");
+    if (!n) return TIRA_OK;
+    const float *dm, *dl; void* dt;
+    TH_TRY(stage_in(mats, n * 16, WS_IN0, (const void**)&dm));
+    TH_TRY(stage_in(evals, n * 8, WS_IN1, (const void**)&dl));
+    TH_TRY(stage_out(thetas, n * 8, WS_OUT0, &dt));
+    k_evecs2polar<<<ceil_div(n, 256), 256, 0, stream()>>>(dm, dl, n, (float*)dt);
+    TH_HIP(hipGetLastError());
+    return finish_out(thetas, dt, n * 8);
+}
+
+int tira_evals3_symmetric(const float* mats, size_t n, float* evals, int device) {
+    TH_TRY(use_device(device));
+    TH_ARG(mats && evals, "NULL pointer");
+    if (!n) return TIRA_OK;
+    const float* dm; void* de;
+    TH_TRY(stage_in(mats, n * 36, WS_IN0, (const void**)&dm));
+    TH_TRY(stage_out(evals, n * 12, WS_OUT0, &de));
+    TH_TRY(launch_eig3(dm, n, TIRA_LAYOUT_MAT9, (float*)de, nullptr, nullptr, TIRA_EIG_REFINE_FP64, nullptr,
+                       stream()));
+    return finish_out(evals, de, n * 12);
+}
+
+int tira_evecs3spherical_symmetric(const float* mats, const float* lambdas, size_t n, float* evecs, int device) {
+    TH_TRY(use_device(device));
+    TH_ARG(mats && lambdas && evecs, "NULL pointer");
+    if (!n) return TIRA_OK;
+    const float *dm, *dl; void* dv;
+    TH_TRY(stage_in(mats, n * 36, WS_IN0, (const void**)&dm));
+    TH_TRY(stage_in(lambdas, n * 12, WS_IN1, (const void**)&dl));
+    TH_TRY(stage_out(evecs, n * 24, WS_OUT0, &dv));
+    k_evecs3spherical<<<ceil_div(n, 256), 256, 0, stream()>>>(dm, dl, n, (float*)dv);
+    TH_HIP(hipGetLastError());
+    return finish_out(evecs, dv, n * 24);
+}
+
+int tira_eig3_async(const float* tensors, size_t n, int layout, float* evals, float* evec0, float* evec2,
+                    unsigned flags, unsigned long long* n_refined_dev) {
+    TH_ARG(tensors && evals, "NULL pointer");
+    return launch_eig3(tensors, n, layout, evals, evec0, evec2, flags, n_refined_dev, stream());
+}
+
+}  // extern "C"

@@ -1,0 +1,732 @@
+// vote3.hip -- 3D tensor voting (stick + numerically integrated plate) on
+// gfx950, replacing tira::cuda::tensorvote3 / _tensorvote3_tk
+// (tensorvote.h:2412-2633, 2741-2873).
+//
+// Pipeline of one call:
+//  1. k_vote3_prep: eigen-decompose every voxel (FP32, FP64 for near-
+//     degenerate tensors) and write a zero-padded voter payload:
+//       stick  float4(e2, copysign(|l2|-|l1|, l2))    (tensorvote.h:2438-2442)
+//       plate  float4(e0, copysign(|l1|-|l0|, l1))    (tensorvote.h:2609-2616)
+//     The padding (r voxels plus tile round-up on every side) makes the vote
+//     loop branch-free: out-of-volume voters carry scale 0 and contribute 0,
+//     which is what the reference's bounds test skips.
+//  2. one fused stick+plate gather kernel over receivers:
+//       * a wave owns 64 consecutive x receivers; each lane keeps BZ
+//         receivers along z in registers and reuses every voter payload load
+//         for the BZ receivers it reaches (register blocking in z);
+//       * the per-offset constants (direction, attenuation, plate
+//         polynomial coefficients, both normalisations folded in) come from
+//         a host-built table read with wave-uniform (scalar) loads;
+//       * the plate vote is either the closed form of the reference's
+//         N-sample integral (exact once N >= p+2, see plate_pair below) or
+//         the literal N-sample sum (any N, the reference's algorithm);
+//       * blocks are numbered XCD-major so neighbouring tiles -- which share
+//         halo voters -- land on the same XCD's L2.
+#include "common.hpp"
+#include "eig_device.hpp"
+#include "vote3.hpp"
+
+#include <cmath>
+#include <vector>
+
+namespace th {
+
+constexpr int kTileX = 64;      // receivers along x per wave
+constexpr int kTileY = 4;       // waves per block (along y)
+constexpr int kThreads = kTileX * kTileY;
+
+// ---------------------------------------------------------------- tables
+static double binom(int n, int k) {
+    double r = 1;
+    for (int i = 1; i <= k; i++) r = r * (n - k + i) / i;
+    return r;
+}
+static double gk(int k) { return binom(2 * k, k) / std::pow(4.0, k); }  // (2k-1)!!/(2k)!!
+
+static double factorial(unsigned n) {
+    double f = 1;
+    for (unsigned i = 1; i <= n; i++) f *= i;
+    return f;
+}
+
+// sticknorm3 (tensorvote.h:1162-1172), evaluated as the reference does (float).
+float sticknorm3_host(float s1, float s2, unsigned p) {
+    const float pi_term = (float)(M_PI * std::sqrt(M_PI) / 2.0);
+    const float s13 = s1 * s1 * s1, s23 = s2 * s2 * s2;
+    const float num1 = (float)(std::pow(2.0, 2 * p + 1) * factorial(p) * factorial(p));
+    const float den1 = (float)factorial(2 * p + 1);
+    const float den2 = (float)(2 * p + 1);
+    return pi_term * ((s13 * num1 / den1) + (s23 * 2.0f / den2));
+}
+
+// Offset direction and attenuations exactly as cpu::build_neighbors3d
+// (tensorvote.h:1411-1436) computes them in float.
+static void offset_geometry(int du, int dv, int dw, float s1, float s2, float d[3], float& c1, float& c2) {
+    const float is1 = (s1 > 0) ? 1.0f / (s1 * s1) : 0.0f;
+    const float is2 = (s2 > 0) ? 1.0f / (s2 * s2) : 0.0f;
+    const float l2 = (float)(du * du + dv * dv + dw * dw);
+    if (l2 == 0.0f) { d[0] = d[1] = d[2] = 0.0f; }
+    else {
+        const float L = std::sqrt(l2);
+        d[0] = (float)du / L; d[1] = (float)dv / L; d[2] = (float)dw / L;
+    }
+    c1 = s1 == 0.0f ? (l2 > 1e-12f ? 0.0f : 1.0f) : std::exp(-l2 * is1);
+    c2 = s2 == 0.0f ? (l2 > 1e-12f ? 0.0f : 1.0f) : std::exp(-l2 * is2);
+}
+
+// Closed form of the plate integral: for a plate voter with unit normal n at
+// unit offset d, s = n.d, x = 1 - s^2,
+//   int_0^pi eta(q.d) r r^T dbeta = alpha(x) I + beta(x) n n^T
+//                                   + s gamma(x) (n d^T + d n^T) + delta(x) d d^T
+// with q on the unit circle orthogonal to n, r = q - 2 (q.d) d and
+// eta = c1 (1-t^2)^p + c2 t^(2p). alpha..delta are polynomials in x whose
+// coefficients follow from the circle moments int cos^(2k) = pi (2k-1)!!/(2k)!!.
+// The N-point Riemann sum of the reference (pi/N sum over beta_i = i pi/N,
+// tensorvote.h:1676-1689) integrates trigonometric polynomials of degree
+// 2p+2 exactly once N >= p+2, so it equals this closed form there.
+static void plate_polys(int P, double c1, double c2, std::vector<double>& al, std::vector<double>& be,
+                        std::vector<double>& ga, std::vector<double>& de) {
+    std::vector<double> e(P + 1), A(P + 3, 0.0), B(P + 3, 0.0), C(P + 3, 0.0), D(P + 3, 0.0);
+    for (int j = 0; j <= P; j++) e[j] = c1 * binom(P, j) * ((j & 1) ? -1.0 : 1.0) + (j == P ? c2 : 0.0);
+    for (int j = 0; j <= P; j++) {
+        A[j] += M_PI * e[j] * (gk(j) - gk(j + 1));
+        if (j >= 1) B[j - 1] += M_PI * e[j] * (2 * gk(j + 1) - gk(j));
+        C[j] += -2 * M_PI * e[j] * gk(j + 1);
+        D[j + 1] += 4 * M_PI * e[j] * gk(j + 1);
+    }
+    al.assign(P + 1, 0.0); be.assign(P + 1, 0.0); ga.assign(P + 1, 0.0); de.assign(P + 2, 0.0);
+    for (int k = 0; k <= P; k++) {
+        al[k] = A[k];
+        be[k] = B[k] - (k > 0 ? B[k - 1] : 0.0) - A[k];   // B (1 - x) - A
+        ga[k] = -(B[k] + C[k]);
+    }
+    for (int k = 0; k <= P + 1; k++) de[k] = 0.5 * (B[k] + 2 * C[k] + D[k]);  // halved: see plate_pair
+}
+
+int closed_stride(int P) { return (4 * P + 13 + 3) & ~3; }
+
+void build_closed_table(int P, int w, float s1, float s2, unsigned power, std::vector<float>& tab) {
+    const int hw = w / 2, stride = closed_stride(P);
+    tab.assign((size_t)w * w * w * stride, 0.0f);
+    const double sn = 1.0 / (double)sticknorm3_host(s1, s2, power);
+    const float snf = 1.0f / sticknorm3_host(s1, s2, power);
+    const double pn = 1.0 / M_PI;
+    std::vector<double> al, be, ga, de;
+    size_t k = 0;
+    for (int dw = -hw; dw <= hw; dw++)
+        for (int dv = -hw; dv <= hw; dv++)
+            for (int du = -hw; du <= hw; du++, k++) {
+                float d[3], c1, c2;
+                offset_geometry(du, dv, dw, s1, s2, d, c1, c2);
+                float* t = &tab[k * stride];
+                t[0] = d[0]; t[1] = d[1]; t[2] = d[2];
+                t[3] = 2 * d[0]; t[4] = 2 * d[1]; t[5] = 2 * d[2];
+                if (P == 1) { t[6] = snf * c1; t[7] = (float)(sn * ((double)c2 - (double)c1)); }
+                else { t[6] = snf * c1; t[7] = snf * c2; }
+                plate_polys(P, c1, c2, al, be, ga, de);
+                int o = 8;
+                for (int i = 0; i <= P; i++) t[o++] = (float)(pn * al[i]);
+                for (int i = 0; i <= P; i++) t[o++] = (float)(pn * be[i]);
+                for (int i = 0; i <= P; i++) t[o++] = (float)(pn * ga[i]);
+                for (int i = 0; i <= P + 1; i++) t[o++] = (float)(pn * de[i]);
+            }
+}
+
+// Literal-sum table: d, stick c1/c2 (sticknorm folded), plate c1/c2.
+void build_sum_table(int w, float s1, float s2, unsigned power, std::vector<float>& tab) {
+    const int hw = w / 2;
+    tab.assign((size_t)w * w * w * kSumStride, 0.0f);
+    const float snf = 1.0f / sticknorm3_host(s1, s2, power);
+    size_t k = 0;
+    for (int dw = -hw; dw <= hw; dw++)
+        for (int dv = -hw; dv <= hw; dv++)
+            for (int du = -hw; du <= hw; du++, k++) {
+                float d[3], c1, c2;
+                offset_geometry(du, dv, dw, s1, s2, d, c1, c2);
+                float* t = &tab[k * kSumStride];
+                t[0] = d[0]; t[1] = d[1]; t[2] = d[2];
+                t[3] = snf * c1; t[4] = snf * c2;
+                t[5] = c1; t[6] = c2; t[7] = 0.0f;
+            }
+}
+
+// ---------------------------------------------------------------- device pair kernels
+struct Acc {
+    float xx, xy, xz, yy, yz, zz, ii;
+};
+
+template <int N>
+__device__ __forceinline__ float ipow(float x) {
+    float r = x;
+#pragma unroll
+    for (int i = 1; i < N; i++) r *= x;
+    return r;
+}
+
+template <int N>  // c[0] + c[1] x + ... + c[N] x^N
+__device__ __forceinline__ float horner(const float* c, float x) {
+    float r = c[N];
+#pragma unroll
+    for (int i = N - 1; i >= 0; i--) r = fmaf(r, x, c[i]);
+    return r;
+}
+
+// Stick vote of voter S = (q, scale) at offset entry e (stickvote3_accumulate_kernel,
+// tensorvote.h:1183-1213, sticknorm folded into e[6..7]).
+template <int P>
+__device__ __forceinline__ void stick_pair(Acc& a, const float4 S, const float* __restrict__ e) {
+    const float d0 = e[0], d1 = e[1], d2 = e[2];
+    const float t = fmaf(S.x, d0, fmaf(S.y, d1, S.z * d2));
+    const float t2 = t * t;
+    float eta;
+    if (P == 1) eta = fmaf(e[7], t2, e[6]);
+    else eta = fmaf(e[6], ipow<P>(1.0f - t2), e[7] * ipow<P>(t2));
+    const float f = eta * S.w;
+    const float m2t = -2.0f * t;
+    const float r0 = fmaf(m2t, d0, S.x), r1 = fmaf(m2t, d1, S.y), r2 = fmaf(m2t, d2, S.z);
+    const float f0 = f * r0, f1 = f * r1, f2 = f * r2;
+    a.xx = fmaf(f0, r0, a.xx); a.xy = fmaf(f0, r1, a.xy); a.xz = fmaf(f0, r2, a.xz);
+    a.yy = fmaf(f1, r1, a.yy); a.yz = fmaf(f1, r2, a.yz); a.zz = fmaf(f2, r2, a.zz);
+}
+
+// Plate vote of voter N = (n, scale) at offset entry e, closed form:
+//   scale * [alpha I + beta n n^T + gamma (n d^T + d n^T) + delta d d^T]
+// with gamma(n d^T + d n^T) + delta d d^T = h d^T + d h^T, h = gamma n + delta/2 d.
+template <int P>
+__device__ __forceinline__ void plate_pair(Acc& a, const float4 N, const float* __restrict__ e) {
+    const float d0 = e[0], d1 = e[1], d2 = e[2];
+    const float s = fmaf(N.x, d0, fmaf(N.y, d1, N.z * d2));
+    const float x = fmaf(-s, s, 1.0f);
+    const float* al = e + 8;
+    const float* be = al + (P + 1);
+    const float* ga = be + (P + 1);
+    const float* de = ga + (P + 1);
+    const float sp = N.w;
+    const float A = sp * horner<P>(al, x);
+    const float B = sp * horner<P>(be, x);
+    const float G = (sp * s) * horner<P>(ga, x);
+    const float D = sp * horner<P + 1>(de, x);
+    a.ii += A;
+    const float b0 = B * N.x, b1 = B * N.y, b2 = B * N.z;
+    a.xx = fmaf(b0, N.x, a.xx); a.xy = fmaf(b0, N.y, a.xy); a.xz = fmaf(b0, N.z, a.xz);
+    a.yy = fmaf(b1, N.y, a.yy); a.yz = fmaf(b1, N.z, a.yz); a.zz = fmaf(b2, N.z, a.zz);
+    const float h0 = fmaf(D, d0, G * N.x), h1 = fmaf(D, d1, G * N.y), h2 = fmaf(D, d2, G * N.z);
+    a.xx = fmaf(h0, e[3], a.xx); a.yy = fmaf(h1, e[4], a.yy); a.zz = fmaf(h2, e[5], a.zz);
+    a.xy = fmaf(h0, d1, fmaf(h1, d0, a.xy));
+    a.xz = fmaf(h0, d2, fmaf(h2, d0, a.xz));
+    a.yz = fmaf(h1, d2, fmaf(h2, d1, a.yz));
+}
+
+// Literal N-sample plate sum (plate3_numerical_device, tensorvote.h:2457-2485):
+// U = (u, scale/N), V = (v, 0) is the voter's plate basis.
+__device__ __forceinline__ void plate_sum_pair(Acc& a, const float4 U, const float4 V, const float* __restrict__ e,
+                                               const float2* __restrict__ cs, int nsamp, int power) {
+    const float d0 = e[0], d1 = e[1], d2 = e[2], c1 = e[5], c2 = e[6];
+    const float k = c2 - c1;
+    for (int i = 0; i < nsamp; i++) {
+        const float2 sc = cs[i];  // (cos beta_i, sin beta_i)
+        const float q0 = fmaf(sc.x, U.x, sc.y * V.x), q1 = fmaf(sc.x, U.y, sc.y * V.y),
+                    q2 = fmaf(sc.x, U.z, sc.y * V.z);
+        const float t = fmaf(q0, d0, fmaf(q1, d1, q2 * d2));
+        const float t2 = t * t;
+        float eta;
+        if (power == 1) eta = fmaf(k, t2, c1);
+        else {
+            const float t1 = 1.0f - t2;
+            float e1 = t1, e2 = t2;
+            for (int j = 1; j < power; j++) { e1 *= t1; e2 *= t2; }
+            eta = fmaf(c1, e1, c2 * e2);
+        }
+        const float f = eta * U.w;
+        const float m2t = -2.0f * t;
+        const float r0 = fmaf(m2t, d0, q0), r1 = fmaf(m2t, d1, q1), r2 = fmaf(m2t, d2, q2);
+        const float f0 = f * r0, f1 = f * r1, f2 = f * r2;
+        a.xx = fmaf(f0, r0, a.xx); a.xy = fmaf(f0, r1, a.xy); a.xz = fmaf(f0, r2, a.xz);
+        a.yy = fmaf(f1, r1, a.yy); a.yz = fmaf(f1, r2, a.yz); a.zz = fmaf(f2, r2, a.zz);
+    }
+}
+
+// Stick pair for the literal path (runtime power).
+__device__ __forceinline__ void stick_pair_rt(Acc& a, const float4 S, const float* __restrict__ e, int power) {
+    const float d0 = e[0], d1 = e[1], d2 = e[2];
+    const float t = fmaf(S.x, d0, fmaf(S.y, d1, S.z * d2));
+    const float t2 = t * t;
+    const float t1 = 1.0f - t2;
+    float e1 = t1, e2 = t2;
+    for (int j = 1; j < power; j++) { e1 *= t1; e2 *= t2; }
+    const float f = fmaf(e[3], e1, e[4] * e2) * S.w;
+    const float m2t = -2.0f * t;
+    const float r0 = fmaf(m2t, d0, S.x), r1 = fmaf(m2t, d1, S.y), r2 = fmaf(m2t, d2, S.z);
+    const float f0 = f * r0, f1 = f * r1, f2 = f * r2;
+    a.xx = fmaf(f0, r0, a.xx); a.xy = fmaf(f0, r1, a.xy); a.xz = fmaf(f0, r2, a.xz);
+    a.yy = fmaf(f1, r1, a.yy); a.yz = fmaf(f1, r2, a.yz); a.zz = fmaf(f2, r2, a.zz);
+}
+
+__device__ __forceinline__ void write_out(float* __restrict__ out, int layout, size_t v, const Acc& a) {
+    const float xx = a.xx + a.ii, yy = a.yy + a.ii, zz = a.zz + a.ii;
+    if (layout == TIRA_LAYOUT_MAT9) {
+        float* o = out + v * 9;
+        o[0] = xx; o[1] = a.xy; o[2] = a.xz;
+        o[3] = a.xy; o[4] = yy; o[5] = a.yz;
+        o[6] = a.xz; o[7] = a.yz; o[8] = zz;
+    } else {
+        float* o = out + v * 6;
+        o[0] = xx; o[1] = a.xy; o[2] = a.xz; o[3] = yy; o[4] = a.yz; o[5] = zz;
+    }
+}
+
+// XCD-major block numbering: hardware hands consecutive workgroup ids to the
+// 8 XCDs round-robin; remap so each XCD walks a contiguous run of tiles.
+__device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nblocks) {
+    constexpr unsigned kXcd = 8;
+    const unsigned per = (nblocks + kXcd - 1) / kXcd;
+    const unsigned xcd = b % kXcd, idx = b / kXcd;
+    const unsigned full = nblocks % kXcd == 0 ? kXcd : nblocks % kXcd;  // XCDs that own `per` blocks
+    const unsigned start = xcd < full ? xcd * per : full * per + (xcd - full) * (per - 1);
+    return start + idx;
+}
+
+__device__ __forceinline__ void tile_coords(const VoteGeom& g, int bz_tile, unsigned& bx, unsigned& by,
+                                            unsigned& bz) {
+    const unsigned nb = g.nbx * g.nby * g.nbz;
+    const unsigned b = xcd_remap(blockIdx.x, nb);
+    bx = b % g.nbx;
+    by = (b / g.nbx) % g.nby;
+    bz = b / (g.nbx * g.nby);
+    (void)bz_tile;
+}
+
+// Closed-form stick + plate gather. Payloads are padded volumes of
+// (P0, P1, P2) voxels; receiver (z, y, x) reads voter (z+dw+r, y+dv+r, x+du+r).
+template <int P, bool STICK, bool PLATE, int BZ>
+__global__ __launch_bounds__(kThreads) void k_vote3_closed(const float4* __restrict__ pay_s,
+                                                           const float4* __restrict__ pay_p,
+                                                           const float* __restrict__ tab, VoteGeom g,
+                                                           float* __restrict__ out, int out_layout) {
+    constexpr int STRIDE = (4 * P + 13 + 3) & ~3;
+    unsigned bx, by, bz;
+    tile_coords(g, BZ, bx, by, bz);
+    const int x = (int)bx * kTileX + (int)(threadIdx.x % kTileX);
+    const int y = (int)by * kTileY + (int)(threadIdx.x / kTileX);
+    const int z0 = (int)bz * BZ;
+    const int r = g.r, w = 2 * g.r + 1;
+    const size_t plane = (size_t)g.P1 * g.P2;
+
+    Acc acc[BZ];
+#pragma unroll
+    for (int j = 0; j < BZ; j++) acc[j] = Acc{0, 0, 0, 0, 0, 0, 0};
+
+    for (int dv = -r; dv <= r; dv++) {
+        for (int du = -r; du <= r; du++) {
+            const size_t col = ((size_t)z0 * g.P1 + (size_t)(y + r + dv)) * g.P2 + (size_t)(x + r + du);
+            const int ebase = (dv + r) * w + (du + r);
+            for (int zz = 0; zz < BZ + 2 * r; zz++) {
+                float4 S, N;
+                if (STICK) S = pay_s[col + (size_t)zz * plane];
+                if (PLATE) N = pay_p[col + (size_t)zz * plane];
+#pragma unroll
+                for (int j = 0; j < BZ; j++) {
+                    const int dw = zz - r - j;
+                    if (dw >= -r && dw <= r) {
+                        const float* e = tab + (size_t)((dw + r) * w * w + ebase) * STRIDE;
+                        if (STICK) stick_pair<P>(acc[j], S, e);
+                        if (PLATE) plate_pair<P>(acc[j], N, e);
+                    }
+                }
+            }
+        }
+    }
+    if (x < g.s2 && y < g.s1) {
+#pragma unroll
+        for (int j = 0; j < BZ; j++) {
+            const int z = z0 + j;
+            if (z < g.s0) write_out(out, out_layout, ((size_t)z * g.s1 + y) * g.s2 + x, acc[j]);
+        }
+    }
+}
+
+// Literal-sum stick + plate gather (any power, any N >= 1).
+template <bool STICK, bool PLATE, int BZ>
+__global__ __launch_bounds__(kThreads) void k_vote3_sum(const float4* __restrict__ pay_s,
+                                                        const float4* __restrict__ pay_u,
+                                                        const float4* __restrict__ pay_v,
+                                                        const float* __restrict__ tab,
+                                                        const float2* __restrict__ cs, int nsamp, int power,
+                                                        VoteGeom g, float* __restrict__ out, int out_layout) {
+    unsigned bx, by, bz;
+    tile_coords(g, BZ, bx, by, bz);
+    const int x = (int)bx * kTileX + (int)(threadIdx.x % kTileX);
+    const int y = (int)by * kTileY + (int)(threadIdx.x / kTileX);
+    const int z0 = (int)bz * BZ;
+    const int r = g.r, w = 2 * g.r + 1;
+    const size_t plane = (size_t)g.P1 * g.P2;
+
+    Acc acc[BZ];
+#pragma unroll
+    for (int j = 0; j < BZ; j++) acc[j] = Acc{0, 0, 0, 0, 0, 0, 0};
+
+    for (int dv = -r; dv <= r; dv++) {
+        for (int du = -r; du <= r; du++) {
+            const size_t col = ((size_t)z0 * g.P1 + (size_t)(y + r + dv)) * g.P2 + (size_t)(x + r + du);
+            const int ebase = (dv + r) * w + (du + r);
+            for (int zz = 0; zz < BZ + 2 * r; zz++) {
+                float4 S, U, V;
+                if (STICK) S = pay_s[col + (size_t)zz * plane];
+                if (PLATE) { U = pay_u[col + (size_t)zz * plane]; V = pay_v[col + (size_t)zz * plane]; }
+#pragma unroll
+                for (int j = 0; j < BZ; j++) {
+                    const int dw = zz - r - j;
+                    if (dw >= -r && dw <= r) {
+                        const float* e = tab + (size_t)((dw + r) * w * w + ebase) * kSumStride;
+                        if (STICK) stick_pair_rt(acc[j], S, e, power);
+                        if (PLATE) plate_sum_pair(acc[j], U, V, e, cs, nsamp, power);
+                    }
+                }
+            }
+        }
+    }
+    if (x < g.s2 && y < g.s1) {
+#pragma unroll
+        for (int j = 0; j < BZ; j++) {
+            const int z = z0 + j;
+            if (z < g.s0) write_out(out, out_layout, ((size_t)z * g.s1 + y) * g.s2 + x, acc[j]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- payload prep
+__device__ __forceinline__ void load_tensor(const float* __restrict__ in, int layout, size_t v, float& a, float& b,
+                                            float& c, float& d, float& e, float& f) {
+    if (layout == TIRA_LAYOUT_MAT9) {
+        const float* m = in + v * 9;
+        a = m[0]; b = m[1]; d = m[2]; c = m[4]; e = m[5]; f = m[8];
+    } else {
+        const float* m = in + v * 6;
+        a = m[0]; b = m[1]; d = m[2]; c = m[3]; e = m[4]; f = m[5];
+    }
+}
+
+// Plate basis of tensorvote.h:1667-1673 / 2468-2470 for a unit normal n.
+__device__ __forceinline__ void plate_basis(const float n[3], float u[3], float v[3]) {
+    float a0 = 0.0f, a1 = 0.0f, a2 = 1.0f;
+    if (!(fabsf(n[2]) < 0.999f)) { a0 = 1.0f; a2 = 0.0f; }
+    u[0] = a1 * n[2] - a2 * n[1];
+    u[1] = a2 * n[0] - a0 * n[2];
+    u[2] = a0 * n[1] - a1 * n[0];
+    const float l2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    const float inv = l2 > 0.0f ? rsqrtf(l2) : 0.0f;
+    u[0] *= inv; u[1] *= inv; u[2] *= inv;
+    v[0] = n[1] * u[2] - n[2] * u[1];
+    v[1] = n[2] * u[0] - n[0] * u[2];
+    v[2] = n[0] * u[1] - n[1] * u[0];
+}
+
+// Writes the voter payload for one voxel given its eigen-decomposition.
+__device__ __forceinline__ void write_payload(const VoteGeom& g, size_t pidx, float l0, float l1, float l2,
+                                              const float* e0, const float* e2, float4* pay_s, float4* pay_p,
+                                              float4* pay_v, float sum_inv_n) {
+    if (pay_s) pay_s[pidx] = make_float4(e2[0], e2[1], e2[2], copysignf(fabsf(l2) - fabsf(l1), l2));
+    if (pay_p) {
+        // The reference skips plate voters whose normal has |n|^2 <= 1e-12 and
+        // normalises the rest (tensorvote.h:2464-2467).
+        float n[3] = {e0[0], e0[1], e0[2]};
+        const float nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+        float sp = copysignf(fabsf(l1) - fabsf(l0), l1);
+        if (!(nn > 1e-12f)) { sp = 0.0f; n[0] = n[1] = n[2] = 0.0f; }
+        else { const float inv = rsqrtf(nn); n[0] *= inv; n[1] *= inv; n[2] *= inv; }
+        if (!pay_v) {
+            pay_p[pidx] = make_float4(n[0], n[1], n[2], sp);
+        } else {
+            float u[3], v[3];
+            plate_basis(n, u, v);
+            pay_p[pidx] = make_float4(u[0], u[1], u[2], sp * sum_inv_n);
+            pay_v[pidx] = make_float4(v[0], v[1], v[2], 0.0f);
+        }
+    }
+}
+
+// Grid-stride over the padded payload volume: voxels outside the field get a
+// zero payload.
+__global__ __launch_bounds__(256) void k_vote3_prep(const float* __restrict__ in, int layout, VoteGeom g,
+                                                    float4* __restrict__ pay_s, float4* __restrict__ pay_p,
+                                                    float4* __restrict__ pay_v, float sum_inv_n) {
+    const size_t total = (size_t)g.P0 * g.P1 * g.P2;
+    const float4 zero = make_float4(0, 0, 0, 0);
+    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < total; p += (size_t)gridDim.x * blockDim.x) {
+        const int px = (int)(p % g.P2), py = (int)((p / g.P2) % g.P1), pz = (int)(p / ((size_t)g.P1 * g.P2));
+        const int x = px - g.r, y = py - g.r, z = pz - g.r;
+        if (x < 0 || y < 0 || z < 0 || x >= g.s2 || y >= g.s1 || z >= g.s0) {
+            if (pay_s) pay_s[p] = zero;
+            if (pay_p) pay_p[p] = zero;
+            if (pay_v) pay_v[p] = zero;
+            continue;
+        }
+        const size_t v = ((size_t)z * g.s1 + y) * g.s2 + x;
+        float a, b, c, d, e, f;
+        load_tensor(in, layout, v, a, b, c, d, e, f);
+        float l0, l1, l2, e0[3], e1[3], e2[3];
+        eig3_vals<float>(a, b, c, d, e, f, 1e-6f, l0, l1, l2);
+        if (eig3_near_degenerate(l0, l1, l2)) {
+            double dl0, dl1, dl2, de0[3], de1[3], de2[3];
+            eig3_vals<double>(a, b, c, d, e, f, 1e-6, dl0, dl1, dl2);
+            eig3_vecs<double>(a, b, c, d, e, f, dl0, dl1, dl2, 1e-6, de0, de1, de2);
+            l0 = (float)dl0; l1 = (float)dl1; l2 = (float)dl2;
+            for (int k = 0; k < 3; k++) { e0[k] = (float)de0[k]; e2[k] = (float)de2[k]; }
+        } else {
+            eig3_vecs<float>(a, b, c, d, e, f, l0, l1, l2, 1e-6f, e0, e1, e2);
+        }
+        write_payload(g, p, l0, l1, l2, e0, e2, pay_s, pay_p, pay_v, sum_inv_n);
+    }
+}
+
+// Payload from caller eigenpairs (_tensorvote3_tk inputs).
+__global__ __launch_bounds__(256) void k_vote3_prep_tk(const float* __restrict__ lam, const float* __restrict__ ev2,
+                                                       const float* __restrict__ ev0, VoteGeom g,
+                                                       float4* __restrict__ pay_s, float4* __restrict__ pay_p,
+                                                       float4* __restrict__ pay_v, float sum_inv_n) {
+    const size_t total = (size_t)g.P0 * g.P1 * g.P2;
+    const float4 zero = make_float4(0, 0, 0, 0);
+    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < total; p += (size_t)gridDim.x * blockDim.x) {
+        const int px = (int)(p % g.P2), py = (int)((p / g.P2) % g.P1), pz = (int)(p / ((size_t)g.P1 * g.P2));
+        const int x = px - g.r, y = py - g.r, z = pz - g.r;
+        if (x < 0 || y < 0 || z < 0 || x >= g.s2 || y >= g.s1 || z >= g.s0) {
+            if (pay_s) pay_s[p] = zero;
+            if (pay_p) pay_p[p] = zero;
+            if (pay_v) pay_v[p] = zero;
+            continue;
+        }
+        const size_t v = ((size_t)z * g.s1 + y) * g.s2 + x;
+        const float z3[3] = {0, 0, 0};
+        const float* e2 = ev2 ? ev2 + v * 3 : z3;
+        const float* e0 = ev0 ? ev0 + v * 3 : z3;
+        write_payload(g, p, lam[v * 3], lam[v * 3 + 1], lam[v * 3 + 2], e0, e2, pay_s, pay_p, pay_v, sum_inv_n);
+    }
+}
+
+__global__ void k_fill_zero(float* __restrict__ out, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = 0.0f;
+}
+
+// ---------------------------------------------------------------- host driver
+VoteGeom make_geom(unsigned s0, unsigned s1, unsigned s2, unsigned w, int bz) {
+    VoteGeom g;
+    g.s0 = (int)s0; g.s1 = (int)s1; g.s2 = (int)s2;
+    g.r = (int)(w / 2);
+    g.nbx = ceil_div(s2, kTileX);
+    g.nby = ceil_div(s1, kTileY);
+    g.nbz = ceil_div(s0, bz);
+    g.P2 = (int)(g.nbx * kTileX) + 2 * g.r;
+    g.P1 = (int)(g.nby * kTileY) + 2 * g.r;
+    g.P0 = (int)(g.nbz * bz) + 2 * g.r;
+    return g;
+}
+
+int resolve_plate_mode(int plate_mode, unsigned power, unsigned samples) {
+    const unsigned p = power < 1 ? 1 : power;
+    if (plate_mode == TIRA_PLATE_AUTO) {
+        if (p <= 4 && (samples == 0 || samples >= p + 2)) return TIRA_PLATE_CLOSED;
+        return TIRA_PLATE_SUM;
+    }
+    if (plate_mode == TIRA_PLATE_CLOSED && (p > 4 || (samples > 0 && samples < p + 2))) return TIRA_PLATE_SUM;
+    return plate_mode;
+}
+
+struct TableKey {
+    int sum;
+    int w;
+    float s1, s2;
+    unsigned power, nsamp;
+    bool operator==(const TableKey& o) const {
+        return sum == o.sum && w == o.w && s1 == o.s1 && s2 == o.s2 && power == o.power && nsamp == o.nsamp;
+    }
+};
+
+struct TableCache {
+    TableKey key{-1, 0, 0, 0, 0, 0};
+    int device = -1;
+    void* ptr = nullptr;
+};
+static thread_local TableCache g_tab;
+
+// Device copy of the offset table for `key` (plus the cos/sin table of the
+// literal sum), rebuilt and uploaded only when the key changes.
+static int table_for(const TableKey& key, int P, hipStream_t st, char** dtab, float2** dcs) {
+    int dev = 0;
+    TH_HIP(hipGetDevice(&dev));
+    std::vector<float> tab;
+    std::vector<float2> cs;
+    size_t tab_bytes, cs_bytes = 0;
+    if (key.sum) {
+        tab_bytes = (size_t)key.w * key.w * key.w * kSumStride * 4;
+        cs_bytes = (size_t)key.nsamp * 8;
+    } else {
+        tab_bytes = (size_t)key.w * key.w * key.w * closed_stride(P) * 4;
+    }
+    const size_t cs_off = (tab_bytes + 255) & ~size_t(255);
+    void* buf = nullptr;
+    TH_TRY(ws_get(WS_TABLE, cs_off + cs_bytes + 256, &buf));
+    *dtab = (char*)buf;
+    *dcs = key.sum ? (float2*)((char*)buf + cs_off) : nullptr;
+    if (g_tab.key == key && g_tab.device == dev && g_tab.ptr == buf) return TIRA_OK;
+    if (key.sum) {
+        build_sum_table(key.w, key.s1, key.s2, key.power, tab);
+        cs.resize(key.nsamp);
+        const float db = (float)M_PI / (float)key.nsamp;  // dbeta of tensorvote.h:2473
+        for (unsigned i = 0; i < key.nsamp; i++) {
+            const float beta = db * (float)i;
+            cs[i] = make_float2(std::cos(beta), std::sin(beta));
+        }
+    } else {
+        build_closed_table(P, key.w, key.s1, key.s2, key.power, tab);
+    }
+    // Synchronous upload from pageable memory: ordered after earlier work on
+    // `st` that may still read the previous table.
+    TH_HIP(hipStreamSynchronize(st));
+    TH_HIP(hipMemcpy(buf, tab.data(), tab_bytes, hipMemcpyHostToDevice));
+    if (key.sum) TH_HIP(hipMemcpy((char*)buf + cs_off, cs.data(), cs_bytes, hipMemcpyHostToDevice));
+    g_tab.key = key;
+    g_tab.device = dev;
+    g_tab.ptr = buf;
+    return TIRA_OK;
+}
+
+constexpr int kBZ = 4;
+
+template <int P>
+static void launch_closed(bool stick, bool plate, unsigned grid, hipStream_t st, const float4* ps, const float4* pp,
+                          const float* tab, const VoteGeom& g, float* out, int layout) {
+    if (stick && plate) k_vote3_closed<P, true, true, kBZ><<<grid, kThreads, 0, st>>>(ps, pp, tab, g, out, layout);
+    else if (stick) k_vote3_closed<P, true, false, kBZ><<<grid, kThreads, 0, st>>>(ps, pp, tab, g, out, layout);
+    else k_vote3_closed<P, false, true, kBZ><<<grid, kThreads, 0, st>>>(ps, pp, tab, g, out, layout);
+}
+
+// Runs the vote on device buffers. `eig_in`: tensors (layout) or NULL when
+// lam/ev2/ev0 eigenpairs are given.
+int vote3_run(const float* tensors, int in_layout, const float* lam, const float* ev2, const float* ev0, float* out,
+              int out_layout, unsigned s0, unsigned s1, unsigned s2, float sigma1, float sigma2, unsigned w,
+              unsigned power, bool stick, bool plate, unsigned samples, int plate_mode, hipStream_t st) {
+    const size_t nvox = (size_t)s0 * s1 * s2;
+    if (nvox == 0) return TIRA_OK;
+    if (!stick && !plate) {
+        const size_t nf = nvox * (out_layout == TIRA_LAYOUT_MAT9 ? 9 : 6);
+        k_fill_zero<<<std::min<size_t>(4096, ceil_div(nf, 256)), 256, 0, st>>>(out, nf);
+        TH_HIP(hipGetLastError());
+        return TIRA_OK;
+    }
+    const unsigned p = power < 1 ? 1 : power;
+    const int mode = plate ? resolve_plate_mode(plate_mode, power, samples) : TIRA_PLATE_CLOSED;
+    if (mode == TIRA_PLATE_MLP) {
+        set_error("plate_mode MLP is served by tira_tensorvote3_mlp");
+        return TIRA_ERR_INVALID_ARG;
+    }
+    const VoteGeom g = make_geom(s0, s1, s2, w, kBZ);
+    const size_t ptotal = (size_t)g.P0 * g.P1 * g.P2;
+    // Payload buffers.
+    float4 *ps = nullptr, *pp = nullptr, *pv = nullptr;
+    if (stick) TH_TRY(ws_get(WS_PAY0, ptotal * 16, (void**)&ps));
+    if (plate) TH_TRY(ws_get(WS_PAY1, ptotal * 16, (void**)&pp));
+    const bool sum = plate && mode == TIRA_PLATE_SUM;
+    const unsigned nsamp = samples == 0 ? p + 2 : samples;  // samples == 0 (analytic) equals the exact integral
+    if (sum) TH_TRY(ws_get(WS_SCRATCH0, ptotal * 16, (void**)&pv));
+    const float inv_n = sum ? 1.0f / (float)nsamp : 1.0f;
+    const unsigned pgrid = (unsigned)std::min<size_t>(ceil_div(ptotal, 256), 65536);
+    {
+        ProfScope prof("vote3_prep", st);
+        if (tensors) k_vote3_prep<<<pgrid, 256, 0, st>>>(tensors, in_layout, g, ps, pp, pv, inv_n);
+        else k_vote3_prep_tk<<<pgrid, 256, 0, st>>>(lam, ev2, ev0, g, ps, pp, pv, inv_n);
+    }
+    TH_HIP(hipGetLastError());
+
+    // Offset table: host-built, uploaded once per (mode, w, sigmas, power, N)
+    // and kept on the device, so repeated calls launch without a host sync.
+    const bool sum_tab = sum;
+    const int P = (int)p;
+    const TableKey key{sum_tab ? 1 : 0, (int)w, sigma1, sigma2, power, sum_tab ? nsamp : 0u};
+    char* dtab = nullptr;
+    float2* dcs = nullptr;
+    TH_TRY(table_for(key, P, st, &dtab, &dcs));
+    const unsigned grid = g.nbx * g.nby * g.nbz;
+    const float* t = (const float*)dtab;
+    ProfScope prof_vote(sum ? "vote3_sum" : "vote3_closed", st);
+    if (sum) {
+        if (stick && plate) k_vote3_sum<true, true, kBZ><<<grid, kThreads, 0, st>>>(ps, pp, pv, t, dcs, nsamp, p, g, out, out_layout);
+        else if (stick) k_vote3_sum<true, false, kBZ><<<grid, kThreads, 0, st>>>(ps, pp, pv, t, dcs, nsamp, p, g, out, out_layout);
+        else k_vote3_sum<false, true, kBZ><<<grid, kThreads, 0, st>>>(ps, pp, pv, t, dcs, nsamp, p, g, out, out_layout);
+    } else {
+        switch (P) {
+            case 1: launch_closed<1>(stick, plate, grid, st, ps, pp, t, g, out, out_layout); break;
+            case 2: launch_closed<2>(stick, plate, grid, st, ps, pp, t, g, out, out_layout); break;
+            case 3: launch_closed<3>(stick, plate, grid, st, ps, pp, t, g, out, out_layout); break;
+            default: launch_closed<4>(stick, plate, grid, st, ps, pp, t, g, out, out_layout); break;
+        }
+    }
+    TH_HIP(hipGetLastError());
+    return TIRA_OK;
+}
+
+static int check_vote_args(unsigned s0, unsigned s1, unsigned s2, unsigned w, float sigma1, float sigma2) {
+    TH_ARG(w >= 1 && (w % 2) == 1, "w must be a positive odd integer (got %u)", w);
+    TH_ARG(sigma1 >= 0.0f && sigma2 >= 0.0f, "sigma must be >= 0");
+    TH_ARG((size_t)s0 * s1 * s2 < ((size_t)1 << 40), "volume too large");
+    TH_ARG(w <= 129, "window %u exceeds the supported maximum (129)", w);
+    (void)s0; (void)s1; (void)s2;
+    return TIRA_OK;
+}
+
+}  // namespace th
+
+using namespace th;
+
+extern "C" {
+
+int tira_tensorvote3(const float* input_field, float* output_field, unsigned s0, unsigned s1, unsigned s2,
+                     float sigma, float sigma2, unsigned w, unsigned power, int device, int stick, int plate,
+                     unsigned samples, int is_atv) {
+    TH_TRY(use_device(device));
+    TH_ARG(input_field && output_field, "NULL pointer");
+    if (is_atv) {
+        set_error("analytical tensor voting (is_atv) is not provided by this library");
+        return TIRA_ERR_UNSUPPORTED;
+    }
+    TH_TRY(check_vote_args(s0, s1, s2, w, sigma, sigma2));
+    const size_t n = (size_t)s0 * s1 * s2;
+    const float* din; void* dout;
+    TH_TRY(stage_in(input_field, n * 36, WS_IN0, (const void**)&din));
+    TH_TRY(stage_out(output_field, n * 36, WS_OUT0, &dout));
+    TH_TRY(vote3_run(din, TIRA_LAYOUT_MAT9, nullptr, nullptr, nullptr, (float*)dout, TIRA_LAYOUT_MAT9, s0, s1, s2,
+                     sigma, sigma2, w, power, stick != 0, plate != 0, samples, TIRA_PLATE_AUTO, stream()));
+    return finish_out(output_field, dout, n * 36);
+}
+
+int tira_tensorvote3_tk(float* output_field, const float* lambdas, const float* evec2, const float* evec0,
+                        unsigned s0, unsigned s1, unsigned s2, float sigma1, float sigma2, unsigned power,
+                        unsigned w, int stick, int plate, unsigned samples) {
+    TH_ARG(output_field && lambdas, "NULL pointer");
+    TH_ARG(!stick || evec2, "stick voting needs evec2");
+    TH_ARG(!plate || evec0, "plate voting needs evec0");
+    TH_TRY(check_vote_args(s0, s1, s2, w, sigma1, sigma2));
+    const size_t n = (size_t)s0 * s1 * s2;
+    const float *dl, *d2 = nullptr, *d0 = nullptr; void* dout;
+    TH_TRY(stage_in(lambdas, n * 12, WS_IN0, (const void**)&dl));
+    if (stick) TH_TRY(stage_in(evec2, n * 12, WS_IN1, (const void**)&d2));
+    if (plate) TH_TRY(stage_in(evec0, n * 12, WS_IN2, (const void**)&d0));
+    TH_TRY(stage_out(output_field, n * 36, WS_OUT0, &dout));
+    TH_TRY(vote3_run(nullptr, 0, dl, d2, d0, (float*)dout, TIRA_LAYOUT_MAT9, s0, s1, s2, sigma1, sigma2, w, power,
+                     stick != 0, plate != 0, samples, TIRA_PLATE_AUTO, stream()));
+    return finish_out(output_field, dout, n * 36);
+}
+
+int tira_tensorvote3_async(const float* input_field, int in_layout, float* output_field, int out_layout,
+                           unsigned s0, unsigned s1, unsigned s2, float sigma1, float sigma2, unsigned w,
+                           unsigned power, int stick, int plate, unsigned samples, int plate_mode) {
+    TH_ARG(input_field && output_field, "NULL pointer");
+    TH_ARG(in_layout == TIRA_LAYOUT_MAT9 || in_layout == TIRA_LAYOUT_PACKED6, "bad input layout");
+    TH_ARG(out_layout == TIRA_LAYOUT_MAT9 || out_layout == TIRA_LAYOUT_PACKED6, "bad output layout");
+    TH_ARG(plate_mode >= TIRA_PLATE_AUTO && plate_mode <= TIRA_PLATE_CLOSED, "bad plate mode %d", plate_mode);
+    TH_TRY(check_vote_args(s0, s1, s2, w, sigma1, sigma2));
+    return vote3_run(input_field, in_layout, nullptr, nullptr, nullptr, output_field, out_layout, s0, s1, s2, sigma1,
+                     sigma2, w, power, stick != 0, plate != 0, samples, plate_mode, stream());
+}
+
+}  // extern "C"
