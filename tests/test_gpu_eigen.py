@@ -42,12 +42,20 @@ def test_evals3_random_vs_fp64(hip, oracle):
 
 
 def test_evals3_vs_reference_fp32(hip, oracle):
+    """Where the reference's own FP32 result is accurate (within 1e-6 of FP64)
+    ours matches it within 2e-6; elsewhere the reference is the one that is off
+    (its absolute 1e-6 diagonal threshold and the trigonometric formula's
+    1/gap error growth) and ours stays within 1e-6 of FP64."""
     t6, _ = synth.eig_tensors(100_000, seed=12)
     m9 = np.concatenate([synth.mat9_from_packed6(t6), _special_mats()])
-    ev = api.evals3_symmetric(m9)
-    ref = oracle.evals3(m9)
-    err = _normwise(ev.astype(np.float64), ref.astype(np.float64))
-    assert err.max() <= 2e-5
+    ev = api.evals3_symmetric(m9).astype(np.float64)
+    ref = oracle.evals3(m9).astype(np.float64)
+    t6all = synth.packed6_from_mat9(m9)
+    ref64 = oracle.eig3_packed6_f64(t6all)
+    ref_ok = _normwise(ref, ref64) <= 1e-6
+    print(f"reference FP32 accurate on {ref_ok.mean():.3f} of the tensors")
+    assert _normwise(ev, ref)[ref_ok].max() <= 2e-6
+    assert _normwise(ev, ref64).max() <= 1e-6
     # exactly diagonal / zero tensors are exact
     sp = _special_mats()
     np.testing.assert_array_equal(api.evals3_symmetric(sp[:4]), oracle.evals3(sp[:4]))
@@ -102,11 +110,20 @@ def test_evals2_evecs2_vs_reference(hip, oracle):
     rng = np.random.default_rng(5)
     m = rng.normal(size=(70_001, 4)).astype(np.float32)
     m[:, 2] = m[:, 1]
-    ev = api.evals2_symmetric(m)
-    ref = oracle.evals2(m)
-    np.testing.assert_allclose(ev, ref, rtol=2e-6, atol=2e-6)
-    th = api.evecs2polar_symmetric(m, ref)
-    thr = oracle.evecs2polar(m, ref)
+    ev = api.evals2_symmetric(m).astype(np.float64)
+    ref = oracle.evals2(m).astype(np.float64)
+    # FP64 truth: mean -+ hypot
+    a, b, c = m[:, 0].astype(np.float64), m[:, 1].astype(np.float64), m[:, 3].astype(np.float64)
+    h = np.hypot(0.5 * (a - c), b)
+    t64 = np.stack([0.5 * (a + c) - h, 0.5 * (a + c) + h], 1)
+    scale = np.maximum(np.abs(t64).max(1), 1e-30)
+    e_gpu = np.abs(ev - t64).max(1) / scale
+    e_ref = np.abs(ref - t64).max(1) / scale
+    print(f"evals2 normwise: GPU {e_gpu.max():.2e}, reference FP32 {e_ref.max():.2e}")
+    assert e_gpu.max() <= 1e-6
+    assert (np.abs(ev - ref).max(1) / scale).max() <= 1e-5
+    th = api.evecs2polar_symmetric(m, ref.astype(np.float32))
+    thr = oracle.evecs2polar(m, ref.astype(np.float32))
     d = np.abs(np.angle(np.exp(1j * (th.astype(np.float64) - thr))))
     gap = (ref[:, 1] - ref[:, 0]) / np.maximum(np.abs(ref).max(1), 1e-30)
     assert np.max(d[gap > 1e-3]) <= 1e-4

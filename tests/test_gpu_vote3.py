@@ -18,10 +18,13 @@ SLOTS = [0, 1, 2, 4, 5, 8]
 NT = 8  # oracle threads
 
 
-def _random_field(shape, seed, sparse=0.0):
+def _random_field(shape, seed, sparse=0.0, lo=0.1, hi=10.0):
+    """Random SPD tensors, 5% near-degenerate. The default eigenvalue range keeps
+    off-diagonal energy far above the reference's absolute 1e-6 "diagonal"
+    threshold (eigen.h:399-406); small-tensor fields are covered separately."""
     rng = np.random.default_rng(seed)
     n = int(np.prod(shape))
-    t6, _ = synth.eig_tensors(n, seed=seed, lo=1e-2, hi=1.0, degenerate_frac=0.05)
+    t6, _ = synth.eig_tensors(n, seed=seed, lo=lo, hi=hi, degenerate_frac=0.05)
     m9 = synth.mat9_from_packed6(t6)
     if sparse:
         m9[rng.random(n) < sparse] = 0.0
@@ -71,7 +74,14 @@ def test_vote3_modes(hip, oracle, power, samples, stick, plate, sigma2):
     f = _random_field((12, 14, 16), seed=31 + power + samples)
     gpu = api.tensorvote3(f, 1.3, sigma2, 7, power, stick=stick, plate=plate, samples=samples)
     ref = oracle.tensorvote3(f, 1.3, sigma2, power, 7, stick=stick, plate=plate, samples=samples, nthreads=NT)
-    _check(gpu, ref)
+    ref64 = oracle.tensorvote3_f64(f, 1.3, sigma2, power, 7, stick=stick, plate=plate, samples=samples,
+                                   nthreads=NT)
+    r32 = ref.reshape(-1, 9)[:, SLOTS]
+    r64 = ref64.reshape(-1, 9)[:, SLOTS]
+    inherent = rel_frob(r32, r64)  # the reference FP32 path's own distance from FP64
+    print(f"reference FP32 vs FP64: {inherent:.3e}")
+    _check(gpu, ref, tol=max(1e-4, 3 * inherent))
+    _check(gpu, ref64, tol=max(1e-4, 3 * inherent))
 
 
 def test_vote3_none(hip):
@@ -80,7 +90,9 @@ def test_vote3_none(hip):
     assert np.all(out == 0)
 
 
-def test_vote3_literal_vs_closed(hip):
+def test_vote3_literal_vs_closed(hip, oracle):
+    """The literal N-sample kernel and the closed form agree with the FP64
+    oracle (and with each other) for N >= p + 2."""
     f = _random_field((16, 16, 16), seed=41)
     d = DeviceArray.from_host(f)
     outs = {}
@@ -88,8 +100,27 @@ def test_vote3_literal_vs_closed(hip):
         o = DeviceArray((16, 16, 16, 9))
         api.tensorvote3_async(d, o, (16, 16, 16), 2.0, 0.0, 13, 1, samples=20, plate_mode=mode)
         _lib.call("tira_hip_synchronize")
-        outs[mode] = o.to_host()
-    assert rel_frob(outs[_lib.PLATE_SUM], outs[_lib.PLATE_CLOSED]) <= 2e-6
+        outs[mode] = o.to_host().reshape(-1, 9)[:, SLOTS]
+    ref64 = oracle.tensorvote3_f64(f, 2.0, 0.0, 1, 13, samples=20, nthreads=NT).reshape(-1, 9)[:, SLOTS]
+    e_sum, e_cl = rel_frob(outs[_lib.PLATE_SUM], ref64), rel_frob(outs[_lib.PLATE_CLOSED], ref64)
+    e_sc = rel_frob(outs[_lib.PLATE_SUM], outs[_lib.PLATE_CLOSED])
+    print(f"sum vs FP64 {e_sum:.3e}, closed vs FP64 {e_cl:.3e}, sum vs closed {e_sc:.3e}")
+    assert e_sum <= 1e-4 and e_cl <= 1e-4 and e_sc <= 1e-4
+
+
+def test_vote3_small_tensors_scale_invariant(hip, oracle):
+    """Tensors with norm ~1e-3: the reference's absolute 1e-6 off-diagonal
+    threshold (eigen.h:399-406) treats many as diagonal and votes with the
+    wrong eigenvectors; this implementation's thresholds are relative, so it
+    tracks the FP64 oracle. Documented deviation from the reference."""
+    f = _random_field((12, 12, 12), seed=77, lo=1e-4, hi=1e-2)
+    gpu = api.tensorvote3(f, 1.5, 0.0, 7, 1, samples=20).reshape(-1, 9)[:, SLOTS]
+    ref = oracle.tensorvote3(f, 1.5, 0.0, 1, 7, samples=20, nthreads=NT).reshape(-1, 9)[:, SLOTS]
+    ref64 = oracle.tensorvote3_f64(f, 1.5, 0.0, 1, 7, samples=20, nthreads=NT).reshape(-1, 9)[:, SLOTS]
+    e_gpu, e_ref = rel_frob(gpu, ref64), rel_frob(ref, ref64)
+    print(f"GPU vs FP64 {e_gpu:.3e}; reference FP32 vs FP64 {e_ref:.3e}")
+    assert e_gpu <= 1e-4
+    assert e_gpu <= e_ref
 
 
 def test_vote3_tk_entry(hip, oracle):

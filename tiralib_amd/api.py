@@ -75,7 +75,8 @@ def evecs3spherical_symmetric(mats, lambdas, device: int = 0):
     return out
 
 
-def eig3(tensors, layout=_lib.LAYOUT_PACKED6, vectors=False, flags=_lib.EIG_REFINE_FP64):
+def eig3(tensors, layout=_lib.LAYOUT_PACKED6, vectors=False,
+         flags=_lib.EIG_REFINE_FP64 | _lib.EIG_SCALE_INVARIANT):
     """Native batched eig3 on device arrays: returns (evals, evec0, evec2)."""
     if not isinstance(tensors, DeviceArray):
         raise ValueError("eig3 works on DeviceArray inputs")

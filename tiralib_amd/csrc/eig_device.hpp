@@ -1,39 +1,47 @@
-// eig_device.hpp -- device eigen-solvers for symmetric 2x2 / 3x3 tensors.
+// eig_device.hpp -- eigen-solvers for symmetric 2x2 / 3x3 tensors (host and
+// device).
 //
 // Same mathematics as TIRA's eigen.h, written for CDNA4:
 //  * eigenvalues of 3x3: Smith's trigonometric closed form (eigen.h:391-446),
-//    with cos(phi + 2pi/3) taken from a single sincos(phi) by angle addition;
+//    with cos(phi + 2pi/3) taken from a single sincos(phi) by angle addition,
+//    then repaired by deflation (eig3_solve): the trigonometric values alone
+//    lose accuracy as 1 / (relative gap of the closest pair) -- up to 2e-4
+//    normwise in FP32 for eigenvalues spread over five decades -- while the
+//    deflated FP32 result stays within ~6e-7 of FP64 normwise;
 //  * eigenvectors of 3x3: cofactor row of the best-separated eigenvalue, the
-//    middle one from the 2x2 problem on its orthogonal complement, the third
-//    by a cross product (eigen.h:217-370, 452-512), canonical sign
-//    (largest |component| positive, eigen.h:525-537);
+//    other two from the 2x2 problem on its orthogonal complement
+//    (eigen.h:217-370, 452-512), canonical sign (largest |component|
+//    positive, eigen.h:525-537);
 //  * 2x2: stable quadratic and the larger-row null vector (eigen.h:89-212).
 // Every routine is a template on the arithmetic type: float carries the bulk
-// of the work, double re-solves near-degenerate tensors (relative eigengap
-// below kRefineGap) so that the cubic-root / trigonometric step and the
-// cross-product step keep their accuracy where FP32 loses it.
+// of the work; double re-solves tensors whose relative eigengap is below
+// kRefineGap, where the eigenvectors of the close pair are ill-conditioned.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#ifndef TH_HD
+#define TH_HD __host__ __device__ __forceinline__
+#endif
 
 namespace th {
 
 // Absolute thresholds of the reference (TIRA_EIGEN_EPSILON, eigen.h:13).
-template <typename T> __device__ __forceinline__ T eig_eps() { return T(1e-6); }
+template <typename T> TH_HD T eig_eps() { return T(1e-6); }
 
 constexpr float kRefineGap = 1e-4f;  // relative eigengap that sends a tensor to FP64
 
-__device__ __forceinline__ float d_acos(float x) { return acosf(x); }
-__device__ __forceinline__ double d_acos(double x) { return acos(x); }
-__device__ __forceinline__ void d_sincos(float x, float* s, float* c) { sincosf(x, s, c); }
-__device__ __forceinline__ void d_sincos(double x, double* s, double* c) { sincos(x, s, c); }
-__device__ __forceinline__ float d_sqrt(float x) { return sqrtf(x); }
-__device__ __forceinline__ double d_sqrt(double x) { return sqrt(x); }
-__device__ __forceinline__ float d_atan2(float y, float x) { return atan2f(y, x); }
-__device__ __forceinline__ double d_atan2(double y, double x) { return atan2(y, x); }
-template <typename T> __device__ __forceinline__ T d_abs(T x) { return x < T(0) ? -x : x; }
+TH_HD float d_acos(float x) { return acosf(x); }
+TH_HD double d_acos(double x) { return acos(x); }
+TH_HD void d_sincos(float x, float* s, float* c) { sincosf(x, s, c); }
+TH_HD void d_sincos(double x, double* s, double* c) { sincos(x, s, c); }
+TH_HD float d_sqrt(float x) { return sqrtf(x); }
+TH_HD double d_sqrt(double x) { return sqrt(x); }
+TH_HD float d_atan2(float y, float x) { return atan2f(y, x); }
+TH_HD double d_atan2(double y, double x) { return atan2(y, x); }
+template <typename T> TH_HD T d_abs(T x) { return x < T(0) ? -x : x; }
 
 template <typename T>
-__device__ __forceinline__ void sort3(T& x, T& y, T& z) {
+TH_HD void sort3(T& x, T& y, T& z) {
     T t;
     if (x > y) { t = x; x = y; y = t; }
     if (y > z) { t = y; y = z; z = t; }
@@ -45,7 +53,7 @@ __device__ __forceinline__ void sort3(T& x, T& y, T& z) {
 // reference uses an absolute 1e-6; scale-invariant callers pass a value
 // relative to the tensor norm).
 template <typename T>
-__device__ __forceinline__ void eig3_vals(T a, T b, T c, T d, T e, T f, T diag_eps, T& l0, T& l1, T& l2) {
+TH_HD void eig3_vals(T a, T b, T c, T d, T e, T f, T diag_eps, T& l0, T& l1, T& l2) {
     const T p1 = b * b + d * d + e * e;
     if (p1 < diag_eps) {
         l0 = a; l1 = c; l2 = f;
@@ -76,21 +84,21 @@ __device__ __forceinline__ void eig3_vals(T a, T b, T c, T d, T e, T f, T diag_e
 // True when the relative gap between adjacent eigenvalues is below kRefineGap
 // (and the tensor is not exactly diagonal-handled / zero).
 template <typename T>
-__device__ __forceinline__ bool eig3_near_degenerate(T l0, T l1, T l2) {
+TH_HD bool eig3_near_degenerate(T l0, T l1, T l2) {
     const T scale = d_abs(l0) > d_abs(l2) ? d_abs(l0) : d_abs(l2);
     const T g = (l1 - l0) < (l2 - l1) ? (l1 - l0) : (l2 - l1);
     return scale > T(0) && g < T(kRefineGap) * scale;
 }
 
 template <typename T>
-__device__ __forceinline__ void cross3(const T* a, const T* b, T* r) {
+TH_HD void cross3(const T* a, const T* b, T* r) {
     r[0] = a[1] * b[2] - a[2] * b[1];
     r[1] = a[2] * b[0] - a[0] * b[2];
     r[2] = a[0] * b[1] - a[1] * b[0];
 }
 
 template <typename T>
-__device__ __forceinline__ void normalize3_eps(T* v) {  // normalize3, eigen.h:56-65
+TH_HD void normalize3_eps(T* v) {  // normalize3, eigen.h:56-65
     const T n = d_sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
     if (n > eig_eps<T>()) {
         const T inv = T(1) / n;
@@ -98,9 +106,25 @@ __device__ __forceinline__ void normalize3_eps(T* v) {  // normalize3, eigen.h:5
     }
 }
 
+// Largest cofactor row of (A - lambda I) (unnormalised); returns its squared norm.
+template <typename T>
+TH_HD T cofactor_row(T a, T b, T c, T d, T e, T f, T lam, T* v) {
+    const T m00 = a - lam, m11 = c - lam, m22 = f - lam;
+    const T c00 = m11 * m22 - e * e, c01 = e * d - b * m22, c02 = b * e - m11 * d;
+    const T c10 = d * e - b * m22, c11 = m00 * m22 - d * d, c12 = b * d - m00 * e;
+    const T c20 = b * e - d * m11, c21 = d * b - m00 * e, c22 = m00 * m11 - b * b;
+    const T r0 = c00 * c00 + c01 * c01 + c02 * c02;
+    const T r1 = c10 * c10 + c11 * c11 + c12 * c12;
+    const T r2 = c20 * c20 + c21 * c21 + c22 * c22;
+    if (r0 >= r1 && r0 >= r2) { v[0] = c00; v[1] = c01; v[2] = c02; return r0; }
+    if (r1 >= r0 && r1 >= r2) { v[0] = c10; v[1] = c11; v[2] = c12; return r1; }
+    v[0] = c20; v[1] = c21; v[2] = c22;
+    return r2;
+}
+
 // Null vector of (A - lambda I) from its largest cofactor row (eigen.h:247-288).
 template <typename T>
-__device__ __forceinline__ void evec_cofactor(T a, T b, T c, T d, T e, T f, T lam, T* v) {
+TH_HD void evec_cofactor(T a, T b, T c, T d, T e, T f, T lam, T* v) {
     const T m00 = a - lam, m11 = c - lam, m22 = f - lam;
     const T c00 = m11 * m22 - e * e, c01 = e * d - b * m22, c02 = b * e - m11 * d;
     const T c10 = d * e - b * m22, c11 = m00 * m22 - d * d, c12 = b * d - m00 * e;
@@ -116,7 +140,7 @@ __device__ __forceinline__ void evec_cofactor(T a, T b, T c, T d, T e, T f, T la
 
 // Eigenvector of lam1 in the orthogonal complement of `ev` (eigen.h:217-245, 290-370).
 template <typename T>
-__device__ __forceinline__ void evec_complement(T a, T b, T c, T d, T e, T f, T lam1, const T* ev, T* out) {
+TH_HD void evec_complement(T a, T b, T c, T d, T e, T f, T lam1, const T* ev, T* out) {
     const T eps = eig_eps<T>();
     const T ax = d_abs(ev[0]), ay = d_abs(ev[1]), az = d_abs(ev[2]);
     T t[3] = {T(0), T(0), T(1)};
@@ -155,7 +179,7 @@ __device__ __forceinline__ void evec_complement(T a, T b, T c, T d, T e, T f, T 
 }
 
 template <typename T>
-__device__ __forceinline__ void canonical_sign(T* v) {  // eigen.h:525-537
+TH_HD void canonical_sign(T* v) {  // eigen.h:525-537
     const T ax = d_abs(v[0]), ay = d_abs(v[1]), az = d_abs(v[2]);
     int k = 0;
     T am = ax;
@@ -168,7 +192,7 @@ __device__ __forceinline__ void canonical_sign(T* v) {  // eigen.h:525-537
 // Eigenvectors e0 (smallest), e1, e2 (largest) of the tensor with eigenvalues
 // l0 <= l1 <= l2 (eigen.h:452-512), canonical signs applied.
 template <typename T>
-__device__ __forceinline__ void eig3_vecs(T a, T b, T c, T d, T e, T f, T l0, T l1, T l2, T diag_eps,
+TH_HD void eig3_vecs(T a, T b, T c, T d, T e, T f, T l0, T l1, T l2, T diag_eps,
                                           T* e0, T* e1, T* e2) {
     const T ud = b * b + d * d + e * e;
     if (ud < diag_eps) {
@@ -210,17 +234,110 @@ __device__ __forceinline__ void eig3_vecs(T a, T b, T c, T d, T e, T f, T l0, T 
     canonical_sign(e2);
 }
 
+// Eigenvalues with normwise accuracy: the trigonometric formula loses
+// accuracy in proportion to 1 / (relative gap) for the two closest
+// eigenvalues (r = det(B)/2 approaches +-1 and acos amplifies its rounding).
+// Deflation repairs that: the best-separated eigenvalue keeps its cofactor
+// eigenvector v, A is projected onto the plane orthogonal to v, and the close
+// pair comes from that symmetric 2x2 problem (perfectly conditioned); the
+// separated eigenvalue is the Rayleigh quotient v.Av. Optional outputs:
+// unit eigenvectors e0 (smallest) and e2 (largest) with canonical sign.
+template <typename T>
+TH_HD void eig3_solve(T a, T b, T c, T d, T e, T f, T diag_eps, T* lam, T* e0, T* e2) {
+    T l0, l1, l2;
+    eig3_vals<T>(a, b, c, d, e, f, diag_eps, l0, l1, l2);
+    const T p1 = b * b + d * d + e * e;
+    const bool diag = p1 < diag_eps;
+    if (diag) {
+        lam[0] = l0; lam[1] = l1; lam[2] = l2;
+        if (e0) {
+            T v0[3], v1[3], v2[3];
+            eig3_vecs<T>(a, b, c, d, e, f, l0, l1, l2, diag_eps, v0, v1, v2);
+            for (int k = 0; k < 3; k++) { e0[k] = v0[k]; e2[k] = v2[k]; }
+        }
+        return;
+    }
+    const bool top = (l2 - l1) >= (l1 - l0);  // the largest eigenvalue is the separated one
+    T v[3];
+    // cofactors scale with the square of the eigenvalue spread: with callers
+    // prescaling |A| into [0.5, 1), a row below 1e-24 carries no direction
+    const T rr = cofactor_row(a, b, c, d, e, f, top ? l2 : l0, v);
+    if (!(rr > T(1e-24))) {
+        // no usable cofactor row (near-isotropic tensor): the trigonometric values stand
+        lam[0] = l0; lam[1] = l1; lam[2] = l2;
+        if (e0) {
+            T v0[3], v1[3], v2[3];
+            eig3_vecs<T>(a, b, c, d, e, f, l0, l1, l2, diag_eps, v0, v1, v2);
+            for (int k = 0; k < 3; k++) { e0[k] = v0[k]; e2[k] = v2[k]; }
+        }
+        return;
+    }
+    {
+        const T iv = T(1) / d_sqrt(rr);
+        v[0] *= iv; v[1] *= iv; v[2] *= iv;
+    }
+    // orthonormal complement {U, W} of v
+    T t[3] = {T(0), T(0), T(0)};
+    {
+        const T ax = d_abs(v[0]), ay = d_abs(v[1]), az = d_abs(v[2]);
+        if (ax <= ay && ax <= az) t[0] = T(1); else if (ay <= az) t[1] = T(1); else t[2] = T(1);
+    }
+    T U[3], W[3];
+    cross3(v, t, U);
+    const T iu = T(1) / d_sqrt(U[0] * U[0] + U[1] * U[1] + U[2] * U[2]);
+    U[0] *= iu; U[1] *= iu; U[2] *= iu;
+    cross3(v, U, W);
+    const T AU0 = a * U[0] + b * U[1] + d * U[2], AU1 = b * U[0] + c * U[1] + e * U[2],
+            AU2 = d * U[0] + e * U[1] + f * U[2];
+    const T AW0 = a * W[0] + b * W[1] + d * W[2], AW1 = b * W[0] + c * W[1] + e * W[2],
+            AW2 = d * W[0] + e * W[1] + f * W[2];
+    const T Av0 = a * v[0] + b * v[1] + d * v[2], Av1 = b * v[0] + c * v[1] + e * v[2],
+            Av2 = d * v[0] + e * v[1] + f * v[2];
+    const T m00 = U[0] * AU0 + U[1] * AU1 + U[2] * AU2;
+    const T m01 = U[0] * AW0 + U[1] * AW1 + U[2] * AW2;
+    const T m11 = W[0] * AW0 + W[1] * AW1 + W[2] * AW2;
+    const T ls = v[0] * Av0 + v[1] * Av1 + v[2] * Av2;
+    const T mean = T(0.5) * (m00 + m11), hd = T(0.5) * (m00 - m11);
+    const T h = d_sqrt(hd * hd + m01 * m01);
+    const T lo = mean - h, hi = mean + h;
+    T x0, x1, x2;
+    if (top) { x0 = lo; x1 = hi; x2 = ls; } else { x0 = ls; x1 = lo; x2 = hi; }
+    // keep the ascending order when rounding crosses the separated value
+    sort3(x0, x1, x2);
+    lam[0] = x0; lam[1] = x1; lam[2] = x2;
+    if (e0) {
+        // eigenvector of the 2x2 problem for eigenvalue `lo` (unit, in the U/W plane)
+        T cx = m01, cy = lo - m00;          // row 0 null vector of (M - lo I)
+        T dx = lo - m11, dy = m01;          // row 1 null vector
+        const T n0 = cx * cx + cy * cy, n1 = dx * dx + dy * dy;
+        T px = n0 >= n1 ? cx : dx, py = n0 >= n1 ? cy : dy;
+        T pn = px * px + py * py;
+        if (!(pn > T(0))) { px = T(1); py = T(0); pn = T(1); }
+        const T ip = T(1) / d_sqrt(pn);
+        px *= ip; py *= ip;
+        T vlo[3], vhi[3];
+        for (int k = 0; k < 3; k++) {
+            vlo[k] = px * U[k] + py * W[k];
+            vhi[k] = -py * U[k] + px * W[k];
+        }
+        const T* s0 = top ? vlo : v;
+        const T* s2 = top ? v : vhi;
+        for (int k = 0; k < 3; k++) { e0[k] = s0[k]; e2[k] = s2[k]; }
+        canonical_sign(e0);
+        canonical_sign(e2);
+    }
+}
+
 // ---------------------------------------------------------------- 2x2
-// eval2_symmetric (eigen.h:118-133) with quad_root (89-98).
-__device__ __forceinline__ void eig2_vals(float a, float b, float c, float& e0, float& e1) {
-    const float tr = a + c, det = a * c - b * b;
-    const float bq = -tr;
-    float disc = bq * bq - 4.0f * det;
-    if (disc < 0.0f) disc = 0.0f;
-    const float sg = (bq < 0.0f) ? -1.0f : 1.0f;
-    const float qq = -0.5f * (bq + sg * sqrtf(disc));
-    float r1 = qq, r0 = (qq == 0.0f) ? 0.0f : det / qq;
-    if (r0 > r1) { float t = r0; r0 = r1; r1 = t; }
+// Eigenvalues of [a b; b c] (eval2_symmetric, eigen.h:118-133) in the stable
+// form its documentation states, mean -+ hypot((a - c)/2, b): the quadratic-
+// root form the reference evaluates (quad_root, eigen.h:89-98) loses sqrt(eps)
+// when the eigenvalues nearly coincide (tr^2 - 4 det cancels). Tiny negatives
+// are clamped as the reference does (eigen.h:127-132).
+TH_HD void eig2_vals(float a, float b, float c, float& e0, float& e1) {
+    const float mean = 0.5f * (a + c);
+    const float h = hypotf(0.5f * (a - c), b);
+    float r0 = mean - h, r1 = mean + h;
     const float ar1 = fabsf(r1);
     const float tol = 1e-6f * (ar1 > 1.0f ? ar1 : 1.0f);
     if (r0 < 0.0f && r0 > -tol) r0 = 0.0f;
@@ -230,7 +347,7 @@ __device__ __forceinline__ void eig2_vals(float a, float b, float c, float& e0, 
 
 // Unit eigenvectors (v for l0, u = perp(v) for l1) with the reference's sign
 // convention (evec2polar_symmetric, eigen.h:149-212) -- cartesian, no angles.
-__device__ __forceinline__ void eig2_vecs(float a, float b, float c, float l0, float& vx, float& vy,
+TH_HD void eig2_vecs(float a, float b, float c, float l0, float& vx, float& vy,
                                           float& ux, float& uy) {
     const float vx0 = -b, vy0 = a - l0, vx1 = c - l0, vy1 = -b;
     const float n0 = vx0 * vx0 + vy0 * vy0, n1 = vx1 * vx1 + vy1 * vy1;

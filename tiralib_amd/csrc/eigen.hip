@@ -100,7 +100,8 @@ __global__ __launch_bounds__(kEigThreads) void k_eig3(const float* __restrict__ 
     const bool scale_inv = (flags & TIRA_EIG_SCALE_INVARIANT) != 0;
     const bool refine = (flags & TIRA_EIG_REFINE_FP64) != 0;
 
-    // ---- FP32 pass
+    // ---- FP32 pass (trigonometric values repaired by deflation, see eig3_solve)
+    const float deps = scale_inv ? 1e-30f : 1e-6f;
 #pragma unroll
     for (int j = 0; j < kEigPerThread; j++) {
         const int i = j * kEigThreads + tid;
@@ -109,22 +110,20 @@ __global__ __launch_bounds__(kEigThreads) void k_eig3(const float* __restrict__ 
         lds_tensor<LAYOUT>(s_in, i, a, b, c, d, e, f);
         int ex = 0;
         if (scale_inv) ex = prescale6(a, b, c, d, e, f);
-        float l0, l1, l2;
-        eig3_vals<float>(a, b, c, d, e, f, 1e-6f, l0, l1, l2);
-        if (refine && eig3_near_degenerate(l0, l1, l2)) {
+        float lam[3], v0[3], v2[3];
+        eig3_solve<float>(a, b, c, d, e, f, deps, lam, VECS ? v0 : nullptr, v2);
+        if (refine && eig3_near_degenerate(lam[0], lam[1], lam[2])) {
             const int slot = atomicAdd(&s_count, 1);
             s_list[slot] = (short)i;
         }
         if (VECS) {
-            float v0[3], v1[3], v2[3];
-            eig3_vecs<float>(a, b, c, d, e, f, l0, l1, l2, 1e-6f, v0, v1, v2);
             const size_t g = base + i;
             if (evec0) { evec0[g * 3 + 0] = v0[0]; evec0[g * 3 + 1] = v0[1]; evec0[g * 3 + 2] = v0[2]; }
             if (evec2) { evec2[g * 3 + 0] = v2[0]; evec2[g * 3 + 1] = v2[1]; evec2[g * 3 + 2] = v2[2]; }
         }
-        s_out[i * 3 + 0] = ldexpf(l0, ex);
-        s_out[i * 3 + 1] = ldexpf(l1, ex);
-        s_out[i * 3 + 2] = ldexpf(l2, ex);
+        s_out[i * 3 + 0] = ldexpf(lam[0], ex);
+        s_out[i * 3 + 1] = ldexpf(lam[1], ex);
+        s_out[i * 3 + 2] = ldexpf(lam[2], ex);
     }
     __syncthreads();
 
@@ -136,14 +135,12 @@ __global__ __launch_bounds__(kEigThreads) void k_eig3(const float* __restrict__ 
         lds_tensor<LAYOUT>(s_in, i, af, bf, cf, df, ef, ff);
         int ex = 0;
         if (scale_inv) ex = prescale6(af, bf, cf, df, ef, ff);
-        double l0, l1, l2;
-        eig3_vals<double>(af, bf, cf, df, ef, ff, 1e-6, l0, l1, l2);
-        s_out[i * 3 + 0] = ldexpf((float)l0, ex);
-        s_out[i * 3 + 1] = ldexpf((float)l1, ex);
-        s_out[i * 3 + 2] = ldexpf((float)l2, ex);
+        double lam[3], v0[3], v2[3];
+        eig3_solve<double>(af, bf, cf, df, ef, ff, scale_inv ? 1e-30 : 1e-6, lam, VECS ? v0 : nullptr, v2);
+        s_out[i * 3 + 0] = ldexpf((float)lam[0], ex);
+        s_out[i * 3 + 1] = ldexpf((float)lam[1], ex);
+        s_out[i * 3 + 2] = ldexpf((float)lam[2], ex);
         if (VECS) {
-            double v0[3], v1[3], v2[3];
-            eig3_vecs<double>(af, bf, cf, df, ef, ff, l0, l1, l2, 1e-6, v0, v1, v2);
             const size_t g = base + i;
             if (evec0) { evec0[g * 3 + 0] = (float)v0[0]; evec0[g * 3 + 1] = (float)v0[1]; evec0[g * 3 + 2] = (float)v0[2]; }
             if (evec2) { evec2[g * 3 + 0] = (float)v2[0]; evec2[g * 3 + 1] = (float)v2[1]; evec2[g * 3 + 2] = (float)v2[2]; }
@@ -275,8 +272,8 @@ int tira_evals3_symmetric(const float* mats, size_t n, float* evals, int device)
     const float* dm; void* de;
     TH_TRY(stage_in(mats, n * 36, WS_IN0, (const void**)&dm));
     TH_TRY(stage_out(evals, n * 12, WS_OUT0, &de));
-    TH_TRY(launch_eig3(dm, n, TIRA_LAYOUT_MAT9, (float*)de, nullptr, nullptr, TIRA_EIG_REFINE_FP64, nullptr,
-                       stream()));
+    TH_TRY(launch_eig3(dm, n, TIRA_LAYOUT_MAT9, (float*)de, nullptr, nullptr,
+                       TIRA_EIG_REFINE_FP64 | TIRA_EIG_SCALE_INVARIANT, nullptr, stream()));
     return finish_out(evals, de, n * 12);
 }
 

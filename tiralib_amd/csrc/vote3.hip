@@ -123,7 +123,16 @@ void build_closed_table(int P, int w, float s1, float s2, unsigned power, std::v
                 t[3] = 2 * d[0]; t[4] = 2 * d[1]; t[5] = 2 * d[2];
                 if (P == 1) { t[6] = snf * c1; t[7] = (float)(sn * ((double)c2 - (double)c1)); }
                 else { t[6] = snf * c1; t[7] = snf * c2; }
-                plate_polys(P, c1, c2, al, be, ga, de);
+                if (du == 0 && dv == 0 && dw == 0) {
+                    // Self offset: t = q.d = 0 for every sample, so the plate is
+                    // c1 int q q^T = c1 (pi/2)(I - n n^T) -- constant polynomials
+                    // (the general form assumes |d| = 1).
+                    al.assign(P + 1, 0.0); be.assign(P + 1, 0.0); ga.assign(P + 1, 0.0); de.assign(P + 2, 0.0);
+                    al[0] = 0.5 * M_PI * c1;
+                    be[0] = -0.5 * M_PI * c1;
+                } else {
+                    plate_polys(P, c1, c2, al, be, ga, de);
+                }
                 int o = 8;
                 for (int i = 0; i <= P; i++) t[o++] = (float)(pn * al[i]);
                 for (int i = 0; i <= P; i++) t[o++] = (float)(pn * be[i]);
@@ -218,11 +227,15 @@ __device__ __forceinline__ void plate_pair(Acc& a, const float4 N, const float* 
 }
 
 // Literal N-sample plate sum (plate3_numerical_device, tensorvote.h:2457-2485):
-// U = (u, scale/N), V = (v, 0) is the voter's plate basis.
+// U = (u, scale/N), V = (v, 0) is the voter's plate basis. Samples are summed
+// into a per-pair tensor first and scaled into the receiver afterwards, as the
+// reference does: two-level summation keeps FP32 error at the level of the
+// closed form instead of growing with N * w^3 terms.
 __device__ __forceinline__ void plate_sum_pair(Acc& a, const float4 U, const float4 V, const float* __restrict__ e,
                                                const float2* __restrict__ cs, int nsamp, int power) {
     const float d0 = e[0], d1 = e[1], d2 = e[2], c1 = e[5], c2 = e[6];
     const float k = c2 - c1;
+    float v00 = 0.0f, v01 = 0.0f, v02 = 0.0f, v11 = 0.0f, v12 = 0.0f, v22 = 0.0f;
     for (int i = 0; i < nsamp; i++) {
         const float2 sc = cs[i];  // (cos beta_i, sin beta_i)
         const float q0 = fmaf(sc.x, U.x, sc.y * V.x), q1 = fmaf(sc.x, U.y, sc.y * V.y),
@@ -237,13 +250,15 @@ __device__ __forceinline__ void plate_sum_pair(Acc& a, const float4 U, const flo
             for (int j = 1; j < power; j++) { e1 *= t1; e2 *= t2; }
             eta = fmaf(c1, e1, c2 * e2);
         }
-        const float f = eta * U.w;
         const float m2t = -2.0f * t;
         const float r0 = fmaf(m2t, d0, q0), r1 = fmaf(m2t, d1, q1), r2 = fmaf(m2t, d2, q2);
-        const float f0 = f * r0, f1 = f * r1, f2 = f * r2;
-        a.xx = fmaf(f0, r0, a.xx); a.xy = fmaf(f0, r1, a.xy); a.xz = fmaf(f0, r2, a.xz);
-        a.yy = fmaf(f1, r1, a.yy); a.yz = fmaf(f1, r2, a.yz); a.zz = fmaf(f2, r2, a.zz);
+        const float f0 = eta * r0, f1 = eta * r1, f2 = eta * r2;
+        v00 = fmaf(f0, r0, v00); v01 = fmaf(f0, r1, v01); v02 = fmaf(f0, r2, v02);
+        v11 = fmaf(f1, r1, v11); v12 = fmaf(f1, r2, v12); v22 = fmaf(f2, r2, v22);
     }
+    const float w = U.w;
+    a.xx = fmaf(w, v00, a.xx); a.xy = fmaf(w, v01, a.xy); a.xz = fmaf(w, v02, a.xz);
+    a.yy = fmaf(w, v11, a.yy); a.yz = fmaf(w, v12, a.yz); a.zz = fmaf(w, v22, a.zz);
 }
 
 // Stick pair for the literal path (runtime power).
@@ -464,17 +479,22 @@ __global__ __launch_bounds__(256) void k_vote3_prep(const float* __restrict__ in
         const size_t v = ((size_t)z * g.s1 + y) * g.s2 + x;
         float a, b, c, d, e, f;
         load_tensor(in, layout, v, a, b, c, d, e, f);
-        float l0, l1, l2, e0[3], e1[3], e2[3];
-        eig3_vals<float>(a, b, c, d, e, f, 1e-6f, l0, l1, l2);
-        if (eig3_near_degenerate(l0, l1, l2)) {
-            double dl0, dl1, dl2, de0[3], de1[3], de2[3];
-            eig3_vals<double>(a, b, c, d, e, f, 1e-6, dl0, dl1, dl2);
-            eig3_vecs<double>(a, b, c, d, e, f, dl0, dl1, dl2, 1e-6, de0, de1, de2);
-            l0 = (float)dl0; l1 = (float)dl1; l2 = (float)dl2;
-            for (int k = 0; k < 3; k++) { e0[k] = (float)de0[k]; e2[k] = (float)de2[k]; }
-        } else {
-            eig3_vecs<float>(a, b, c, d, e, f, l0, l1, l2, 1e-6f, e0, e1, e2);
+        // power-of-two prescale: exact, makes every threshold relative to |A|
+        const float m = fmaxf(fmaxf(fmaxf(fabsf(a), fabsf(b)), fmaxf(fabsf(c), fabsf(d))), fmaxf(fabsf(e), fabsf(f)));
+        int ex = 0;
+        if (m > 0.0f && isfinite(m)) {
+            frexpf(m, &ex);
+            a = ldexpf(a, -ex); b = ldexpf(b, -ex); c = ldexpf(c, -ex);
+            d = ldexpf(d, -ex); e = ldexpf(e, -ex); f = ldexpf(f, -ex);
         }
+        float lam[3], e0[3], e2[3];
+        eig3_solve<float>(a, b, c, d, e, f, 1e-30f, lam, e0, e2);
+        if (eig3_near_degenerate(lam[0], lam[1], lam[2])) {
+            double dl[3], de0[3], de2[3];
+            eig3_solve<double>(a, b, c, d, e, f, 1e-30, dl, de0, de2);
+            for (int k = 0; k < 3; k++) { lam[k] = (float)dl[k]; e0[k] = (float)de0[k]; e2[k] = (float)de2[k]; }
+        }
+        const float l0 = ldexpf(lam[0], ex), l1 = ldexpf(lam[1], ex), l2 = ldexpf(lam[2], ex);
         write_payload(g, p, l0, l1, l2, e0, e2, pay_s, pay_p, pay_v, sum_inv_n);
     }
 }
