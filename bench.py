@@ -127,12 +127,7 @@ def main(argv=None):
     call("tira_hip_profile_end")
     ms = e0.elapsed_ms(e1) / a.steps
 
-    import ctypes as C
-
-    def prof(name):
-        tot, cnt = C.c_double(), C.c_long()
-        _lib.load().tira_hip_profile_get(name.encode(), C.byref(tot), C.byref(cnt))
-        return tot.value / max(cnt.value, 1), cnt.value
+    prof = _lib.profile_get
 
     vote_name = "vote3_sum" if (mode == _lib.PLATE_SUM) else "vote3_closed"
     vote_ms, vote_cnt = prof(vote_name)

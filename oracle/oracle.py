@@ -63,6 +63,7 @@ def lib():
         _sig(L, "oracle_eig3_packed6_f64", [_f32p, _sz, _f64p])
         _sig(L, "oracle_build_neighbors2d", [_i, _f, _f, _u, _u, _i, _i, C.c_void_p], _i)
         _sig(L, "oracle_tk_stick_norm_2d", [_f, _f, _u], _f)
+        _sig(L, "oracle_tk_stickvote_2d", [_f, _f, _f, _f, _u, _f, C.POINTER(_f), C.POINTER(_f), C.POINTER(_f)])
         _sig(L, "oracle_tensorvote2", [_f32p, _f32p, _f, _f, _i, _u, _u, _u, _i, _i, _u, _i])
         _sig(L, "oracle_tensorvote2_tk", [_f32p, _f32p, _f32p, _f, _f, _u, _u, _u, _u, _i, _i, _u,
                                           C.c_void_p, _i, _i])
@@ -205,6 +206,13 @@ def tensorvote2_f64(field, sigma1, sigma2, power, w, stick=True, plate=True, sam
     lib().oracle_tensorvote2_f64(out, f.reshape(H, W, 4), float(sigma1), float(sigma2), int(power), int(w), H, W,
                                  int(stick), int(plate), int(samples), int(nthreads))
     return out
+
+
+def tk_stickvote_2d(rx, ry, sigma1, sigma2, power, theta):
+    a, b, c = _f(), _f(), _f()
+    lib().oracle_tk_stickvote_2d(float(rx), float(ry), float(sigma1), float(sigma2), int(power), float(theta),
+                                 C.byref(a), C.byref(b), C.byref(c))
+    return np.array([a.value, b.value, c.value], np.float32)
 
 
 def tk_plate_numerical_2d(rx, ry, sigma1, sigma2, power, n):

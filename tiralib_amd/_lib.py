@@ -71,6 +71,9 @@ SIGNATURES = {
     "tira_hip_event_record": [_vp, _vp],
     "tira_hip_event_elapsed_ms": [_vp, _vp, C.POINTER(_f)],
     "tira_hip_release_workspace": [],
+    "tira_hip_profile_begin": [],
+    "tira_hip_profile_end": [],
+    "tira_hip_profile_get": [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_long)],
     "tira_evals2_symmetric": [_vp, _sz, _vp, _i],
     "tira_evecs2polar_symmetric": [_vp, _vp, _sz, _vp, _i],
     "tira_evals3_symmetric": [_vp, _sz, _vp, _i],
@@ -133,6 +136,13 @@ def ptr(a) -> int:
     if isinstance(a, DeviceArray):
         return a.ptr
     return a.ctypes.data
+
+
+def profile_get(name: str):
+    """(average ms, launches) of a library kernel between profile_begin/end."""
+    tot, cnt = C.c_double(), C.c_long()
+    call("tira_hip_profile_get", name.encode(), C.byref(tot), C.byref(cnt))
+    return tot.value / max(cnt.value, 1), cnt.value
 
 
 class DeviceArray:
