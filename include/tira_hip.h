@@ -107,6 +107,33 @@ int tira_evecs3spherical_symmetric(const float* mats, const float* lambdas, size
 int tira_eig3_async(const float* tensors, size_t n, int layout, float* evals, float* evec0, float* evec2,
                     unsigned flags, unsigned long long* n_refined_dev);
 
+/* ------------------------------------------------------------------ 2D voting */
+/* 48-byte neighbour record, field order of tira::Neighbor2D (tensorvote.h:776-786):
+ * int32 du, dv; float dx, dy, c1, c2, Ra, Rb, Rc, Va, Vb, Vc. */
+/* tira::cpu::build_neighbors2d(w, sigma1, sigma2, power, samples, stick, plate),
+ * tensorvote.h:1223-1280 (host). Writes up to `capacity` records to `out`
+ * (may be NULL to query), the record count to *count. */
+int tira_build_neighbors2d(int w, float sigma1, float sigma2, unsigned power, unsigned samples, int stick, int plate,
+                           void* out, int capacity, int* count);
+/* tira::cuda::tensorvote<T>(t_out, t_in, sigma1, sigma2, power, w, shape0, shape1,
+ * stick, plate, samples, is_atv), tensorvote.h:2200-2226. t_in / t_out:
+ * shape0 x shape1 x 4 floats [a b b c], shape1 fastest. is_atv != 0 returns
+ * TIRA_ERR_UNSUPPORTED. */
+int tira_tensorvote2(float* t_out, const float* t_in, float sigma1, float sigma2, int power, unsigned w,
+                     unsigned shape0, unsigned shape1, int stick, int plate, unsigned samples, int is_atv);
+/* tira::cuda::tensorvote_prebuilt<T>(t_out, t_in, host_nb, nb_count, sigma1,
+ * sigma2, power, w, shape0, shape1, stick, plate, samples), tensorvote.h:2236-2305.
+ * host_nb: nb_count 48-byte records (host memory). */
+int tira_tensorvote2_prebuilt(float* t_out, const float* t_in, const void* host_nb, int nb_count, float sigma1,
+                              float sigma2, int power, unsigned w, unsigned shape0, unsigned shape1, int stick,
+                              int plate, unsigned samples);
+/* tira::cuda::tensorvote_tk<T>(t_out, lambdas, thetas, sigma1, sigma2, power, w,
+ * s0, s1, stick, plate, samples), tensorvote.h:2125-2179. lambdas / thetas
+ * s0 x s1 x 2 (ascending eigenvalues, eigenvector angles). */
+int tira_tensorvote2_tk(float* t_out, const float* lambdas, const float* thetas, float sigma1, float sigma2,
+                        unsigned power, unsigned w, unsigned s0, unsigned s1, int stick, int plate,
+                        unsigned samples);
+
 /* ------------------------------------------------------------------ 3D voting */
 /* tira::cuda::tensorvote3(input_field, output_field, s0, s1, s2, sigma, sigma2, w,
  * power, device, stick, plate, samples, is_atv), tensorvote.h:2816-2873.

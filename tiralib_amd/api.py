@@ -138,3 +138,63 @@ def tensorvote3_async(field, out, shape, sigma1, sigma2, w, power, stick=True, p
     call("tira_tensorvote3_async", field.ptr, int(in_layout), out.ptr, int(out_layout), s0, s1, s2,
          float(sigma1), float(sigma2), int(w), int(power), int(stick), int(plate), int(samples), int(plate_mode))
     return out
+
+
+# ------------------------------------------------------------------ 2D voting
+NEIGHBOR2D_DTYPE = np.dtype([
+    ("du", np.int32), ("dv", np.int32),
+    ("dx", np.float32), ("dy", np.float32), ("c1", np.float32), ("c2", np.float32),
+    ("Ra", np.float32), ("Rb", np.float32), ("Rc", np.float32),
+    ("Va", np.float32), ("Vb", np.float32), ("Vc", np.float32),
+])
+
+
+def build_neighbors2d(w, sigma1, sigma2, power, samples, stick=True, plate=True):
+    """tira::cpu::build_neighbors2d (tensorvote.h:1223-1280): structured array of
+    48-byte Neighbor2D records (dv outer, du inner)."""
+    cnt = C.c_int()
+    call("tira_build_neighbors2d", int(w), float(sigma1), float(sigma2), int(power), int(samples), int(stick),
+         int(plate), None, 0, C.byref(cnt))
+    nb = np.zeros(cnt.value, NEIGHBOR2D_DTYPE)
+    call("tira_build_neighbors2d", int(w), float(sigma1), float(sigma2), int(power), int(samples), int(stick),
+         int(plate), nb.ctypes.data if cnt.value else None, cnt.value, C.byref(cnt))
+    return nb
+
+
+def _field2_shape(field):
+    if field.ndim != 3 or field.shape[2] != 4:
+        raise ValueError("2D field must have shape (H, W, 4)")
+    return field.shape[0], field.shape[1]
+
+
+def tensorvote(t_in, sigma1, sigma2=0.0, power=1, w=0, stick=True, plate=True, samples=10, is_atv=False):
+    """tira::cuda::tensorvote (2D, tensorvote.h:2200-2226) on an (H, W, 4) field."""
+    f = _f32(t_in)
+    H, W = _field2_shape(f)
+    if w <= 0:
+        w = 2 * int(3.0 * max(sigma1, sigma2)) + 1
+    out = _like(f, (H, W, 4))
+    call("tira_tensorvote2", ptr(out), ptr(f), float(sigma1), float(sigma2), int(power), int(w), H, W, int(stick),
+         int(plate), int(samples), int(is_atv))
+    return out
+
+
+def tensorvote_prebuilt(t_in, nb, sigma1, sigma2, power, w, stick=True, plate=True, samples=10):
+    """tira::cuda::tensorvote_prebuilt (tensorvote.h:2236-2305) with a host Neighbor2D table."""
+    f = _f32(t_in)
+    H, W = _field2_shape(f)
+    nb = np.ascontiguousarray(nb)
+    out = _like(f, (H, W, 4))
+    call("tira_tensorvote2_prebuilt", ptr(out), ptr(f), nb.ctypes.data if nb.size else None, int(nb.size),
+         float(sigma1), float(sigma2), int(power), int(w), H, W, int(stick), int(plate), int(samples))
+    return out
+
+
+def tensorvote_tk(lambdas, thetas, shape, sigma1, sigma2, power, w, stick=True, plate=True, samples=10):
+    """tira::cuda::tensorvote_tk (2D, tensorvote.h:2125-2179) from eigenpairs (H, W, 2) each."""
+    H, W = shape
+    l, t = _f32(lambdas), _f32(thetas)
+    out = _like(l, (H, W, 4))
+    call("tira_tensorvote2_tk", ptr(out), ptr(l), ptr(t), float(sigma1), float(sigma2), int(power), int(w), H, W,
+         int(stick), int(plate), int(samples))
+    return out
