@@ -154,6 +154,44 @@ int tira_tensorvote3_async(const float* input_field, int in_layout, float* outpu
                            unsigned s0, unsigned s1, unsigned s2, float sigma1, float sigma2, unsigned w,
                            unsigned power, int stick, int plate, unsigned samples, int plate_mode);
 
+/* ------------------------------------------------------------------ Gaussian filter */
+/* tira::cuda::gaussian_filter3d<float>(src, width, height, depth, sigma_w, sigma_h,
+ * sigma_d, out_width, out_height, out_depth), filter.h:400-534. Valid-mode
+ * separable Gaussian of a scalar volume (width fastest); kernel lengths
+ * ceil(6 sigma) forced odd (at most 17). dst == NULL only reports the output
+ * size; otherwise dst holds out_width*out_height*out_depth floats. */
+int tira_gaussian_filter3d(const float* src, unsigned width, unsigned height, unsigned depth, float sigma_w,
+                           float sigma_h, float sigma_d, float* dst, unsigned* out_width, unsigned* out_height,
+                           unsigned* out_depth);
+/* tira::cuda::gaussian_convolve2<float>(source, in_sx, in_sy, sigma1, sigma2,
+ * out_width, out_height), filter.h:304-397. Kernel lengths (unsigned)(6 sigma). */
+int tira_gaussian_convolve2(const float* src, unsigned in_sx, unsigned in_sy, float sigma1, float sigma2, float* dst,
+                            unsigned* out_width, unsigned* out_height);
+/* tira::cpu::gaussian1d(ksize, 0, sigma, 1), filter.h:37-64 (host): ksize taps. */
+int tira_gaussian_taps(unsigned ksize, float sigma, float* out);
+/* Native fused separable filter of an s0 x s1 x s2 field with `components`
+ * (1 or 6, packed) floats per voxel, the same odd-length kernel on all axes;
+ * same != 0: zero-padded output of the input size, else valid mode. Device
+ * pointers, non-blocking. No reference counterpart beyond filter.h. */
+int tira_gaussian_filter3_tensor_async(const float* in, float* out, unsigned s0, unsigned s1, unsigned s2,
+                                       unsigned components, const float* taps, unsigned ntaps, int same);
+
+/* ------------------------------------------------------------------ batched volumes */
+/* Many uneven 6-component volumes in one persistent launch (no reference
+ * counterpart): for every volume i, out[i] = zero-padded separable Gaussian of
+ * in[i] (the same odd kernel of ntaps taps, 7..13, on all three axes) and
+ * evals[i] = ascending eigenvalues of every input tensor. shapes: nvol x 3
+ * (s0, s1, s2), all pointers device memory. autotune != 0 times the candidate
+ * z-chunks / occupancies on this batch once per process and batch shape. The
+ * launch is captured in a hipGraph at creation; run() replays it on the
+ * library stream (non-blocking). Buffers must stay valid until destroy(). */
+int tira_batch_gauss_eig3_create(void** handle, int nvol, const float* const* in, float* const* out,
+                                 float* const* evals, const unsigned* shapes, const float* taps, unsigned ntaps,
+                                 int autotune);
+int tira_batch_gauss_eig3_run(void* handle);
+int tira_batch_gauss_eig3_plan(void* handle, int* zchunk, int* blocks_per_cu, float* tuned_ms, unsigned* items);
+int tira_batch_gauss_eig3_destroy(void* handle);
+
 #ifdef __cplusplus
 }
 #endif

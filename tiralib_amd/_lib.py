@@ -86,6 +86,14 @@ SIGNATURES = {
     "tira_tensorvote3": [_vp, _vp, _u, _u, _u, _f, _f, _u, _u, _i, _i, _i, _u, _i],
     "tira_tensorvote3_tk": [_vp, _vp, _vp, _vp, _u, _u, _u, _f, _f, _u, _u, _i, _i, _u],
     "tira_tensorvote3_async": [_vp, _i, _vp, _i, _u, _u, _u, _f, _f, _u, _u, _i, _i, _u, _i],
+    "tira_gaussian_filter3d": [_vp, _u, _u, _u, _f, _f, _f, _vp, C.POINTER(_u), C.POINTER(_u), C.POINTER(_u)],
+    "tira_gaussian_convolve2": [_vp, _u, _u, _f, _f, _vp, C.POINTER(_u), C.POINTER(_u)],
+    "tira_gaussian_taps": [_u, _f, _vp],
+    "tira_gaussian_filter3_tensor_async": [_vp, _vp, _u, _u, _u, _u, _vp, _u, _i],
+    "tira_batch_gauss_eig3_create": [C.POINTER(_vp), _i, _vp, _vp, _vp, _vp, _vp, _u, _i],
+    "tira_batch_gauss_eig3_run": [_vp],
+    "tira_batch_gauss_eig3_plan": [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_f), C.POINTER(_u)],
+    "tira_batch_gauss_eig3_destroy": [_vp],
 }
 
 _lib = None

@@ -198,3 +198,99 @@ def tensorvote_tk(lambdas, thetas, shape, sigma1, sigma2, power, w, stick=True, 
     call("tira_tensorvote2_tk", ptr(out), ptr(l), ptr(t), float(sigma1), float(sigma2), int(power), int(w), H, W,
          int(stick), int(plate), int(samples))
     return out
+
+
+# ------------------------------------------------------------------ Gaussian filter
+def gaussian_taps(ksize, sigma):
+    """tira::cpu::gaussian1d(ksize, 0, sigma, 1) (filter.h:37-64)."""
+    out = np.empty(int(ksize), np.float32)
+    call("tira_gaussian_taps", int(ksize), float(sigma), out.ctypes.data)
+    return out
+
+
+def gaussian_filter3d(src, sigma_w, sigma_h, sigma_d):
+    """tira::cuda::gaussian_filter3d (filter.h:400-534): valid-mode, src (depth, height, width)."""
+    s = _f32(src)
+    if len(s.shape) != 3:
+        raise ValueError("volume must be 3D (depth, height, width)")
+    d, h, w = s.shape
+    ow, oh, od = C.c_uint(), C.c_uint(), C.c_uint()
+    call("tira_gaussian_filter3d", ptr(s), w, h, d, float(sigma_w), float(sigma_h), float(sigma_d), None,
+         C.byref(ow), C.byref(oh), C.byref(od))
+    out = _like(s, (od.value, oh.value, ow.value))
+    call("tira_gaussian_filter3d", ptr(s), w, h, d, float(sigma_w), float(sigma_h), float(sigma_d), ptr(out),
+         C.byref(ow), C.byref(oh), C.byref(od))
+    return out
+
+
+def gaussian_convolve2(src, sigma1, sigma2):
+    """tira::cuda::gaussian_convolve2 (filter.h:304-397): valid-mode, src (height, width)."""
+    s = _f32(src)
+    if len(s.shape) != 2:
+        raise ValueError("image must be 2D (height, width)")
+    h, w = s.shape
+    ow, oh = C.c_uint(), C.c_uint()
+    call("tira_gaussian_convolve2", ptr(s), w, h, float(sigma1), float(sigma2), None, C.byref(ow), C.byref(oh))
+    out = _like(s, (oh.value, ow.value))
+    call("tira_gaussian_convolve2", ptr(s), w, h, float(sigma1), float(sigma2), ptr(out), C.byref(ow), C.byref(oh))
+    return out
+
+
+def gaussian_filter3_tensor(field, taps, same=True):
+    """Native fused separable filter of a (s0, s1, s2, C) device field, C in {1, 6}."""
+    if not isinstance(field, DeviceArray):
+        raise ValueError("gaussian_filter3_tensor works on DeviceArray inputs")
+    s0, s1, s2 = field.shape[:3]
+    comp = field.shape[3] if len(field.shape) == 4 else 1
+    k = np.ascontiguousarray(taps, np.float32)
+    if same:
+        out = DeviceArray(field.shape)
+    else:
+        K = k.shape[0]
+        out = DeviceArray((s0 - K + 1, s1 - K + 1, s2 - K + 1) + ((comp,) if comp > 1 else ()))
+    call("tira_gaussian_filter3_tensor_async", field.ptr, out.ptr, s0, s1, s2, comp, k.ctypes.data, k.shape[0],
+         int(same))
+    call("tira_hip_synchronize")
+    return out
+
+
+# ------------------------------------------------------------------ batched volumes
+class BatchGaussEig3:
+    """Persistent, graph-captured Gaussian smoothing + eig3 of many uneven
+    6-component device volumes (tira_batch_gauss_eig3_*)."""
+
+    def __init__(self, volumes, taps, autotune=True):
+        self.volumes = list(volumes)
+        self.out = [DeviceArray(v.shape) for v in self.volumes]
+        self.evals = [DeviceArray(v.shape[:3] + (3,)) for v in self.volumes]
+        n = len(self.volumes)
+        P = C.c_void_p * n
+        self._in = P(*[v.ptr for v in self.volumes])
+        self._out = P(*[o.ptr for o in self.out])
+        self._ev = P(*[e.ptr for e in self.evals])
+        self._shapes = np.ascontiguousarray([v.shape[:3] for v in self.volumes], np.uint32)
+        self._taps = np.ascontiguousarray(taps, np.float32)
+        h = C.c_void_p()
+        call("tira_batch_gauss_eig3_create", C.byref(h), n, C.cast(self._in, C.c_void_p), C.cast(self._out, C.c_void_p),
+             C.cast(self._ev, C.c_void_p), self._shapes.ctypes.data, self._taps.ctypes.data, self._taps.shape[0],
+             int(autotune))
+        self.handle = h.value
+
+    def run(self):
+        call("tira_batch_gauss_eig3_run", self.handle)
+
+    def plan(self):
+        zc, occ, ms, items = C.c_int(), C.c_int(), C.c_float(), C.c_uint()
+        call("tira_batch_gauss_eig3_plan", self.handle, C.byref(zc), C.byref(occ), C.byref(ms), C.byref(items))
+        return {"zchunk": zc.value, "blocks_per_cu": occ.value, "tuned_ms": ms.value, "items": items.value}
+
+    def close(self):
+        if self.handle:
+            call("tira_batch_gauss_eig3_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -234,21 +234,61 @@ TH_HD void eig3_vecs(T a, T b, T c, T d, T e, T f, T l0, T l1, T l2, T diag_eps,
     canonical_sign(e2);
 }
 
-// Eigenvalues with normwise accuracy: the trigonometric formula loses
-// accuracy in proportion to 1 / (relative gap) for the two closest
-// eigenvalues (r = det(B)/2 approaches +-1 and acos amplifies its rounding).
-// Deflation repairs that: the best-separated eigenvalue keeps its cofactor
-// eigenvector v, A is projected onto the plane orthogonal to v, and the close
-// pair comes from that symmetric 2x2 problem (perfectly conditioned); the
-// separated eigenvalue is the Rayleigh quotient v.Av. Optional outputs:
-// unit eigenvectors e0 (smallest) and e2 (largest) with canonical sign.
+// Fast reciprocal / reciprocal square root: hardware approximations in FP32
+// device code (used only inside self-correcting or normalising steps), exact
+// division elsewhere.
+TH_HD float d_rcp(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+TH_HD double d_rcp(double x) { return 1.0 / x; }
+TH_HD float d_rsqrt(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_rsqf(x);
+#else
+    return 1.0f / sqrtf(x);
+#endif
+}
+TH_HD double d_rsqrt(double x) { return 1.0 / sqrt(x); }
+
+// Largest root of beta^3 - 3 beta - 2 r = 0 for r in [0, 1], i.e.
+// beta = 2 cos(acos(r) / 3) in [sqrt 3, 2]: a cubic fit (max error 1.3e-4)
+// refined by Newton (derivative >= 6 on the interval, so one step reaches
+// FP32 accuracy and two reach FP64 accuracy).
+template <typename T>
+TH_HD T cubic_top_root(T r) {
+    T b = ((T(0.01874897) * r - T(0.08126101)) * r + T(0.33042943)) * r + T(1.73218504);
+    constexpr int kIt = sizeof(T) == 4 ? 1 : 2;
+#pragma unroll
+    for (int i = 0; i < kIt; i++) {
+        const T b2 = b * b;
+        const T f = (b2 - T(3)) * b - T(2) * r;
+        b -= f * d_rcp(T(3) * (b2 - T(1)));
+    }
+    return b;
+}
+
+// Eigenvalues with normwise accuracy. Smith's closed form (eigen.h:391-446)
+// writes the eigenvalues as q + p beta_k, beta_k the roots of
+// beta^3 - 3 beta - 2 r = 0 with r = det((A - qI)/p) / 2; its trigonometric
+// evaluation loses accuracy as 1 / (relative gap of the closest pair), up to
+// 2e-4 normwise in FP32 for eigenvalues spread over five decades. Here only
+// the best-separated root is taken from the cubic (largest when r >= 0,
+// smallest otherwise -- the reference's halfDet test, eigen.h:497); its
+// cofactor eigenvector v (eigen.h:247-288) deflates A to the plane orthogonal
+// to v, where the close pair is a symmetric 2x2 problem (perfectly
+// conditioned), and the separated eigenvalue becomes the Rayleigh quotient
+// v.Av. Optional outputs: unit eigenvectors e0 (smallest) and e2 (largest)
+// with canonical sign.
 template <typename T>
 TH_HD void eig3_solve(T a, T b, T c, T d, T e, T f, T diag_eps, T* lam, T* e0, T* e2) {
-    T l0, l1, l2;
-    eig3_vals<T>(a, b, c, d, e, f, diag_eps, l0, l1, l2);
     const T p1 = b * b + d * d + e * e;
-    const bool diag = p1 < diag_eps;
-    if (diag) {
+    if (p1 < diag_eps) {
+        T l0 = a, l1 = c, l2 = f;
+        sort3(l0, l1, l2);
         lam[0] = l0; lam[1] = l1; lam[2] = l2;
         if (e0) {
             T v0[3], v1[3], v2[3];
@@ -257,23 +297,37 @@ TH_HD void eig3_solve(T a, T b, T c, T d, T e, T f, T diag_eps, T* lam, T* e0, T
         }
         return;
     }
-    const bool top = (l2 - l1) >= (l1 - l0);  // the largest eigenvalue is the separated one
+    const T q = (a + c + f) * T(1.0 / 3.0);
+    const T aq = a - q, cq = c - q, fq = f - q;
+    const T p = d_sqrt((aq * aq + cq * cq + fq * fq + T(2) * p1) * T(1.0 / 6.0));
+    const T pinv = d_rcp(p);
+    const T det = aq * (cq * fq - e * e) - b * (b * fq - e * d) + d * (b * e - cq * d);
+    T r = det * (pinv * pinv * pinv) * T(0.5);
+    r = r > T(1) ? T(1) : (r < T(-1) ? T(-1) : r);
+    const bool top = r >= T(0);
+    const T beta = cubic_top_root<T>(top ? r : -r);
+    const T lsep = top ? q + p * beta : q - p * beta;
     T v[3];
     // cofactors scale with the square of the eigenvalue spread: with callers
     // prescaling |A| into [0.5, 1), a row below 1e-24 carries no direction
-    const T rr = cofactor_row(a, b, c, d, e, f, top ? l2 : l0, v);
+    const T rr = cofactor_row(a, b, c, d, e, f, lsep, v);
     if (!(rr > T(1e-24))) {
-        // no usable cofactor row (near-isotropic tensor): the trigonometric values stand
-        lam[0] = l0; lam[1] = l1; lam[2] = l2;
+        // near-isotropic: the other two roots by Vieta (sum 0, pairwise products -3)
+        const T h = T(0.5) * d_sqrt(fmax(T(12) - T(3) * beta * beta, T(0)));
+        T x0, x1, x2;
+        if (top) { x2 = lsep; x0 = q + p * (T(-0.5) * beta - h); x1 = q + p * (T(-0.5) * beta + h); }
+        else { x0 = lsep; x1 = q + p * (T(0.5) * beta - h); x2 = q + p * (T(0.5) * beta + h); }
+        sort3(x0, x1, x2);
+        lam[0] = x0; lam[1] = x1; lam[2] = x2;
         if (e0) {
             T v0[3], v1[3], v2[3];
-            eig3_vecs<T>(a, b, c, d, e, f, l0, l1, l2, diag_eps, v0, v1, v2);
+            eig3_vecs<T>(a, b, c, d, e, f, x0, x1, x2, diag_eps, v0, v1, v2);
             for (int k = 0; k < 3; k++) { e0[k] = v0[k]; e2[k] = v2[k]; }
         }
         return;
     }
     {
-        const T iv = T(1) / d_sqrt(rr);
+        const T iv = d_rsqrt(rr);
         v[0] *= iv; v[1] *= iv; v[2] *= iv;
     }
     // orthonormal complement {U, W} of v
@@ -284,7 +338,7 @@ TH_HD void eig3_solve(T a, T b, T c, T d, T e, T f, T diag_eps, T* lam, T* e0, T
     }
     T U[3], W[3];
     cross3(v, t, U);
-    const T iu = T(1) / d_sqrt(U[0] * U[0] + U[1] * U[1] + U[2] * U[2]);
+    const T iu = d_rsqrt(U[0] * U[0] + U[1] * U[1] + U[2] * U[2]);
     U[0] *= iu; U[1] *= iu; U[2] *= iu;
     cross3(v, U, W);
     const T AU0 = a * U[0] + b * U[1] + d * U[2], AU1 = b * U[0] + c * U[1] + e * U[2],
@@ -313,7 +367,7 @@ TH_HD void eig3_solve(T a, T b, T c, T d, T e, T f, T diag_eps, T* lam, T* e0, T
         T px = n0 >= n1 ? cx : dx, py = n0 >= n1 ? cy : dy;
         T pn = px * px + py * py;
         if (!(pn > T(0))) { px = T(1); py = T(0); pn = T(1); }
-        const T ip = T(1) / d_sqrt(pn);
+        const T ip = d_rsqrt(pn);
         px *= ip; py *= ip;
         T vlo[3], vhi[3];
         for (int k = 0; k < 3; k++) {

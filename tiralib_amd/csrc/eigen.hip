@@ -4,7 +4,7 @@
 //
 // eig3 kernel design (HBM-bound: 36 B per tensor for packed-6 in / 3 floats
 // out, 48 B in the reference 9-float layout):
-//  * a 256-thread block owns 1024 consecutive tensors; the AoS input is read
+//  * a 256-thread block owns 512 consecutive tensors; the AoS input is read
 //    with fully coalesced 16-byte loads into LDS and each lane then picks its
 //    tensors out of LDS (the 9-float stride is odd, so no bank conflicts);
 //  * FP32 trigonometric solve for every tensor;
@@ -25,7 +25,7 @@ namespace th {
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kEigThreads = 256;
-constexpr int kEigPerThread = 4;
+constexpr int kEigPerThread = 2;
 constexpr int kEigTile = kEigThreads * kEigPerThread;  // tensors per block
 
 // Power-of-two prescale so that the largest |entry| lies in [0.5, 1): exact
@@ -89,9 +89,10 @@ __global__ __launch_bounds__(kEigThreads) void k_eig3(const float* __restrict__ 
     if (vec_ok) {
         const f32x4* g4 = reinterpret_cast<const f32x4*>(gin);
         f32x4* s4 = reinterpret_cast<f32x4*>(s_in);
+        static_assert((kEigTile * W) % 4 == 0, "tile must be a whole number of float4");
 #pragma unroll
-        for (int k = 0; k < (kEigTile * W) / (4 * kEigThreads); k++)
-            s4[k * kEigThreads + tid] = __builtin_nontemporal_load(&g4[k * kEigThreads + tid]);
+        for (int k = tid; k < (kEigTile * W) / 4; k += kEigThreads)
+            s4[k] = __builtin_nontemporal_load(&g4[k]);
     } else {
         for (int k = tid; k < cnt * W; k += kEigThreads) s_in[k] = gin[k];
     }
@@ -155,8 +156,8 @@ __global__ __launch_bounds__(kEigThreads) void k_eig3(const float* __restrict__ 
         f32x4* g4 = reinterpret_cast<f32x4*>(gout);
         const f32x4* s4 = reinterpret_cast<const f32x4*>(s_out);
 #pragma unroll
-        for (int k = 0; k < (kEigTile * 3) / (4 * kEigThreads); k++)
-            __builtin_nontemporal_store(s4[k * kEigThreads + tid], &g4[k * kEigThreads + tid]);
+        for (int k = tid; k < (kEigTile * 3) / 4; k += kEigThreads)
+            __builtin_nontemporal_store(s4[k], &g4[k]);
     } else {
         for (int k = tid; k < cnt * 3; k += kEigThreads) gout[k] = s_out[k];
     }
