@@ -83,6 +83,9 @@ int tira_hip_stream_synchronize(void* stream);
 int tira_hip_profile_begin(void);
 int tira_hip_profile_end(void);
 int tira_hip_profile_get(const char* kernel_name, double* total_ms, long* count);
+/* Plain device-to-device copy kernel (bytes a multiple of 16): the reference
+ * for bandwidth and profiler-counter calibration. Non-blocking. */
+int tira_hip_copy_async(void* dst, const void* src, size_t bytes);
 /* Release every cached device buffer held by the library on the current device. */
 int tira_hip_release_workspace(void);
 
@@ -191,6 +194,15 @@ int tira_batch_gauss_eig3_create(void** handle, int nvol, const float* const* in
 int tira_batch_gauss_eig3_run(void* handle);
 int tira_batch_gauss_eig3_plan(void* handle, int* zchunk, int* blocks_per_cu, float* tuned_ms, unsigned* items);
 int tira_batch_gauss_eig3_destroy(void* handle);
+
+/* ------------------------------------------------------------------ synthetic inputs */
+/* Seeded packed-6 test tensors generated on the device (benchmark inputs, no
+ * reference counterpart): eigenvalues log-uniform in [lo, hi], a fraction
+ * degenerate_frac with one adjacent pair at relative gap 10^U(log gap_lo,
+ * log gap_hi), Haar-random rotations; tensor i depends only on (seed, i).
+ * lambdas (optional, n x 3) receives the construction eigenvalues. */
+int tira_synth_eig_tensors_async(float* out6, size_t n, unsigned long long seed, float lo, float hi,
+                                 float degenerate_frac, float gap_lo, float gap_hi, float* lambdas);
 
 #ifdef __cplusplus
 }

@@ -71,6 +71,7 @@ SIGNATURES = {
     "tira_hip_event_record": [_vp, _vp],
     "tira_hip_event_elapsed_ms": [_vp, _vp, C.POINTER(_f)],
     "tira_hip_release_workspace": [],
+    "tira_hip_copy_async": [_vp, _vp, _sz],
     "tira_hip_profile_begin": [],
     "tira_hip_profile_end": [],
     "tira_hip_profile_get": [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_long)],
@@ -94,6 +95,7 @@ SIGNATURES = {
     "tira_batch_gauss_eig3_run": [_vp],
     "tira_batch_gauss_eig3_plan": [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_f), C.POINTER(_u)],
     "tira_batch_gauss_eig3_destroy": [_vp],
+    "tira_synth_eig_tensors_async": [_vp, _sz, C.c_ulonglong, _f, _f, _f, _f, _f, _vp],
 }
 
 _lib = None

@@ -67,6 +67,9 @@ class ProfScope {
     hipStream_t st_;
 };
 
+// Plain 16-byte-vector copy kernel (bandwidth reference for profiler counters).
+int launch_copy(void* dst, const void* src, size_t bytes, hipStream_t st);
+
 inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
 }  // namespace th

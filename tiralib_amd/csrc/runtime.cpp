@@ -275,6 +275,11 @@ int tira_hip_profile_get(const char* name, double* total_ms, long* count) {
     return TIRA_OK;
 }
 
+int tira_hip_copy_async(void* dst, const void* src, size_t bytes) {
+    TH_ARG(dst && src && bytes % 16 == 0, "copy needs device pointers and a multiple of 16 bytes");
+    return th::launch_copy(dst, src, bytes, g_stream);
+}
+
 int tira_hip_release_workspace(void) {
     int dev = 0;
     TH_HIP(hipGetDevice(&dev));
