@@ -33,6 +33,16 @@ PLATE_CLOSED_PAIR_FLOP = 65  # s = n.d, 4 polynomials in 1 - s^2, alpha I + beta
 # Reference formulation (SURVEY counting rule): stick 33, N-sample plate 41 N + 12.
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs (scaled by the copy-kernel calibration), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            return int(json.load(f)["kernels"][kernel]["traffic_bytes"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def in_bounds_pairs(shape, w):
     r = (w - 1) // 2
     p = 1
@@ -112,16 +122,18 @@ def bench_batch(steps, warmup, check=True):
         "plan": b.plan(),
         "roofline": {"kernel": "batch_gauss_eig3", "bound": "hbm", "achieved": round(bytes_launch / (ms * 1e-3) / 1e9, 1),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(bytes_launch / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4), "traffic": None,
+                     "frac": round(bytes_launch / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                     "traffic": profiled_traffic("k_batch_gauss_eig"),
                      "bytes_per_voxel": 60},
     }
-    # eig3 alone (packed-6 in, 3 eigenvalues out, 36 B per tensor), FP32 deflation and with FP64 re-solve
+    # eig3 alone over all 48 volumes in one launch (packed-6 in, 3 eigenvalues
+    # out, 36 B per tensor), FP32 deflation and with the FP64 re-solve
     ev = [DeviceArray(tuple(s) + (3,)) for s in shapes]
+    counts = [int(np.prod(s)) for s in shapes]
     for name, flags in (("eig3_fp32", _lib.EIG_SCALE_INVARIANT),
                         ("eig3_fp64_refine", _lib.EIG_SCALE_INVARIANT | _lib.EIG_REFINE_FP64)):
         def run_eig():
-            for v, e, s in zip(vols, ev, shapes):
-                call("tira_eig3_async", v.ptr, int(np.prod(s)), _lib.LAYOUT_PACKED6, e.ptr, None, None, flags, None)
+            api.eig3_batch(vols, counts=counts, evals=ev, flags=flags)
         run_eig()
         st.synchronize()
         e0.record(st)
@@ -130,7 +142,8 @@ def bench_batch(steps, warmup, check=True):
         e1.record(st)
         st.synchronize()
         ems = e0.elapsed_ms(e1) / steps
-        out[name] = {"ms": round(ems, 4), "eig3_per_s": round(nvox / (ems * 1e-3), 1),
+        out[name] = {"kernel": "eig3_batch (one launch, 48 arrays)", "ms": round(ems, 4),
+                     "eig3_per_s": round(nvox / (ems * 1e-3), 1),
                      "gbs": round(nvox * 36 / (ems * 1e-3) / 1e9, 1),
                      "frac_hbm": round(nvox * 36 / (ems * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
     if check:
@@ -241,7 +254,8 @@ def main(argv=None):
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-            "traffic": None,
+            "traffic": profiled_traffic("k_" + vote_name),
+            "traffic_source": "profiles/traffic.json (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per launch)",
             "flop_per_pair": pair_flop,
             "avg_launch_ms": round(vote_ms, 4),
             "launches": vote_cnt,

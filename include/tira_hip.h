@@ -109,6 +109,15 @@ int tira_evecs3spherical_symmetric(const float* mats, const float* lambdas, size
  * the number of tensors re-solved in FP64. Device pointers only. */
 int tira_eig3_async(const float* tensors, size_t n, int layout, float* evals, float* evec0, float* evec2,
                     unsigned flags, unsigned long long* n_refined_dev);
+/* tira_eig3_async over nvol independent tensor arrays in one launch (64
+ * arrays per launch; the array table travels in the kernel arguments, so the
+ * call uploads nothing and can be graph-captured). tensors[v] / evals[v] /
+ * evec0[v] / evec2[v] are device pointers for counts[v] tensors; evec0 and
+ * evec2 (the arrays of pointers) may be NULL. Results equal tira_eig3_async
+ * per array bit for bit. */
+int tira_eig3_batch_async(const float* const* tensors, const size_t* counts, int nvol, int layout,
+                          float* const* evals, float* const* evec0, float* const* evec2, unsigned flags,
+                          unsigned long long* n_refined_dev);
 
 /* ------------------------------------------------------------------ 2D voting */
 /* 48-byte neighbour record, field order of tira::Neighbor2D (tensorvote.h:776-786):

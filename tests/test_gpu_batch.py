@@ -63,3 +63,41 @@ def test_batch_graph_replays(hip):
             np.testing.assert_array_equal(o.to_host(), fo)
             np.testing.assert_array_equal(e.to_host(), fe)
     b.close()
+
+
+@pytest.mark.parametrize("layout", [_lib.LAYOUT_PACKED6, _lib.LAYOUT_MAT9])
+def test_eig3_batch_one_launch(hip, oracle, layout):
+    """tira_eig3_batch_async over uneven arrays (including an empty one and one
+    of 1 tensor) equals per-array tira_eig3_async bit for bit, and FP64 Jacobi
+    to 1e-6 normwise."""
+    counts = [int(np.prod(s)) for s in SHAPES] + [0, 1, 513]
+    host = []
+    for i, c in enumerate(counts):
+        t6, _ = synth.eig_tensors(max(c, 1), seed=3000 + i)
+        t6 = t6[:c]
+        host.append(synth.mat9_from_packed6(t6) if layout == _lib.LAYOUT_MAT9 else t6)
+    dev = [DeviceArray.from_host(h if h.size else np.zeros((1, 6), np.float32)) for h in host]
+    got = api.eig3_batch(dev, counts=counts, layout=layout)
+    singles = [api.eig3(d, layout=layout)[0].to_host() if c else None for d, c in zip(dev, counts)]
+    _lib.call("tira_hip_synchronize")
+    for h, c, g, s in zip(host, counts, got, singles):
+        if not c:
+            continue
+        ev = g.to_host()
+        assert np.array_equal(ev, s)
+        t6 = h if layout == _lib.LAYOUT_PACKED6 else synth.packed6_from_mat9(h)
+        ref = oracle.eig3_packed6_f64(np.ascontiguousarray(t6, np.float32))
+        nw = np.abs(ev.astype(np.float64) - ref).max(1) / np.abs(ref).max(1)
+        assert nw.max() <= 1e-6, nw.max()
+
+
+def test_eig3_batch_many_segments(hip):
+    """More arrays than one launch's segment table (64) holds."""
+    counts = [37 + 11 * i for i in range(150)]
+    host = [synth.eig_tensors(c, seed=4000 + i)[0] for i, c in enumerate(counts)]
+    dev = [DeviceArray.from_host(h) for h in host]
+    got = api.eig3_batch(dev)
+    singles = [api.eig3(d)[0].to_host() for d in dev]
+    _lib.call("tira_hip_synchronize")
+    for g, s in zip(got, singles):
+        assert np.array_equal(g.to_host(), s)

@@ -254,6 +254,27 @@ def gaussian_filter3_tensor(field, taps, same=True):
     return out
 
 
+def eig3_batch(tensors, counts=None, layout=_lib.LAYOUT_PACKED6, evals=None, flags=None):
+    """Eigenvalues of many device tensor arrays in one launch
+    (tira_eig3_batch_async). `tensors`: DeviceArrays; returns (and fills) one
+    n x 3 DeviceArray per input. Non-blocking on the current stream."""
+    per = 9 if layout == _lib.LAYOUT_MAT9 else 6
+    if counts is None:
+        counts = [_count(t, per) for t in tensors]
+    if evals is None:
+        evals = [DeviceArray((c, 3)) for c in counts]
+    if flags is None:
+        flags = _lib.EIG_REFINE_FP64 | _lib.EIG_SCALE_INVARIANT
+    n = len(tensors)
+    P = C.c_void_p * max(n, 1)
+    tin = P(*[t.ptr for t in tensors])
+    tout = P(*[e.ptr for e in evals])
+    cnt = np.ascontiguousarray(counts, np.uint64)
+    call("tira_eig3_batch_async", C.cast(tin, C.c_void_p), cnt.ctypes.data, n, layout, C.cast(tout, C.c_void_p),
+         None, None, flags, None)
+    return evals
+
+
 # ------------------------------------------------------------------ batched volumes
 class BatchGaussEig3:
     """Persistent, graph-captured Gaussian smoothing + eig3 of many uneven

@@ -36,6 +36,8 @@ int fail_hip(hipError_t e, const char* what, const char* file, int line);
 hipStream_t stream();
 // True for device (or managed) memory; host / unregistered memory is false.
 bool is_device(const void* p);
+// Compute units of the current device (cached per device).
+int cu_count();
 // Select `device` when >= 0; rejects negative devices.
 int use_device(int device);
 

@@ -4,16 +4,37 @@
 
 namespace th {
 
-__global__ __launch_bounds__(256) void k_copy16(float4* __restrict__ dst, const float4* __restrict__ src, size_t n) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+// Each thread moves kCopyUnroll 16-byte vectors (block-strided, so every
+// load instruction of a wave reads 1 KiB contiguous), all loads issued
+// before the first store.
+constexpr int kCopyUnroll = 4;
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_copy16(f32x4* __restrict__ dst, const f32x4* __restrict__ src, size_t n) {
+    const size_t base = (size_t)blockIdx.x * (256 * kCopyUnroll) + threadIdx.x;
+    f32x4 r[kCopyUnroll];
+#pragma unroll
+    for (int j = 0; j < kCopyUnroll; j++) {
+        const size_t i = base + (size_t)j * 256;
+        if (i < n) r[j] = __builtin_nontemporal_load(&src[i]);
+    }
+#pragma unroll
+    for (int j = 0; j < kCopyUnroll; j++) {
+        const size_t i = base + (size_t)j * 256;
+        if (i < n) __builtin_nontemporal_store(r[j], &dst[i]);
+    }
 }
 
 int launch_copy(void* dst, const void* src, size_t bytes, hipStream_t st) {
     const size_t n = bytes / 16;
-    const unsigned grid = (unsigned)std::min<size_t>(ceil_div(n, 256), 8192);
+    if (!n) return TIRA_OK;
+    const size_t blocks = (n + 256 * kCopyUnroll - 1) / (256 * kCopyUnroll);
+    if (blocks > 0x7fffffffu) {
+        set_error("copy of %zu bytes exceeds one launch", bytes);
+        return TIRA_ERR_INVALID_ARG;
+    }
     ProfScope prof("copy16", st);
-    k_copy16<<<grid, 256, 0, st>>>((float4*)dst, (const float4*)src, n);
+    k_copy16<<<(unsigned)blocks, 256, 0, st>>>((f32x4*)dst, (const f32x4*)src, n);
     TH_HIP(hipGetLastError());
     return TIRA_OK;
 }

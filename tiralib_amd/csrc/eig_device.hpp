@@ -253,6 +253,16 @@ TH_HD float d_rsqrt(float x) {
 #endif
 }
 TH_HD double d_rsqrt(double x) { return 1.0 / sqrt(x); }
+// Square root without the correctly-rounded fix-up sequence (hardware
+// v_sqrt_f32, about 1 ulp) in FP32 device code; exact sqrt elsewhere.
+TH_HD float d_sqrt_fast(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_sqrtf(x);
+#else
+    return sqrtf(x);
+#endif
+}
+TH_HD double d_sqrt_fast(double x) { return sqrt(x); }
 
 // Largest root of beta^3 - 3 beta - 2 r = 0 for r in [0, 1], i.e.
 // beta = 2 cos(acos(r) / 3) in [sqrt 3, 2]: a cubic fit (max error 1.3e-4)
@@ -281,7 +291,7 @@ TH_HD T cubic_top_root(T r) {
 // cofactor eigenvector v (eigen.h:247-288) deflates A to the plane orthogonal
 // to v, where the close pair is a symmetric 2x2 problem (perfectly
 // conditioned), and the separated eigenvalue becomes the Rayleigh quotient
-// v.Av. Optional outputs: unit eigenvectors e0 (smallest) and e2 (largest)
+// v.Av (taken from the trace). Optional outputs: unit eigenvectors e0 (smallest) and e2 (largest)
 // with canonical sign.
 template <typename T>
 TH_HD void eig3_solve(T a, T b, T c, T d, T e, T f, T diag_eps, T* lam, T* e0, T* e2) {
@@ -299,7 +309,7 @@ TH_HD void eig3_solve(T a, T b, T c, T d, T e, T f, T diag_eps, T* lam, T* e0, T
     }
     const T q = (a + c + f) * T(1.0 / 3.0);
     const T aq = a - q, cq = c - q, fq = f - q;
-    const T p = d_sqrt((aq * aq + cq * cq + fq * fq + T(2) * p1) * T(1.0 / 6.0));
+    const T p = d_sqrt_fast((aq * aq + cq * cq + fq * fq + T(2) * p1) * T(1.0 / 6.0));
     const T pinv = d_rcp(p);
     const T det = aq * (cq * fq - e * e) - b * (b * fq - e * d) + d * (b * e - cq * d);
     T r = det * (pinv * pinv * pinv) * T(0.5);
@@ -345,14 +355,14 @@ TH_HD void eig3_solve(T a, T b, T c, T d, T e, T f, T diag_eps, T* lam, T* e0, T
             AU2 = d * U[0] + e * U[1] + f * U[2];
     const T AW0 = a * W[0] + b * W[1] + d * W[2], AW1 = b * W[0] + c * W[1] + e * W[2],
             AW2 = d * W[0] + e * W[1] + f * W[2];
-    const T Av0 = a * v[0] + b * v[1] + d * v[2], Av1 = b * v[0] + c * v[1] + e * v[2],
-            Av2 = d * v[0] + e * v[1] + f * v[2];
     const T m00 = U[0] * AU0 + U[1] * AU1 + U[2] * AU2;
     const T m01 = U[0] * AW0 + U[1] * AW1 + U[2] * AW2;
     const T m11 = W[0] * AW0 + W[1] * AW1 + W[2] * AW2;
-    const T ls = v[0] * Av0 + v[1] * Av1 + v[2] * Av2;
+    // Rayleigh quotient v.Av through the trace: {v, U, W} is orthonormal, so
+    // tr(A) = v.Av + U.AU + W.AW exactly
+    const T ls = (a + c + f) - (m00 + m11);
     const T mean = T(0.5) * (m00 + m11), hd = T(0.5) * (m00 - m11);
-    const T h = d_sqrt(hd * hd + m01 * m01);
+    const T h = d_sqrt_fast(hd * hd + m01 * m01);
     const T lo = mean - h, hi = mean + h;
     T x0, x1, x2;
     if (top) { x0 = lo; x1 = hi; x2 = ls; } else { x0 = ls; x1 = lo; x2 = hi; }

@@ -66,43 +66,97 @@ __device__ __forceinline__ void global_tensor(const float* g, size_t i, float& a
     }
 }
 
-// evals: n x 3. evec0 / evec2 (n x 3) written when VECS. flags: TIRA_EIG_*.
+// Segment table: one launch covers up to kEigMaxSeg independent tensor
+// arrays. It travels in the kernel arguments (no upload, capturable); tile t
+// belongs to the last segment whose tile0 <= t.
+struct EigSeg {
+    const float* in;
+    float* evals;
+    float* evec0;
+    float* evec2;
+    unsigned long long n;
+    unsigned tile0;
+    unsigned pad;
+};
+constexpr int kEigMaxSeg = 64;
+struct EigBatchArgs {
+    EigSeg seg[kEigMaxSeg];
+    int nseg;
+};
+
+struct EigTile {
+    const float* in;
+    float* out;
+    float* v0;
+    float* v2;
+    int cnt;
+    bool vec;  // full tile with a 16-byte aligned input: staged with float4 loads
+};
+
+template <int W>
+__device__ __forceinline__ EigTile eig_tile_at(const EigBatchArgs& args, unsigned t) {
+    int lo = 0, hi = args.nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (args.seg[mid].tile0 <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const EigSeg& sg = args.seg[lo];
+    const size_t base = (size_t)(t - sg.tile0) * kEigTile;
+    const size_t left = sg.n - base;
+    EigTile r;
+    r.in = sg.in + base * W;
+    r.out = sg.evals + base * 3;
+    r.v0 = sg.evec0 ? sg.evec0 + base * 3 : nullptr;
+    r.v2 = sg.evec2 ? sg.evec2 + base * 3 : nullptr;
+    r.cnt = left < (size_t)kEigTile ? (int)left : kEigTile;
+    r.vec = (r.cnt == kEigTile) && ((((uintptr_t)r.in) & 15u) == 0);
+    return r;
+}
+
+// One block per tile of kEigTile consecutive tensors of one array: staged in
+// LDS with 16-byte loads, solved in FP32 (deflation, eig3_solve); tensors
+// whose relative eigengap is below 1e-4 are compacted into a block-local list
+// and re-solved in FP64 when TIRA_EIG_REFINE_FP64 is set; results are staged
+// in LDS and written with 16-byte stores. evec0 / evec2 are written when VECS.
+// (The kernel is VALU-bound near the HBM rate: a persistent variant that
+// prefetches the next tile into registers lost more to the occupancy its
+// extra registers cost than it gained in overlap.)
 template <int LAYOUT, bool VECS>
-__global__ __launch_bounds__(kEigThreads) void k_eig3(const float* __restrict__ in, size_t n,
-                                                      float* __restrict__ evals, float* __restrict__ evec0,
-                                                      float* __restrict__ evec2, unsigned flags,
+__global__ __launch_bounds__(kEigThreads) void k_eig3(const EigBatchArgs args, unsigned ntiles, unsigned flags,
                                                       unsigned long long* __restrict__ n_refined) {
     constexpr int W = (LAYOUT == TIRA_LAYOUT_MAT9) ? 9 : 6;
-    __shared__ float s_in[kEigTile * W];
-    __shared__ float s_out[kEigTile * 3];
+    constexpr int TOT4 = kEigTile * W / 4;
+    static_assert((kEigTile * W) % 4 == 0, "tile must be a whole number of float4");
+    __shared__ f32x4 s_in4[TOT4];
+    __shared__ f32x4 s_out4[kEigTile * 3 / 4];
     __shared__ short s_list[kEigTile];
     __shared__ int s_count;
+    float* s_in = reinterpret_cast<float*>(s_in4);
+    float* s_out = reinterpret_cast<float*>(s_out4);
 
     const int tid = threadIdx.x;
-    const size_t base = (size_t)blockIdx.x * kEigTile;
-    const int cnt = (int)((n - base) < (size_t)kEigTile ? (n - base) : (size_t)kEigTile);
+    const unsigned t = blockIdx.x;
+    if (t >= ntiles) return;
+    const EigTile cur = eig_tile_at<W>(args, t);
+    const int cnt = cur.cnt;
     if (tid == 0) s_count = 0;
 
-    // ---- stage the tile: 16-byte loads when full and aligned, else scalar
-    const float* gin = in + base * W;
-    const bool vec_ok = (cnt == kEigTile) && ((((uintptr_t)gin) & 15u) == 0);
-    if (vec_ok) {
-        const f32x4* g4 = reinterpret_cast<const f32x4*>(gin);
-        f32x4* s4 = reinterpret_cast<f32x4*>(s_in);
-        static_assert((kEigTile * W) % 4 == 0, "tile must be a whole number of float4");
+    // ---- stage the tile
+    if (cur.vec) {
+        const f32x4* g4 = reinterpret_cast<const f32x4*>(cur.in);
 #pragma unroll
-        for (int k = tid; k < (kEigTile * W) / 4; k += kEigThreads)
-            s4[k] = __builtin_nontemporal_load(&g4[k]);
+        for (int k = tid; k < TOT4; k += kEigThreads) s_in4[k] = __builtin_nontemporal_load(&g4[k]);
     } else {
-        for (int k = tid; k < cnt * W; k += kEigThreads) s_in[k] = gin[k];
+        for (int k = tid; k < cnt * W; k += kEigThreads) s_in[k] = cur.in[k];
     }
     __syncthreads();
 
     const bool scale_inv = (flags & TIRA_EIG_SCALE_INVARIANT) != 0;
     const bool refine = (flags & TIRA_EIG_REFINE_FP64) != 0;
-
-    // ---- FP32 pass (trigonometric values repaired by deflation, see eig3_solve)
     const float deps = scale_inv ? 1e-30f : 1e-6f;
+
+    // ---- FP32 pass
 #pragma unroll
     for (int j = 0; j < kEigPerThread; j++) {
         const int i = j * kEigThreads + tid;
@@ -118,9 +172,8 @@ __global__ __launch_bounds__(kEigThreads) void k_eig3(const float* __restrict__ 
             s_list[slot] = (short)i;
         }
         if (VECS) {
-            const size_t g = base + i;
-            if (evec0) { evec0[g * 3 + 0] = v0[0]; evec0[g * 3 + 1] = v0[1]; evec0[g * 3 + 2] = v0[2]; }
-            if (evec2) { evec2[g * 3 + 0] = v2[0]; evec2[g * 3 + 1] = v2[1]; evec2[g * 3 + 2] = v2[2]; }
+            if (cur.v0) { cur.v0[i * 3 + 0] = v0[0]; cur.v0[i * 3 + 1] = v0[1]; cur.v0[i * 3 + 2] = v0[2]; }
+            if (cur.v2) { cur.v2[i * 3 + 0] = v2[0]; cur.v2[i * 3 + 1] = v2[1]; cur.v2[i * 3 + 2] = v2[2]; }
         }
         s_out[i * 3 + 0] = ldexpf(lam[0], ex);
         s_out[i * 3 + 1] = ldexpf(lam[1], ex);
@@ -142,24 +195,20 @@ __global__ __launch_bounds__(kEigThreads) void k_eig3(const float* __restrict__ 
         s_out[i * 3 + 1] = ldexpf((float)lam[1], ex);
         s_out[i * 3 + 2] = ldexpf((float)lam[2], ex);
         if (VECS) {
-            const size_t g = base + i;
-            if (evec0) { evec0[g * 3 + 0] = (float)v0[0]; evec0[g * 3 + 1] = (float)v0[1]; evec0[g * 3 + 2] = (float)v0[2]; }
-            if (evec2) { evec2[g * 3 + 0] = (float)v2[0]; evec2[g * 3 + 1] = (float)v2[1]; evec2[g * 3 + 2] = (float)v2[2]; }
+            if (cur.v0) { cur.v0[i * 3 + 0] = (float)v0[0]; cur.v0[i * 3 + 1] = (float)v0[1]; cur.v0[i * 3 + 2] = (float)v0[2]; }
+            if (cur.v2) { cur.v2[i * 3 + 0] = (float)v2[0]; cur.v2[i * 3 + 1] = (float)v2[1]; cur.v2[i * 3 + 2] = (float)v2[2]; }
         }
     }
     if (n_refined && tid == 0 && nflag) atomicAdd(n_refined, (unsigned long long)nflag);
     __syncthreads();
 
     // ---- write back
-    float* gout = evals + base * 3;
-    if (cnt == kEigTile && ((((uintptr_t)gout) & 15u) == 0)) {
-        f32x4* g4 = reinterpret_cast<f32x4*>(gout);
-        const f32x4* s4 = reinterpret_cast<const f32x4*>(s_out);
+    if (cnt == kEigTile && ((((uintptr_t)cur.out) & 15u) == 0)) {
+        f32x4* g4 = reinterpret_cast<f32x4*>(cur.out);
 #pragma unroll
-        for (int k = tid; k < (kEigTile * 3) / 4; k += kEigThreads)
-            __builtin_nontemporal_store(s4[k], &g4[k]);
+        for (int k = tid; k < (kEigTile * 3) / 4; k += kEigThreads) __builtin_nontemporal_store(s_out4[k], &g4[k]);
     } else {
-        for (int k = tid; k < cnt * 3; k += kEigThreads) gout[k] = s_out[k];
+        for (int k = tid; k < cnt * 3; k += kEigThreads) cur.out[k] = s_out[k];
     }
 }
 
@@ -215,23 +264,81 @@ __global__ __launch_bounds__(256) void k_evecs2polar(const float* __restrict__ m
     reinterpret_cast<float2*>(th)[i] = make_float2(t0, t1);
 }
 
+template <int LAYOUT, bool VECS>
+static void launch_eig3_args(const EigBatchArgs& a, unsigned tiles, unsigned flags, unsigned long long* n_refined,
+                             hipStream_t st) {
+    k_eig3<LAYOUT, VECS><<<tiles, kEigThreads, 0, st>>>(a, tiles, flags, n_refined);
+}
+
+static int launch_eig3_table(const EigBatchArgs& a, unsigned tiles, int layout, bool vecs, unsigned flags,
+                             unsigned long long* n_refined, hipStream_t st) {
+    if (layout == TIRA_LAYOUT_MAT9) {
+        if (vecs) launch_eig3_args<TIRA_LAYOUT_MAT9, true>(a, tiles, flags, n_refined, st);
+        else launch_eig3_args<TIRA_LAYOUT_MAT9, false>(a, tiles, flags, n_refined, st);
+    } else {
+        if (vecs) launch_eig3_args<TIRA_LAYOUT_PACKED6, true>(a, tiles, flags, n_refined, st);
+        else launch_eig3_args<TIRA_LAYOUT_PACKED6, false>(a, tiles, flags, n_refined, st);
+    }
+    TH_HIP(hipGetLastError());
+    return TIRA_OK;
+}
+
+// Largest tensor count one launch addresses (32-bit tile index).
+constexpr size_t kEigMaxTensors = (size_t)0x7fffffffu * kEigTile;
+
 int launch_eig3(const float* in, size_t n, int layout, float* evals, float* evec0, float* evec2, unsigned flags,
                 unsigned long long* n_refined, hipStream_t st) {
     if (n == 0) return TIRA_OK;
-    const unsigned grid = ceil_div(n, kEigTile);
-    const bool vecs = evec0 || evec2;
-    ProfScope prof("eig3", st);
-    if (layout == TIRA_LAYOUT_MAT9) {
-        if (vecs) k_eig3<TIRA_LAYOUT_MAT9, true><<<grid, kEigThreads, 0, st>>>(in, n, evals, evec0, evec2, flags, n_refined);
-        else k_eig3<TIRA_LAYOUT_MAT9, false><<<grid, kEigThreads, 0, st>>>(in, n, evals, evec0, evec2, flags, n_refined);
-    } else if (layout == TIRA_LAYOUT_PACKED6) {
-        if (vecs) k_eig3<TIRA_LAYOUT_PACKED6, true><<<grid, kEigThreads, 0, st>>>(in, n, evals, evec0, evec2, flags, n_refined);
-        else k_eig3<TIRA_LAYOUT_PACKED6, false><<<grid, kEigThreads, 0, st>>>(in, n, evals, evec0, evec2, flags, n_refined);
-    } else {
+    if (layout != TIRA_LAYOUT_MAT9 && layout != TIRA_LAYOUT_PACKED6) {
         set_error("unknown tensor layout %d", layout);
         return TIRA_ERR_INVALID_ARG;
     }
-    TH_HIP(hipGetLastError());
+    if (n > kEigMaxTensors) {
+        set_error("eig3: %zu tensors exceed the %zu one launch addresses", n, kEigMaxTensors);
+        return TIRA_ERR_INVALID_ARG;
+    }
+    EigBatchArgs a{};
+    a.nseg = 1;
+    a.seg[0] = EigSeg{in, evals, evec0, evec2, (unsigned long long)n, 0u, 0u};
+    ProfScope prof("eig3", st);
+    return launch_eig3_table(a, ceil_div(n, kEigTile), layout, evec0 || evec2, flags, n_refined, st);
+}
+
+// Up to kEigMaxSeg non-empty arrays per launch.
+int launch_eig3_batch(const float* const* in, const size_t* counts, int nvol, int layout, float* const* evals,
+                      float* const* evec0, float* const* evec2, unsigned flags, unsigned long long* n_refined,
+                      hipStream_t st) {
+    if (layout != TIRA_LAYOUT_MAT9 && layout != TIRA_LAYOUT_PACKED6) {
+        set_error("unknown tensor layout %d", layout);
+        return TIRA_ERR_INVALID_ARG;
+    }
+    const bool vecs = evec0 || evec2;
+    ProfScope prof("eig3_batch", st);
+    int v = 0;
+    while (v < nvol) {
+        EigBatchArgs a{};
+        size_t tiles = 0;
+        for (; v < nvol && a.nseg < kEigMaxSeg; v++) {
+            if (!counts[v]) continue;
+            if (counts[v] > kEigMaxTensors) {
+                set_error("eig3: array %d holds %zu tensors, above the %zu one launch addresses", v, counts[v],
+                          kEigMaxTensors);
+                return TIRA_ERR_INVALID_ARG;
+            }
+            const size_t t = ceil_div(counts[v], kEigTile);
+            if (tiles + t > 0x7fffffffu) break;
+            EigSeg& s = a.seg[a.nseg++];
+            s.in = in[v];
+            s.evals = evals[v];
+            s.evec0 = evec0 ? evec0[v] : nullptr;
+            s.evec2 = evec2 ? evec2[v] : nullptr;
+            s.n = counts[v];
+            s.tile0 = (unsigned)tiles;
+            tiles += t;
+        }
+        if (!a.nseg) break;
+        TH_TRY(launch_eig3_table(a, (unsigned)tiles, layout, vecs, flags, n_refined, st));
+    }
     return TIRA_OK;
 }
 
@@ -295,6 +402,20 @@ int tira_eig3_async(const float* tensors, size_t n, int layout, float* evals, fl
                     unsigned flags, unsigned long long* n_refined_dev) {
     TH_ARG(tensors && evals, "NULL pointer");
     return launch_eig3(tensors, n, layout, evals, evec0, evec2, flags, n_refined_dev, stream());
+}
+
+int tira_eig3_batch_async(const float* const* tensors, const size_t* counts, int nvol, int layout,
+                          float* const* evals, float* const* evec0, float* const* evec2, unsigned flags,
+                          unsigned long long* n_refined_dev) {
+    TH_ARG(nvol >= 0, "negative volume count");
+    if (!nvol) return TIRA_OK;
+    TH_ARG(tensors && counts && evals, "NULL pointer");
+    for (int v = 0; v < nvol; v++) {
+        if (!counts[v]) continue;
+        TH_ARG(tensors[v] && evals[v], "NULL tensor or eigenvalue pointer in volume %d", v);
+        TH_ARG(is_device(tensors[v]) && is_device(evals[v]), "volume %d: batched eig3 takes device pointers", v);
+    }
+    return launch_eig3_batch(tensors, counts, nvol, layout, evals, evec0, evec2, flags, n_refined_dev, stream());
 }
 
 }  // extern "C"

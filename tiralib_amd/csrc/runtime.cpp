@@ -1,6 +1,7 @@
 // runtime.cpp -- C-ABI runtime entry points and the plumbing in common.hpp.
 #include <cstdarg>
 #include <cstdio>
+#include <atomic>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -36,6 +37,21 @@ bool is_device(const void* p) {
         return false;
     }
     return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+int cu_count() {
+    static constexpr int kMaxDev = 64;
+    static std::atomic<int> cache[kMaxDev];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return 256;
+    if (dev < kMaxDev) {
+        const int c = cache[dev].load(std::memory_order_relaxed);
+        if (c > 0) return c;
+    }
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n < 1) n = 256;
+    if (dev < kMaxDev) cache[dev].store(n, std::memory_order_relaxed);
+    return n;
 }
 
 int use_device(int device) {
