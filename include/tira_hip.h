@@ -213,6 +213,59 @@ int tira_batch_gauss_eig3_destroy(void* handle);
 int tira_synth_eig_tensors_async(float* out6, size_t n, unsigned long long seed, float lo, float hi,
                                  float degenerate_frac, float gap_lo, float gap_hi, float* lambdas);
 
+/* ------------------------------------------------------------------ learned plate vote
+ * No reference counterpart: a BF16 MLP (5 -> 64 -> 64 -> 64 -> 6, MFMA, FP32
+ * accumulation) that stands in for the plate-vote integral of
+ * tensorvote.h:1651-1695, trained on the device by this library's own forward,
+ * backward and Adam kernels against the FP64 N-sample integrator. Handles are
+ * bound to the device current at creation; calls are synchronous unless
+ * noted. Parameters are FP32, row-major [out][in]: W1 64x32, W2 64x64,
+ * W3 64x64, W4 32x64 (12288 floats); input slots 0-4 feature high parts,
+ * 5-9 residuals, 10 constant 1; hidden unit 63 is a constant-1 unit. */
+#define TIRA_MLP_NPARAM 12288
+#define TIRA_MLP_NBINS 2048
+int tira_plate_mlp_create(void** handle, float sigma1, float sigma2, unsigned power, unsigned integrator_samples,
+                          unsigned long long seed);
+int tira_plate_mlp_destroy(void* handle);
+/* Trains on `nsamples` fresh samples (offset uniform in the 3 sigma1 ball,
+ * normal uniform on the sphere), `epochs` passes in batches of `batch` (a
+ * multiple of 256), Adam with warm-up and cosine decay from `lr` to
+ * `lr_final` (0: lr / 100); then validates on 262144 held-out samples and
+ * rebuilds the per-orientation error table. Outputs relative Frobenius
+ * errors (last batch, validation). */
+int tira_plate_mlp_train(void* handle, size_t nsamples, unsigned epochs, unsigned batch, float lr, float lr_final,
+                         unsigned long long seed, float* train_rel_err, float* val_rel_err);
+/* Host copies of the FP32 parameters: `get` (may be NULL) receives them,
+ * `set` (may be NULL) replaces them first. */
+int tira_plate_mlp_params(void* handle, float* get, const float* set);
+/* n samples into device buffers: X n x 32 BF16 inputs, T n x 8 targets (6
+ * plate-frame components / c1, then 0, 0), raw n x 8 (may be NULL): offset
+ * vector, normal, c1, 0. */
+int tira_plate_mlp_samples(void* handle, size_t n, unsigned long long seed, void* X, float* T, float* raw);
+/* Y (device, n x 32 FP32; columns 0-5 meaningful) = network(X); n % 32 == 0. */
+int tira_plate_mlp_forward(void* handle, const void* X, size_t n, float* Y);
+/* One forward + backward pass over n samples (n % 256 == 0): host `grads`
+ * (TIRA_MLP_NPARAM) receive d loss / d params; sums (host, 2 floats, may be
+ * NULL) receive the summed weighted squared error and target energy. */
+int tira_plate_mlp_gradients(void* handle, const void* X, const float* T, size_t n, float* grads, float* sums);
+/* One training step (forward, backward, Adam) with learning rate lr. */
+int tira_plate_mlp_step(void* handle, const void* X, const float* T, size_t n, float lr);
+/* The FP32 output affine (12 floats: means, then scales of the 6 plate-frame
+ * components; the network predicts standardized components). `get` (may be
+ * NULL) receives it, `set` (may be NULL) replaces it first. Training sets it
+ * from the training targets. */
+int tira_plate_mlp_affine(void* handle, float* get, const float* set);
+/* Per-orientation relative error of the network (TIRA_MLP_NBINS floats). */
+int tira_plate_mlp_bin_errors(void* handle, float* errs);
+int tira_plate_mlp_refresh_bins(void* handle);
+/* 3D stick + learned plate vote (device pointers, non-blocking unless
+ * n_fallback is given): the network's sigma1, sigma2 and power; voters whose
+ * orientation bin's error exceeds `tol` take the integrator's plate vote.
+ * n_fallback (host, may be NULL) receives the number of such voters. */
+int tira_tensorvote3_mlp(void* mlp, const float* input_field, int in_layout, float* output_field, int out_layout,
+                         unsigned s0, unsigned s1, unsigned s2, unsigned w, int stick, float tol,
+                         unsigned long long* n_fallback);
+
 #ifdef __cplusplus
 }
 #endif

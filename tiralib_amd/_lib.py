@@ -32,6 +32,9 @@ PLATE_MLP = 3
 EIG_REFINE_FP64 = 1
 EIG_SCALE_INVARIANT = 2
 
+MLP_NPARAM = 12288
+MLP_NBINS = 2048
+
 
 class TiraError(RuntimeError):
     def __init__(self, status, fn, msg):
@@ -97,6 +100,18 @@ SIGNATURES = {
     "tira_batch_gauss_eig3_plan": [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_f), C.POINTER(_u)],
     "tira_batch_gauss_eig3_destroy": [_vp],
     "tira_synth_eig_tensors_async": [_vp, _sz, C.c_ulonglong, _f, _f, _f, _f, _f, _vp],
+    "tira_plate_mlp_create": [C.POINTER(_vp), _f, _f, _u, _u, C.c_ulonglong],
+    "tira_plate_mlp_destroy": [_vp],
+    "tira_plate_mlp_train": [_vp, _sz, _u, _u, _f, _f, C.c_ulonglong, C.POINTER(_f), C.POINTER(_f)],
+    "tira_plate_mlp_params": [_vp, _vp, _vp],
+    "tira_plate_mlp_samples": [_vp, _sz, C.c_ulonglong, _vp, _vp, _vp],
+    "tira_plate_mlp_forward": [_vp, _vp, _sz, _vp],
+    "tira_plate_mlp_gradients": [_vp, _vp, _vp, _sz, _vp, _vp],
+    "tira_plate_mlp_step": [_vp, _vp, _vp, _sz, _f],
+    "tira_plate_mlp_affine": [_vp, _vp, _vp],
+    "tira_plate_mlp_bin_errors": [_vp, _vp],
+    "tira_plate_mlp_refresh_bins": [_vp],
+    "tira_tensorvote3_mlp": [_vp, _vp, _i, _vp, _i, _u, _u, _u, _u, _i, _f, C.POINTER(C.c_ulonglong)],
 }
 
 _lib = None

@@ -315,3 +315,95 @@ class BatchGaussEig3:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------------ learned plate vote
+class PlateMLP:
+    """BF16 plate-vote surrogate (tira_plate_mlp_*): trained on the device
+    against the FP64 integrator; votes through tira_tensorvote3_mlp."""
+
+    NPARAM = _lib.MLP_NPARAM
+    LAYERS = (("W1", 0, 64, 32), ("W2", 2048, 64, 64), ("W3", 6144, 64, 64), ("W4", 10240, 32, 64))
+
+    def __init__(self, sigma1, sigma2=0.0, power=1, integrator_samples=64, seed=1004):
+        h = C.c_void_p()
+        call("tira_plate_mlp_create", C.byref(h), float(sigma1), float(sigma2), int(power), int(integrator_samples),
+             int(seed))
+        self.handle = h.value
+        self.sigma1, self.sigma2, self.power = float(sigma1), float(sigma2), int(power)
+
+    def train(self, nsamples=1 << 22, epochs=300, batch=8192, lr=1e-2, lr_final=0.0, seed=1004):
+        tr, va = C.c_float(), C.c_float()
+        call("tira_plate_mlp_train", self.handle, int(nsamples), int(epochs), int(batch), float(lr), float(lr_final),
+             int(seed), C.byref(tr), C.byref(va))
+        return tr.value, va.value
+
+    def params(self):
+        p = np.empty(self.NPARAM, np.float32)
+        call("tira_plate_mlp_params", self.handle, p.ctypes.data, None)
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        if p.shape != (self.NPARAM,):
+            raise ValueError(f"expected {self.NPARAM} parameters")
+        call("tira_plate_mlp_params", self.handle, None, p.ctypes.data)
+
+    def affine(self):
+        a = np.empty(12, np.float32)
+        call("tira_plate_mlp_affine", self.handle, a.ctypes.data, None)
+        return a[:6].copy(), a[6:].copy()
+
+    def set_affine(self, mu, sd):
+        a = np.ascontiguousarray(np.concatenate([mu, sd]), np.float32)
+        call("tira_plate_mlp_affine", self.handle, None, a.ctypes.data)
+
+    def samples(self, n, seed, raw=False):
+        X = DeviceArray((n, 32), np.uint16)
+        T = DeviceArray((n, 8))
+        R = DeviceArray((n, 8)) if raw else None
+        call("tira_plate_mlp_samples", self.handle, int(n), int(seed), X.ptr, T.ptr, R.ptr if R else None)
+        return X, T, R
+
+    def forward(self, X):
+        n = X.shape[0]
+        Y = DeviceArray((n, 32))
+        call("tira_plate_mlp_forward", self.handle, X.ptr, n, Y.ptr)
+        return Y
+
+    def gradients(self, X, T):
+        g = np.empty(self.NPARAM, np.float32)
+        sums = np.empty(2, np.float32)
+        call("tira_plate_mlp_gradients", self.handle, X.ptr, T.ptr, X.shape[0], g.ctypes.data, sums.ctypes.data)
+        return g, sums
+
+    def step(self, X, T, lr):
+        call("tira_plate_mlp_step", self.handle, X.ptr, T.ptr, X.shape[0], float(lr))
+
+    def bin_errors(self):
+        e = np.empty(_lib.MLP_NBINS, np.float32)
+        call("tira_plate_mlp_bin_errors", self.handle, e.ctypes.data)
+        return e
+
+    def refresh_bins(self):
+        call("tira_plate_mlp_refresh_bins", self.handle)
+
+    def vote(self, field, out, shape, w, stick=True, tol=1e-3, in_layout=_lib.LAYOUT_MAT9,
+             out_layout=_lib.LAYOUT_PACKED6, count_fallback=False):
+        """Stick + learned plate vote of a device field; returns the number of
+        integrator-fallback voters when count_fallback (synchronises)."""
+        nf = C.c_ulonglong(0)
+        call("tira_tensorvote3_mlp", self.handle, field.ptr, in_layout, out.ptr, out_layout, *[int(s) for s in shape],
+             int(w), int(stick), float(tol), C.byref(nf) if count_fallback else None)
+        return nf.value if count_fallback else None
+
+    def close(self):
+        if self.handle:
+            call("tira_plate_mlp_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -24,6 +24,7 @@
 //         halo voters -- land on the same XCD's L2.
 #include "common.hpp"
 #include "eig_device.hpp"
+#include "mlp.hpp"
 #include "vote3.hpp"
 
 #include <cmath>
@@ -421,6 +422,30 @@ __device__ __forceinline__ void load_tensor(const float* __restrict__ in, int la
     }
 }
 
+// Eigen-decomposition of voxel v for the payload: power-of-two prescale
+// (exact; makes every threshold relative to |A|), FP32 deflated solve, FP64
+// re-solve when the relative eigengap is below 1e-4.
+__device__ __forceinline__ void voxel_eig(const float* __restrict__ in, int layout, size_t v, float& l0, float& l1,
+                                          float& l2, float e0[3], float e2[3]) {
+    float a, b, c, d, e, f;
+    load_tensor(in, layout, v, a, b, c, d, e, f);
+    const float m = fmaxf(fmaxf(fmaxf(fabsf(a), fabsf(b)), fmaxf(fabsf(c), fabsf(d))), fmaxf(fabsf(e), fabsf(f)));
+    int ex = 0;
+    if (m > 0.0f && isfinite(m)) {
+        frexpf(m, &ex);
+        a = ldexpf(a, -ex); b = ldexpf(b, -ex); c = ldexpf(c, -ex);
+        d = ldexpf(d, -ex); e = ldexpf(e, -ex); f = ldexpf(f, -ex);
+    }
+    float lam[3];
+    eig3_solve<float>(a, b, c, d, e, f, 1e-30f, lam, e0, e2);
+    if (eig3_near_degenerate(lam[0], lam[1], lam[2])) {
+        double dl[3], de0[3], de2[3];
+        eig3_solve<double>(a, b, c, d, e, f, 1e-30, dl, de0, de2);
+        for (int k = 0; k < 3; k++) { lam[k] = (float)dl[k]; e0[k] = (float)de0[k]; e2[k] = (float)de2[k]; }
+    }
+    l0 = ldexpf(lam[0], ex); l1 = ldexpf(lam[1], ex); l2 = ldexpf(lam[2], ex);
+}
+
 // Plate basis of tensorvote.h:1667-1673 / 2468-2470 for a unit normal n.
 __device__ __forceinline__ void plate_basis(const float n[3], float u[3], float v[3]) {
     float a0 = 0.0f, a1 = 0.0f, a2 = 1.0f;
@@ -477,24 +502,8 @@ __global__ __launch_bounds__(256) void k_vote3_prep(const float* __restrict__ in
             continue;
         }
         const size_t v = ((size_t)z * g.s1 + y) * g.s2 + x;
-        float a, b, c, d, e, f;
-        load_tensor(in, layout, v, a, b, c, d, e, f);
-        // power-of-two prescale: exact, makes every threshold relative to |A|
-        const float m = fmaxf(fmaxf(fmaxf(fabsf(a), fabsf(b)), fmaxf(fabsf(c), fabsf(d))), fmaxf(fabsf(e), fabsf(f)));
-        int ex = 0;
-        if (m > 0.0f && isfinite(m)) {
-            frexpf(m, &ex);
-            a = ldexpf(a, -ex); b = ldexpf(b, -ex); c = ldexpf(c, -ex);
-            d = ldexpf(d, -ex); e = ldexpf(e, -ex); f = ldexpf(f, -ex);
-        }
-        float lam[3], e0[3], e2[3];
-        eig3_solve<float>(a, b, c, d, e, f, 1e-30f, lam, e0, e2);
-        if (eig3_near_degenerate(lam[0], lam[1], lam[2])) {
-            double dl[3], de0[3], de2[3];
-            eig3_solve<double>(a, b, c, d, e, f, 1e-30, dl, de0, de2);
-            for (int k = 0; k < 3; k++) { lam[k] = (float)dl[k]; e0[k] = (float)de0[k]; e2[k] = (float)de2[k]; }
-        }
-        const float l0 = ldexpf(lam[0], ex), l1 = ldexpf(lam[1], ex), l2 = ldexpf(lam[2], ex);
+        float l0, l1, l2, e0[3], e2[3];
+        voxel_eig(in, layout, v, l0, l1, l2, e0, e2);
         write_payload(g, p, l0, l1, l2, e0, e2, pay_s, pay_p, pay_v, sum_inv_n);
     }
 }
@@ -521,6 +530,240 @@ __global__ __launch_bounds__(256) void k_vote3_prep_tk(const float* __restrict__
         const float* e0 = ev0 ? ev0 + v * 3 : z3;
         write_payload(g, p, lam[v * 3], lam[v * 3 + 1], lam[v * 3 + 2], e0, e2, pay_s, pay_p, pay_v, sum_inv_n);
     }
+}
+
+// ---------------------------------------------------------------- learned plate (MLP)
+// Payload of the surrogate path, per voter: N = (n, plate scale), U = (u,
+// fallback flag) with u the reference's plate-basis vector (used when an
+// offset runs along n); `flag` marks voters whose orientation bin failed the
+// surrogate's error check -- their plate votes come from the integrator (the
+// closed form of the N-sample sum).
+__global__ __launch_bounds__(256) void k_vote3_prep_mlp(const float* __restrict__ in, int layout, VoteGeom g,
+                                                        float4* __restrict__ pay_s, float4* __restrict__ pay_n,
+                                                        float4* __restrict__ pay_u,
+                                                        const float* __restrict__ bin_err, float tol,
+                                                        unsigned long long* __restrict__ n_fallback) {
+    const size_t total = (size_t)g.P0 * g.P1 * g.P2;
+    const float4 zero = make_float4(0, 0, 0, 0);
+    unsigned nf = 0;
+    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < total; p += (size_t)gridDim.x * blockDim.x) {
+        const int px = (int)(p % g.P2), py = (int)((p / g.P2) % g.P1), pz = (int)(p / ((size_t)g.P1 * g.P2));
+        const int x = px - g.r, y = py - g.r, z = pz - g.r;
+        if (x < 0 || y < 0 || z < 0 || x >= g.s2 || y >= g.s1 || z >= g.s0) {
+            if (pay_s) pay_s[p] = zero;
+            pay_n[p] = zero; pay_u[p] = zero;
+            continue;
+        }
+        const size_t v = ((size_t)z * g.s1 + y) * g.s2 + x;
+        float l0, l1, l2, e0[3], e2[3];
+        voxel_eig(in, layout, v, l0, l1, l2, e0, e2);
+        if (pay_s) pay_s[p] = make_float4(e2[0], e2[1], e2[2], copysignf(fabsf(l2) - fabsf(l1), l2));
+        float n[3] = {e0[0], e0[1], e0[2]};
+        const float nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+        float sp = copysignf(fabsf(l1) - fabsf(l0), l1);
+        if (!(nn > 1e-12f)) { sp = 0.0f; n[0] = n[1] = n[2] = 0.0f; n[2] = 1.0f; }
+        else { const float inv = rsqrtf(nn); n[0] *= inv; n[1] *= inv; n[2] *= inv; }
+        float u[3], w[3];
+        plate_basis(n, u, w);
+        const bool fb = sp != 0.0f && bin_err[mlp_bin(n[0], n[1], n[2])] > tol;
+        nf += fb ? 1u : 0u;
+        pay_n[p] = make_float4(n[0], n[1], n[2], sp);
+        pay_u[p] = make_float4(u[0], u[1], u[2], fb ? 1.0f : 0.0f);
+    }
+    if (n_fallback && nf) atomicAdd(n_fallback, (unsigned long long)nf);
+}
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef short s16x2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ unsigned pack_bf16(float a, float b) {  // v_cvt_pk_bf16_f32, element 0 low
+    const bf16x2_t h = __builtin_convertvector((f32x2_t){a, b}, bf16x2_t);
+    return __builtin_bit_cast(unsigned, h);
+}
+__device__ __forceinline__ unsigned relu_bf16x2(unsigned x) {  // max(x, 0) of two BF16 as 16-bit integers
+    const s16x2_t s = __builtin_elementwise_max(__builtin_bit_cast(s16x2_t, x), (s16x2_t){0, 0});
+    return __builtin_bit_cast(unsigned, s);
+}
+__device__ __forceinline__ bf16x8_t op4(unsigned a, unsigned b, unsigned c, unsigned d) {
+    return __builtin_bit_cast(bf16x8_t, make_uint4(a, b, c, d));
+}
+__device__ __forceinline__ bf16x8_t act8(const f32x16_t& c, int o) {
+    return op4(relu_bf16x2(pack_bf16(c[o + 0], c[o + 1])), relu_bf16x2(pack_bf16(c[o + 2], c[o + 3])),
+               relu_bf16x2(pack_bf16(c[o + 4], c[o + 5])), relu_bf16x2(pack_bf16(c[o + 6], c[o + 7])));
+}
+#define TH_MFMA(a, b, c) __builtin_amdgcn_mfma_f32_32x32x16_bf16((a), (b), (c), 0, 0, 0)
+
+// The 64 pairs of a wave: lanes 0-31 carry pairs 0-31 as batch A, lanes
+// 32-63 pairs 32-63 as batch B. The two batches are staggered by half a
+// layer, so the activations (VALU) of one overlap the matrix products of the
+// other within the wave. Outputs 0-3 of a pair sit in registers 0-3 of
+// lanes 0-31, outputs 4-5 in registers 0-1 of lanes 32-63 (row
+// 8 (r / 4) + 4 (l / 32) + r % 4 of the tile); returns the lane's own pair.
+__device__ __forceinline__ void mlp_round(const bf16x8_t* __restrict__ w, bf16x8_t xa, bf16x8_t xb, bool lo,
+                                          float o[6]) {
+    const f32x16_t z = {};
+    f32x16_t a0 = TH_MFMA(w[0], xa, z), a1 = TH_MFMA(w[1], xa, z);
+    f32x16_t b0 = TH_MFMA(w[0], xb, z), b1 = TH_MFMA(w[1], xb, z);
+    bf16x8_t ha[4] = {act8(a0, 0), act8(a0, 8), act8(a1, 0), act8(a1, 8)};
+#pragma unroll
+    for (int layer = 0; layer < 2; layer++) {
+        const bf16x8_t* wl = w + 2 + 8 * layer;
+        a0 = TH_MFMA(wl[0], ha[0], z);
+        a1 = TH_MFMA(wl[4], ha[0], z);
+#pragma unroll
+        for (int s = 1; s < 4; s++) {
+            a0 = TH_MFMA(wl[s], ha[s], a0);
+            a1 = TH_MFMA(wl[4 + s], ha[s], a1);
+        }
+        const bf16x8_t hb[4] = {act8(b0, 0), act8(b0, 8), act8(b1, 0), act8(b1, 8)};
+        b0 = TH_MFMA(wl[0], hb[0], z);
+        b1 = TH_MFMA(wl[4], hb[0], z);
+#pragma unroll
+        for (int s = 1; s < 4; s++) {
+            b0 = TH_MFMA(wl[s], hb[s], b0);
+            b1 = TH_MFMA(wl[4 + s], hb[s], b1);
+        }
+        ha[0] = act8(a0, 0); ha[1] = act8(a0, 8); ha[2] = act8(a1, 0); ha[3] = act8(a1, 8);
+    }
+    f32x16_t ya = TH_MFMA(w[18], ha[0], z);
+#pragma unroll
+    for (int s = 1; s < 4; s++) ya = TH_MFMA(w[18 + s], ha[s], ya);
+    const bf16x8_t hb[4] = {act8(b0, 0), act8(b0, 8), act8(b1, 0), act8(b1, 8)};
+    f32x16_t yb = TH_MFMA(w[18], hb[0], z);
+#pragma unroll
+    for (int s = 1; s < 4; s++) yb = TH_MFMA(w[18 + s], hb[s], yb);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float t = __shfl_xor(yb[k], 32);
+        o[k] = lo ? ya[k] : t;
+    }
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const float t = __shfl_xor(ya[k], 32);
+        o[4 + k] = lo ? t : yb[k];
+    }
+}
+#undef TH_MFMA
+
+// Rotates the local-frame vote s * [o] (uu, uv, un, vv, vn, nn) to the field
+// frame, R [o] R^T with R = [u v n], and accumulates it.
+__device__ __forceinline__ void mlp_accumulate(Acc& a, const float o[6], float s, const float4& U, const float4& V,
+                                               const float4& N) {
+    const float o0 = s * o[0], o1 = s * o[1], o2 = s * o[2], o3 = s * o[3], o4 = s * o[4], o5 = s * o[5];
+    const float u[3] = {U.x, U.y, U.z}, v[3] = {V.x, V.y, V.z}, n[3] = {N.x, N.y, N.z};
+    float zz[3][3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        zz[j][0] = fmaf(o0, u[j], fmaf(o1, v[j], o2 * n[j]));
+        zz[j][1] = fmaf(o1, u[j], fmaf(o3, v[j], o4 * n[j]));
+        zz[j][2] = fmaf(o2, u[j], fmaf(o4, v[j], o5 * n[j]));
+    }
+    a.xx = fmaf(u[0], zz[0][0], fmaf(v[0], zz[0][1], fmaf(n[0], zz[0][2], a.xx)));
+    a.xy = fmaf(u[0], zz[1][0], fmaf(v[0], zz[1][1], fmaf(n[0], zz[1][2], a.xy)));
+    a.xz = fmaf(u[0], zz[2][0], fmaf(v[0], zz[2][1], fmaf(n[0], zz[2][2], a.xz)));
+    a.yy = fmaf(u[1], zz[1][0], fmaf(v[1], zz[1][1], fmaf(n[1], zz[1][2], a.yy)));
+    a.yz = fmaf(u[1], zz[2][0], fmaf(v[1], zz[2][1], fmaf(n[1], zz[2][2], a.yz)));
+    a.zz = fmaf(u[2], zz[2][0], fmaf(v[2], zz[2][1], fmaf(n[2], zz[2][2], a.zz)));
+}
+
+// Stick (closed table) + learned plate gather: a wave owns 64 consecutive x
+// receivers of one (y, z) row and walks every window offset; each offset is
+// one network round over the wave's 64 (receiver, voter) pairs, in the plate
+// frame turned about n so that the offset lies in the (u, n) half-plane (the
+// frame the network was trained in). The self offset and voters flagged by
+// the error check take the closed-form plate.
+// mtab per offset: (d, c1 / pi), (|d| / sigma1 clamped, 0, 0, 0); the network's
+// standardized outputs are mapped back with the FP32 affine `aff`.
+template <int P, bool STICK>
+__global__ __launch_bounds__(kThreads) void k_vote3_mlp(const float4* __restrict__ pay_s,
+                                                        const float4* __restrict__ pay_n,
+                                                        const float4* __restrict__ pay_u,
+                                                        const float* __restrict__ ctab,
+                                                        const float4* __restrict__ mtab,
+                                                        const uint4* __restrict__ wreg, MlpAffine aff,
+                                                        VoteGeom g, float* __restrict__ out, int out_layout) {
+    constexpr int CS = (4 * P + 13 + 3) & ~3;
+    const int lane = threadIdx.x & 63;
+    bf16x8_t w[kMlpWreg];
+#pragma unroll
+    for (int q = 0; q < kMlpWreg; q++) w[q] = __builtin_bit_cast(bf16x8_t, wreg[q * 64 + lane]);
+    unsigned bx, by, bz;
+    tile_coords(g, 1, bx, by, bz);
+    const int x = (int)bx * kTileX + lane;
+    const int y = (int)by * kTileY + (int)(threadIdx.x / kTileX);
+    const int z = (int)bz;
+    const int r = g.r, wd = 2 * g.r + 1;
+    const bool lo = lane < 32;
+    const unsigned one_zero = pack_bf16(1.0f, 0.0f);
+    const size_t plane = (size_t)g.P1 * g.P2;
+    const size_t base = ((size_t)z * g.P1 + (size_t)y) * g.P2 + (size_t)x;  // voter of offset (-r, -r, -r)
+    const int nk = wd * wd * wd, kself = (nk - 1) / 2;
+    Acc acc = Acc{0, 0, 0, 0, 0, 0, 0};
+    // payload of the next offset is loaded one round ahead
+    int du = 0, dv = 0, dw = 0;  // offset + r of the next load
+    float4 Sn = make_float4(0, 0, 0, 0), Nn, Un;
+    if (STICK) Sn = pay_s[base];
+    Nn = pay_n[base];
+    Un = pay_u[base];
+    for (int k = 0; k < nk; k++) {
+        const float4 S = Sn, N = Nn, U = Un;
+        if (++du == wd) {
+            du = 0;
+            if (++dv == wd) { dv = 0; ++dw; }
+        }
+        if (k + 1 < nk) {
+            const size_t vi = base + (size_t)dw * plane + (size_t)dv * g.P2 + (size_t)du;
+            if (STICK) Sn = pay_s[vi];
+            Nn = pay_n[vi];
+            Un = pay_u[vi];
+        }
+        const float* e = ctab + (size_t)k * CS;
+        if (STICK) stick_pair<P>(acc, S, e);
+        if (k == kself) {
+            plate_pair<P>(acc, N, e);
+            continue;
+        }
+        const float4 m0 = mtab[2 * k], m1 = mtab[2 * k + 1];
+        // offset-aligned plate frame: u along the in-plane part of d
+        const float f2 = fmaf(m0.x, N.x, fmaf(m0.y, N.y, m0.z * N.z));
+        float4 Ua, Va;
+        {
+            const float w0 = fmaf(-f2, N.x, m0.x), w1 = fmaf(-f2, N.y, m0.y), w2 = fmaf(-f2, N.z, m0.z);
+            const float ww = fmaf(w0, w0, fmaf(w1, w1, w2 * w2));
+            if (ww > 1e-12f) {
+                const float iw = rsqrtf(ww);
+                Ua = make_float4(w0 * iw, w1 * iw, w2 * iw, 0.0f);
+            } else {
+                Ua = U;
+            }
+            Va = make_float4(N.y * Ua.z - N.z * Ua.y, N.z * Ua.x - N.x * Ua.z, N.x * Ua.y - N.y * Ua.x, 0.0f);
+        }
+        const float f0 = fmaf(m0.x, Ua.x, fmaf(m0.y, Ua.y, m0.z * Ua.z));
+        const float f1 = fmaf(m0.x, Va.x, fmaf(m0.y, Va.y, m0.z * Va.z));
+        const float f4 = fmaf(2.0f * f2, f2, -1.0f);
+        // BF16 high parts and residuals (input slots 0-4, 5-9), constant slot 10
+        const unsigned h01 = pack_bf16(f0, f1), h23 = pack_bf16(f2, m1.x);
+        const float r0 = f0 - __uint_as_float(h01 << 16), r1 = f1 - __uint_as_float(h01 & 0xffff0000u);
+        const float r2 = f2 - __uint_as_float(h23 << 16), r3 = m1.x - __uint_as_float(h23 & 0xffff0000u);
+        const unsigned h4l0 = pack_bf16(f4, r0);
+        const float r4 = f4 - __uint_as_float(h4l0 << 16);
+        const unsigned l12 = pack_bf16(r1, r2), l34 = pack_bf16(r3, r4);
+        // slots 0-7 of the lane's pair, and slots 8-15 of the partner lane's
+        const unsigned s0 = __shfl_xor(h01, 32), s1 = __shfl_xor(h23, 32), s2 = __shfl_xor(h4l0, 32),
+                       s3 = __shfl_xor(l12, 32), s4 = __shfl_xor(l34, 32);
+        const bf16x8_t xa = lo ? op4(h01, h23, h4l0, l12) : op4(s4, one_zero, 0u, 0u);
+        const bf16x8_t xb = lo ? op4(s0, s1, s2, s3) : op4(l34, one_zero, 0u, 0u);
+        float o[6];
+        mlp_round(w, xa, xb, lo, o);
+#pragma unroll
+        for (int q = 0; q < 6; q++) o[q] = fmaf(aff.sd[q], o[q], aff.mu[q]);
+        if (U.w != 0.0f) plate_pair<P>(acc, N, e);
+        else mlp_accumulate(acc, o, N.w * m0.w, Ua, Va, N);
+    }
+    if (x < g.s2 && y < g.s1 && z < g.s0) write_out(out, out_layout, ((size_t)z * g.s1 + y) * g.s2 + x, acc);
 }
 
 __global__ void k_fill_zero(float* __restrict__ out, size_t n) {
@@ -685,6 +928,114 @@ int vote3_run(const float* tensors, int in_layout, const float* lam, const float
     return TIRA_OK;
 }
 
+// ---------------------------------------------------------------- learned plate driver
+// Per-offset table of the surrogate: (d, c1 / pi), (min(|d| / sigma1, 3), 0, 0, 0),
+// d and c1 as cpu::build_neighbors3d computes them.
+static void build_mlp_table(int w, float s1, float s2, std::vector<float4>& t) {
+    const int hw = w / 2;
+    t.assign(2 * (size_t)w * w * w, make_float4(0, 0, 0, 0));
+    size_t k = 0;
+    for (int dw = -hw; dw <= hw; dw++)
+        for (int dv = -hw; dv <= hw; dv++)
+            for (int du = -hw; du <= hw; du++, k++) {
+                float d[3], c1, c2;
+                offset_geometry(du, dv, dw, s1, s2, d, c1, c2);
+                const float L = std::sqrt((float)(du * du + dv * dv + dw * dw));
+                t[2 * k] = make_float4(d[0], d[1], d[2], (float)(c1 / M_PI));
+                t[2 * k + 1] = make_float4(std::min(L / s1, kMlpRhoMax), 0.0f, 0.0f, 0.0f);
+            }
+}
+
+struct MlpTabCache {
+    int w = 0;
+    float s1 = 0, s2 = 0;
+    int device = -1;
+    void* ptr = nullptr;
+};
+static thread_local MlpTabCache g_mtab;
+
+static int mlp_table_for(int w, float s1, float s2, hipStream_t st, float4** dtab) {
+    int dev = 0;
+    TH_HIP(hipGetDevice(&dev));
+    const size_t bytes = 2 * (size_t)w * w * w * 16;
+    void* buf = nullptr;
+    TH_TRY(ws_get(WS_TABLE3, bytes, &buf));
+    *dtab = (float4*)buf;
+    if (g_mtab.w == w && g_mtab.s1 == s1 && g_mtab.s2 == s2 && g_mtab.device == dev && g_mtab.ptr == buf)
+        return TIRA_OK;
+    std::vector<float4> t;
+    build_mlp_table(w, s1, s2, t);
+    TH_HIP(hipStreamSynchronize(st));
+    TH_HIP(hipMemcpy(buf, t.data(), bytes, hipMemcpyHostToDevice));
+    g_mtab.w = w; g_mtab.s1 = s1; g_mtab.s2 = s2; g_mtab.device = dev; g_mtab.ptr = buf;
+    return TIRA_OK;
+}
+
+template <int P>
+static void launch_mlp(bool stick, unsigned grid, hipStream_t st, const float4* ps, const float4* pn,
+                       const float4* pu, const float* ct, const float4* mt, const uint4* wr, const MlpAffine& aff,
+                       const VoteGeom& g, float* out, int layout) {
+    if (stick) k_vote3_mlp<P, true><<<grid, kThreads, 0, st>>>(ps, pn, pu, ct, mt, wr, aff, g, out, layout);
+    else k_vote3_mlp<P, false><<<grid, kThreads, 0, st>>>(ps, pn, pu, ct, mt, wr, aff, g, out, layout);
+}
+
+// Stick + learned plate vote on device buffers with the network `m` (trained
+// for its sigma1, sigma2 and power). Voters whose orientation bin's error
+// exceeds `tol` use the integrator. n_fallback (host, optional) receives the
+// number of such voters (synchronises).
+int vote3_mlp_run(PlateMlp* m, const float* tensors, int in_layout, float* out, int out_layout, unsigned s0,
+                  unsigned s1, unsigned s2, unsigned w, bool stick, float tol, unsigned long long* n_fallback,
+                  hipStream_t st) {
+    const size_t nvox = (size_t)s0 * s1 * s2;
+    if (nvox == 0) return TIRA_OK;
+    const int P = (int)m->power;
+    if (P > 4) {
+        set_error("the learned plate vote supports power 1..4 (network trained for %d)", P);
+        return TIRA_ERR_UNSUPPORTED;
+    }
+    const VoteGeom g = make_geom(s0, s1, s2, w, 1);
+    const size_t ptotal = (size_t)g.P0 * g.P1 * g.P2;
+    float4 *ps = nullptr, *pn = nullptr, *pu = nullptr;
+    if (stick) TH_TRY(ws_get(WS_PAY0, ptotal * 16, (void**)&ps));
+    TH_TRY(ws_get(WS_PAY1, ptotal * 16, (void**)&pn));
+    TH_TRY(ws_get(WS_PAY2, ptotal * 16, (void**)&pu));
+    unsigned long long* cnt = nullptr;
+    if (n_fallback) {
+        TH_TRY(ws_get(WS_COUNTER, 8, (void**)&cnt));
+        k_fill_zero<<<1, 64, 0, st>>>((float*)cnt, 2);
+    }
+    const unsigned pgrid = (unsigned)std::min<size_t>(ceil_div(ptotal, 256), 65536);
+    {
+        ProfScope prof("vote3_prep_mlp", st);
+        k_vote3_prep_mlp<<<pgrid, 256, 0, st>>>(tensors, in_layout, g, ps, pn, pu, m->bin_err, tol, cnt);
+    }
+    TH_HIP(hipGetLastError());
+    const TableKey key{0, (int)w, m->sigma1, m->sigma2, m->power, 0u};
+    char* dtab = nullptr;
+    float2* dcs = nullptr;
+    TH_TRY(table_for(key, P, st, &dtab, &dcs));
+    float4* mt = nullptr;
+    TH_TRY(mlp_table_for((int)w, m->sigma1, m->sigma2, st, &mt));
+    const unsigned grid = g.nbx * g.nby * g.nbz;
+    const float* ct = (const float*)dtab;
+    const uint4* wr = (const uint4*)m->wreg;
+    {
+        ProfScope prof("vote3_mlp", st);
+        switch (P) {
+            case 1: launch_mlp<1>(stick, grid, st, ps, pn, pu, ct, mt, wr, m->aff, g, out, out_layout); break;
+            case 2: launch_mlp<2>(stick, grid, st, ps, pn, pu, ct, mt, wr, m->aff, g, out, out_layout); break;
+            case 3: launch_mlp<3>(stick, grid, st, ps, pn, pu, ct, mt, wr, m->aff, g, out, out_layout); break;
+            default: launch_mlp<4>(stick, grid, st, ps, pn, pu, ct, mt, wr, m->aff, g, out, out_layout); break;
+        }
+    }
+    TH_HIP(hipGetLastError());
+    if (n_fallback) {
+        TH_HIP(hipMemcpyAsync(n_fallback, cnt, 8, hipMemcpyDeviceToHost, st));
+        TH_HIP(hipStreamSynchronize(st));
+    }
+    return TIRA_OK;
+}
+
 static int check_vote_args(unsigned s0, unsigned s1, unsigned s2, unsigned w, float sigma1, float sigma2) {
     TH_ARG(w >= 1 && (w % 2) == 1, "w must be a positive odd integer (got %u)", w);
     TH_ARG(sigma1 >= 0.0f && sigma2 >= 0.0f, "sigma must be >= 0");
@@ -735,6 +1086,24 @@ int tira_tensorvote3_tk(float* output_field, const float* lambdas, const float* 
     TH_TRY(vote3_run(nullptr, 0, dl, d2, d0, (float*)dout, TIRA_LAYOUT_MAT9, s0, s1, s2, sigma1, sigma2, w, power,
                      stick != 0, plate != 0, samples, TIRA_PLATE_AUTO, stream()));
     return finish_out(output_field, dout, n * 36);
+}
+
+int tira_tensorvote3_mlp(void* mlp, const float* input_field, int in_layout, float* output_field, int out_layout,
+                         unsigned s0, unsigned s1, unsigned s2, unsigned w, int stick, float tol,
+                         unsigned long long* n_fallback) {
+    TH_ARG(mlp && input_field && output_field, "NULL pointer");
+    PlateMlp* m = static_cast<PlateMlp*>(mlp);
+    int dev = 0;
+    TH_HIP(hipGetDevice(&dev));
+    TH_ARG(dev == m->device, "MLP handle belongs to device %d (current %d)", m->device, dev);
+    TH_ARG(m->wreg != nullptr, "the plate network has no weights yet (train or set its parameters)");
+    TH_ARG(in_layout == TIRA_LAYOUT_MAT9 || in_layout == TIRA_LAYOUT_PACKED6, "bad input layout");
+    TH_ARG(out_layout == TIRA_LAYOUT_MAT9 || out_layout == TIRA_LAYOUT_PACKED6, "bad output layout");
+    TH_ARG(tol >= 0.0f, "tolerance must be >= 0");
+    TH_ARG(is_device(input_field) && is_device(output_field), "the learned plate vote takes device pointers");
+    TH_TRY(check_vote_args(s0, s1, s2, w, m->sigma1, m->sigma2));
+    return vote3_mlp_run(m, input_field, in_layout, output_field, out_layout, s0, s1, s2, w, stick != 0, tol,
+                         n_fallback, stream());
 }
 
 int tira_tensorvote3_async(const float* input_field, int in_layout, float* output_field, int out_layout,
