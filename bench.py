@@ -161,6 +161,144 @@ def bench_batch(steps, warmup, check=True):
     return out
 
 
+def _frob6(x):
+    """Frobenius norm of packed-6 symmetric tensors (off-diagonals count twice)."""
+    x = np.asarray(x, np.float64).reshape(-1, 6)
+    return float(np.sqrt((x[:, [0, 3, 5]] ** 2).sum() + 2 * (x[:, [1, 2, 4]] ** 2).sum()))
+
+
+def bench_mlp(steps, check=True, log=None):
+    """Learned plate surrogate on the 256^3 curved-sheet volume (12 sheets, 2%
+    plate noise, seed 1004), sigma 4, w 25: the BF16 MLP trained on the
+    device (4M samples from the FP64 64-sample integrator), then stick + plate
+    voting with the MLP plate (per-voter error check, integrator fallback)
+    against the GPU integrated plate kernel (literal 20- and 64-sample sums)
+    and the closed form. Errors against the FP64 64-sample oracle on sampled
+    receivers. Returns the JSON sub-object."""
+    from tiralib_amd import _lib, api, synth
+    from tiralib_amd._lib import DeviceArray, Event, Stream, call
+    say = log or (lambda *a: None)
+    shape, sigma, w, tol = (256, 256, 256), 4.0, 25, 4e-3
+    n = int(np.prod(shape))
+    vol = synth.curved_sheets_volume_device(shape, 12, 0.02, seed=synth.SEED_CFG4)
+    net = api.PlateMLP(sigma, 0.0, 1, 64, seed=synth.SEED_CFG4)
+    t = time.perf_counter()
+    tr, va = net.train(nsamples=1 << 22, epochs=100, batch=8192, lr=1e-2, lr_final=1e-5, seed=synth.SEED_CFG4)
+    train_s = time.perf_counter() - t
+    be = net.bin_errors()
+    say(f"mlp: trained in {train_s:.1f} s, validation error {va:.2e}")
+    outs = {k: DeviceArray((n, 6)) for k in ("mlp", "closed", "sum20", "sum64")}
+    nfall = net.vote(vol, outs["mlp"], shape, w, stick=True, tol=tol, in_layout=_lib.LAYOUT_PACKED6,
+                     count_fallback=True)
+    st = Stream().use()
+    e0, e1 = Event(), Event()
+
+    def timed(fn, reps):
+        e0.record(st)
+        for _ in range(reps):
+            fn()
+        e1.record(st)
+        st.synchronize()
+        return e0.elapsed_ms(e1) / reps
+
+    def trad(mode, samples, out):
+        call("tira_tensorvote3_async", vol.ptr, _lib.LAYOUT_PACKED6, out.ptr, _lib.LAYOUT_PACKED6, *shape, sigma,
+             0.0, w, 1, 1, 1, samples, mode)
+
+    ms = {}
+    ms["mlp"] = timed(lambda: net.vote(vol, outs["mlp"], shape, w, stick=True, tol=tol,
+                                       in_layout=_lib.LAYOUT_PACKED6), steps)
+    say(f"mlp: {ms['mlp']:.0f} ms per vote")
+    trad(_lib.PLATE_CLOSED, 20, outs["closed"])
+    ms["closed"] = timed(lambda: trad(_lib.PLATE_CLOSED, 20, outs["closed"]), steps)
+    ms["sum20"] = timed(lambda: trad(_lib.PLATE_SUM, 20, outs["sum20"]), 1)
+    say(f"integrated plate N=20: {ms['sum20']:.0f} ms")
+    ms["sum64"] = timed(lambda: trad(_lib.PLATE_SUM, 64, outs["sum64"]), 1)
+    say(f"integrated plate N=64: {ms['sum64']:.0f} ms")
+    pairs = in_bounds_pairs(shape, w)
+    res = {
+        "workload": "stick + plate voting, 256^3 curved-sheet volume (12 sheets, 2% plate noise, seed 1004), "
+                    "sigma=4, w=25, p=1; MLP 5->64->64->64->6 BF16 on MFMA trained on 4M FP64 64-sample "
+                    "integrator samples",
+        "pairs_per_step": pairs,
+        "mlp": {"ms": round(ms["mlp"], 2), "mvoxel_s": round(n / (ms["mlp"] * 1e-3) / 1e6, 3),
+                "mfma_tflops_algorithmic": round(pairs * 17792 / (ms["mlp"] * 1e-3) / 1e12, 1),
+                "mfma_frac_of_bf16_peak": round(pairs * 17792 / (ms["mlp"] * 1e-3) / 2.5e15, 4),
+                "fallback_voters": nfall, "error_check_tol": tol,
+                "train_s": round(train_s, 2), "train_val_rel_err": va,
+                "bin_err_median": float(np.median(be)), "bin_err_max": float(be.max())},
+        "integrated_sum20": {"ms": round(ms["sum20"], 2), "mvoxel_s": round(n / (ms["sum20"] * 1e-3) / 1e6, 3)},
+        "integrated_sum64": {"ms": round(ms["sum64"], 2), "mvoxel_s": round(n / (ms["sum64"] * 1e-3) / 1e6, 3)},
+        "closed_form": {"ms": round(ms["closed"], 2), "mvoxel_s": round(n / (ms["closed"] * 1e-3) / 1e6, 3)},
+        "speedup_mlp_vs_sum20": round(ms["sum20"] / ms["mlp"], 3),
+        "speedup_mlp_vs_sum64": round(ms["sum64"] / ms["mlp"], 3),
+        "speedup_closed_vs_mlp": round(ms["mlp"] / ms["closed"], 3),
+    }
+    if check:
+        from oracle import oracle as O
+        # FP64 oracle on a 48^3 crop whose 24^3 core holds the receivers (their
+        # w = 25 windows lie inside the crop)
+        full = vol.to_host()
+        occ = np.abs(full).sum(-1) > 0
+        r = w // 2
+        best, c0 = -1, None
+        for z in range(r + 12, shape[0] - r - 12, 16):
+            for y in range(r + 12, shape[1] - r - 12, 16):
+                for x in range(r + 12, shape[2] - r - 12, 16):
+                    k = int(occ[z - 12:z + 12, y - 12:y + 12, x - 12:x + 12].sum())
+                    if k > best:
+                        best, c0 = k, (z, y, x)
+        z, y, x = c0
+        crop = full[z - 24:z + 24, y - 24:y + 24, x - 24:x + 24]
+        rng = np.random.default_rng(7)
+        core = np.argwhere(np.abs(crop[12:36, 12:36, 12:36]).sum(-1) > 0) + 12
+        pick = core[rng.choice(len(core), min(48, len(core)), replace=False)]
+        pick = np.concatenate([pick, rng.integers(12, 36, (16, 3))])
+        recv = (pick[:, 0] * 48 + pick[:, 1]) * 48 + pick[:, 2]
+        t = time.perf_counter()
+        ref9 = O.tensorvote3_f64(synth.mat9_from_packed6(crop).reshape(48, 48, 48, 9), sigma, 0.0, 1, w, samples=64,
+                                 recv=recv,
+                                 nthreads=min(16, os.cpu_count() or 1))
+        ref = ref9[:, [0, 1, 2, 4, 5, 8]]
+        gidx = ((pick[:, 0] + z - 24) * shape[1] + (pick[:, 1] + y - 24)) * shape[2] + (pick[:, 2] + x - 24)
+        for k in ("mlp", "closed", "sum20", "sum64"):
+            got = outs[k].to_host().reshape(-1, 6)[gidx]
+            key = {"mlp": "mlp", "closed": "closed_form", "sum20": "integrated_sum20",
+                   "sum64": "integrated_sum64"}[k]
+            res[key]["rel_frob_err_vs_fp64_oracle"] = _frob6(got - ref) / _frob6(ref)
+        res["error_check_sample"] = (f"{len(recv)} receivers (48 on sheets, 16 uniform) in the 24^3 core of a 48^3 "
+                                     f"crop; FP64 64-sample oracle, {time.perf_counter() - t:.1f} s")
+    net.close()
+    return res
+
+
+def calibrate():
+    """Achievable rates on this GPU: FP32 FMA loop (unpacked VALU) and a
+    16-byte-vector device copy (HBM), each timed with HIP events."""
+    from tiralib_amd._lib import DeviceArray, Event, Stream, call
+    st = Stream().use()
+    e0, e1 = Event(), Event()
+    blocks, iters = 256 * 64, 4096
+    out = DeviceArray(blocks)
+    call("tira_hip_fma_loop_async", out.ptr, blocks, iters)
+    e0.record(st)
+    call("tira_hip_fma_loop_async", out.ptr, blocks, iters)
+    e1.record(st)
+    st.synchronize()
+    fma_tf = blocks * 256 * 16.0 * iters / (e0.elapsed_ms(e1) * 1e-3) / 1e12
+    nb = 1 << 30
+    a, b = DeviceArray(nb // 4), DeviceArray(nb // 4)
+    call("tira_hip_copy_async", b.ptr, a.ptr, nb)
+    e0.record(st)
+    for _ in range(5):
+        call("tira_hip_copy_async", b.ptr, a.ptr, nb)
+    e1.record(st)
+    st.synchronize()
+    copy_gbs = 5 * 2.0 * nb / (e0.elapsed_ms(e1) * 1e-3) / 1e9
+    return {"fp32_fma_loop_tflops": round(fma_tf, 1), "copy_gbs": round(copy_gbs, 1),
+            "note": "measured here: FP32 FMA loop without packed FP32, 1 GiB device copy"}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,6 +308,7 @@ def main(argv=None):
     ap.add_argument("--plate-mode", choices=["auto", "sum", "closed"], default="auto")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-mlp", action="store_true")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); the vote shards by z-slabs with an "
@@ -265,8 +404,12 @@ def main(argv=None):
         "stages_ms": {"vote3_prep": round(prep_ms, 4), vote_name: round(vote_ms, 4), "eig3": round(eig_ms, 4)},
         "wall_s_timed": round(wall, 3),
     }
+    out["calibration"] = calibrate()
     if not a.no_batch:
         out["eig3_batch"] = bench_batch(max(3, a.steps // 2), max(1, a.warmup))
+        print("bench: batched eig3 + Gaussian done", file=sys.stderr, flush=True)
+    if not a.no_mlp:
+        out["mlp_plate"] = bench_mlp(2, log=lambda m: print("bench: " + m, file=sys.stderr, flush=True))
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(field, sigma, w, samples, a.cpu_seconds)
     print(json.dumps(out))

@@ -51,7 +51,7 @@ typedef enum tira_status {
 #define TIRA_PLATE_AUTO    0   /* closed form when samples >= power+2 (the Riemann sum is then exact), else literal sum */
 #define TIRA_PLATE_SUM     1   /* literal N-sample Riemann sum, the reference's algorithm, any N */
 #define TIRA_PLATE_CLOSED  2   /* closed form of the same integral (exact for N >= p+2 and for samples == 0) */
-#define TIRA_PLATE_MLP     3   /* learned surrogate with per-voxel integrator fallback (tira_mlp_*) */
+#define TIRA_PLATE_MLP     3   /* learned surrogate with per-voxel integrator fallback: served by tira_tensorvote3_mlp */
 
 /* eig3 flags */
 #define TIRA_EIG_REFINE_FP64   1u  /* FP64 re-solve of voxels with relative eigengap < 1e-4 */
@@ -63,6 +63,8 @@ const char* tira_hip_last_error(void);
 int tira_hip_device_count(int* count);
 int tira_hip_set_device(int device);
 int tira_hip_malloc(void** ptr, size_t bytes);
+/* *is_device = 1 for device (or managed) memory, 0 for host memory. */
+int tira_hip_pointer_is_device(const void* ptr, int* is_device);
 int tira_hip_free(void* ptr);
 /* kind: 0 = infer from pointer attributes */
 int tira_hip_memcpy(void* dst, const void* src, size_t bytes);
@@ -86,6 +88,10 @@ int tira_hip_profile_get(const char* kernel_name, double* total_ms, long* count)
 /* Plain device-to-device copy kernel (bytes a multiple of 16): the reference
  * for bandwidth and profiler-counter calibration. Non-blocking. */
 int tira_hip_copy_async(void* dst, const void* src, size_t bytes);
+/* FP32 FMA loop (blocks x 256 threads x 16 x iters FLOP): the achievable
+ * unpacked FP32 VALU rate. `out` (device, blocks floats) is written only in
+ * a case that does not occur; it keeps the arithmetic live. Non-blocking. */
+int tira_hip_fma_loop_async(float* out, unsigned blocks, int iters);
 /* Release every cached device buffer held by the library on the current device. */
 int tira_hip_release_workspace(void);
 
@@ -212,6 +218,13 @@ int tira_batch_gauss_eig3_destroy(void* handle);
  * lambdas (optional, n x 3) receives the construction eigenvalues. */
 int tira_synth_eig_tensors_async(float* out6, size_t n, unsigned long long seed, float lo, float hi,
                                  float degenerate_frac, float gap_lo, float gap_hi, float* lambdas);
+/* Packed-6 plate volume s0 x s1 x s2 (device): n_sheets curved sheets, each
+ * 15 floats of `params` (host): R (3x3 row-major; columns = local axes),
+ * centre (x, y, z), curvatures (a, b, c) of q_z = a q_x^2 + b q_y^2 + c q_x q_y;
+ * thickness 1, patch radius `patch`, tensor I - n n^T; elsewhere plate noise
+ * m (I - q q^T) on a fraction `noise_frac` of the voxels. Non-blocking. */
+int tira_synth_sheets_async(float* out6, unsigned s0, unsigned s1, unsigned s2, int n_sheets, const float* params,
+                            float patch, float noise_frac, unsigned long long seed);
 
 /* ------------------------------------------------------------------ learned plate vote
  * No reference counterpart: a BF16 MLP (5 -> 64 -> 64 -> 64 -> 6, MFMA, FP32

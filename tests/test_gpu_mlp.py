@@ -132,3 +132,79 @@ def test_vote_kernel_rounds_equal_host_network(hip, seed):
     outside = np.ones((S, S, S), bool)
     outside[c - r:c + r + 1, c - r:c + r + 1, c - r:c + r + 1] = False
     assert not got[outside].any()
+
+
+def test_sheet_volume_generator(hip):
+    from tiralib_amd import synth
+    v = synth.curved_sheets_volume_device((40, 36, 48), 3, 0.02, seed=9).to_host().reshape(-1, 6)
+    v2 = synth.curved_sheets_volume_device((40, 36, 48), 3, 0.02, seed=9).to_host().reshape(-1, 6)
+    assert np.array_equal(v, v2)
+    nz = np.abs(v).sum(1) > 0
+    assert 0.02 <= nz.mean() <= 0.2
+    M9 = synth.mat9_from_packed6(v[nz]).reshape(-1, 3, 3).astype(np.float64)
+    ev = np.linalg.eigvalsh(M9)
+    # every tensor is a plate m (I - n n^T): eigenvalues 0, m, m
+    assert np.allclose(ev[:, 0], 0, atol=1e-6) and np.allclose(ev[:, 1], ev[:, 2], rtol=1e-5, atol=1e-6)
+
+
+def test_trained_network_gain_and_fallback(hip, oracle):
+    """Training takes the network from useless to a vote within 1e-3 of the
+    FP64 integrator; the per-voter check routes every voter to the integrator
+    at tolerance 0 (closed form, bit-for-bit the integrator kernel's result);
+    and the trained network votes faster than the 64-sample integrator."""
+    from tiralib_amd import synth
+    sigma, w, shape = 2.0, 13, (32, 40, 48)
+    net = api.PlateMLP(sigma, 0.0, 1, 64, seed=5)
+    X, T, _ = net.samples(65536, seed=123)
+    t = T.to_host()[:, :6]
+
+    def rel_err():
+        d = net.forward(X).to_host()[:, :6] - t
+        return float(np.sqrt((M.COMP_W * d * d).sum() / (M.COMP_W * t * t).sum()))
+
+    e0 = rel_err()
+    tr, va = net.train(nsamples=1 << 20, epochs=40, batch=8192, lr=1e-2, lr_final=1e-5, seed=6)
+    e1 = rel_err()
+    print("untrained", e0, "trained", e1, "validation", va)
+    assert e1 < 0.01 * e0 and e1 < 5e-3
+    vol = synth.curved_sheets_volume_device(shape, 4, 0.02, seed=10)
+    n = int(np.prod(shape))
+    o_m, o_c, o_f = DeviceArray((n, 6)), DeviceArray((n, 6)), DeviceArray((n, 6))
+    nf = net.vote(vol, o_m, shape, w, stick=True, tol=1e30, in_layout=_lib.LAYOUT_PACKED6, count_fallback=True)
+    assert nf == 0
+    nf0 = net.vote(vol, o_f, shape, w, stick=True, tol=0.0, in_layout=_lib.LAYOUT_PACKED6, count_fallback=True)
+    _lib.call("tira_tensorvote3_async", vol.ptr, _lib.LAYOUT_PACKED6, o_c.ptr, _lib.LAYOUT_PACKED6, *shape, sigma,
+              0.0, w, 1, 1, 1, 64, _lib.PLATE_CLOSED)
+    _lib.call("tira_hip_synchronize")
+    host = vol.to_host()
+    assert nf0 == int((np.abs(host).sum(-1) > 0).sum())
+    fb, cl = o_f.to_host(), o_c.to_host()
+    assert _rel(fb, cl) <= 1e-6  # same closed-form plate, different summation order
+    rng = np.random.default_rng(1)
+    recv = rng.choice(n, 96, replace=False)
+    ref = oracle.tensorvote3_f64(synth.mat9_from_packed6(host).reshape(shape + (9,)), sigma, 0.0, 1, w, samples=64,
+                                recv=recv, nthreads=8)
+    ref6 = ref[:, [0, 1, 2, 4, 5, 8]]
+    got = o_m.to_host().reshape(-1, 6)[recv]
+    wgt = np.array([1, 2, 2, 1, 2, 1.0])
+    err = np.sqrt((wgt * (got - ref6) ** 2).sum() / (wgt * ref6 ** 2).sum())
+    print("MLP vote vs FP64 oracle", err)
+    assert err <= 1e-3
+    st = _lib.Stream().use()
+    e_0, e_1 = _lib.Event(), _lib.Event()
+
+    def ms(fn):
+        fn()
+        st.synchronize()
+        e_0.record(st)
+        fn()
+        e_1.record(st)
+        st.synchronize()
+        return e_0.elapsed_ms(e_1)
+
+    t_mlp = ms(lambda: net.vote(vol, o_m, shape, w, stick=True, tol=1e30, in_layout=_lib.LAYOUT_PACKED6))
+    t_sum = ms(lambda: _lib.call("tira_tensorvote3_async", vol.ptr, _lib.LAYOUT_PACKED6, o_c.ptr,
+                                 _lib.LAYOUT_PACKED6, *shape, sigma, 0.0, w, 1, 1, 1, 64, _lib.PLATE_SUM))
+    print("ms mlp", t_mlp, "ms integrated N=64", t_sum)
+    assert t_mlp < t_sum
+    net.close()

@@ -61,6 +61,7 @@ SIGNATURES = {
     "tira_hip_device_count": [C.POINTER(_i)],
     "tira_hip_set_device": [_i],
     "tira_hip_malloc": [C.POINTER(_vp), _sz],
+    "tira_hip_pointer_is_device": [_vp, C.POINTER(_i)],
     "tira_hip_free": [_vp],
     "tira_hip_memcpy": [_vp, _vp, _sz],
     "tira_hip_memset": [_vp, _i, _sz],
@@ -75,6 +76,7 @@ SIGNATURES = {
     "tira_hip_event_elapsed_ms": [_vp, _vp, C.POINTER(_f)],
     "tira_hip_release_workspace": [],
     "tira_hip_copy_async": [_vp, _vp, _sz],
+    "tira_hip_fma_loop_async": [_vp, _u, _i],
     "tira_hip_profile_begin": [],
     "tira_hip_profile_end": [],
     "tira_hip_profile_get": [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_long)],
@@ -100,6 +102,7 @@ SIGNATURES = {
     "tira_batch_gauss_eig3_plan": [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_f), C.POINTER(_u)],
     "tira_batch_gauss_eig3_destroy": [_vp],
     "tira_synth_eig_tensors_async": [_vp, _sz, C.c_ulonglong, _f, _f, _f, _f, _f, _vp],
+    "tira_synth_sheets_async": [_vp, _u, _u, _u, _i, _vp, _f, _f, C.c_ulonglong],
     "tira_plate_mlp_create": [C.POINTER(_vp), _f, _f, _u, _u, C.c_ulonglong],
     "tira_plate_mlp_destroy": [_vp],
     "tira_plate_mlp_train": [_vp, _sz, _u, _u, _f, _f, C.c_ulonglong, C.POINTER(_f), C.POINTER(_f)],

@@ -72,6 +72,8 @@ class ProfScope {
 
 // Plain 16-byte-vector copy kernel (bandwidth reference for profiler counters).
 int launch_copy(void* dst, const void* src, size_t bytes, hipStream_t st);
+// FP32 FMA loop: blocks x 256 threads x 16 x iters FLOP (peak VALU rate reference).
+int launch_fma_loop(float* out, unsigned blocks, int iters, hipStream_t st);
 
 inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 

@@ -1,5 +1,6 @@
-// copy.hip -- plain device copy: the bandwidth / counter reference that the
-// profiler figures of the other kernels are scaled against.
+// copy.hip -- calibration kernels: a plain device copy (the bandwidth and
+// profiler-counter reference the other kernels are scaled against) and an
+// FP32 FMA loop (the achievable unpacked FP32 VALU rate).
 #include "common.hpp"
 
 namespace th {
@@ -35,6 +36,29 @@ int launch_copy(void* dst, const void* src, size_t bytes, hipStream_t st) {
     }
     ProfScope prof("copy16", st);
     k_copy16<<<(unsigned)blocks, 256, 0, st>>>((f32x4*)dst, (const f32x4*)src, n);
+    TH_HIP(hipGetLastError());
+    return TIRA_OK;
+}
+
+// 8 independent FMA chains per thread, `iters` steps: 16 * iters FLOP per thread.
+__global__ __launch_bounds__(256) void k_fma_loop(float* __restrict__ out, int iters, float a, float b) {
+    float x[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) x[k] = (float)(threadIdx.x + k);
+#pragma unroll 4
+    for (int i = 0; i < iters; i++) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) x[k] = fmaf(x[k], a, b);
+    }
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; k++) s += x[k];
+    if (s == 12345.678f) out[blockIdx.x] = s;  // keeps the chains live
+}
+
+int launch_fma_loop(float* out, unsigned blocks, int iters, hipStream_t st) {
+    ProfScope prof("fma_loop", st);
+    k_fma_loop<<<blocks, 256, 0, st>>>(out, iters, 0.999999f, 1e-7f);
     TH_HIP(hipGetLastError());
     return TIRA_OK;
 }

@@ -296,6 +296,17 @@ int tira_hip_copy_async(void* dst, const void* src, size_t bytes) {
     return th::launch_copy(dst, src, bytes, g_stream);
 }
 
+int tira_hip_fma_loop_async(float* out, unsigned blocks, int iters) {
+    TH_ARG(out && blocks > 0 && iters > 0, "fma loop needs a device buffer of `blocks` floats");
+    return th::launch_fma_loop(out, blocks, iters, g_stream);
+}
+
+int tira_hip_pointer_is_device(const void* ptr, int* is_dev) {
+    TH_ARG(is_dev, "NULL pointer");
+    *is_dev = th::is_device(ptr) ? 1 : 0;
+    return TIRA_OK;
+}
+
 int tira_hip_release_workspace(void) {
     int dev = 0;
     TH_HIP(hipGetDevice(&dev));

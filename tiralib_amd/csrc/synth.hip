@@ -65,6 +65,57 @@ __global__ __launch_bounds__(256) void k_synth_eig(float* __restrict__ out, size
     }
 }
 
+// Curved plate sheets: sheet k is the quadric height field
+// q_z = a q_x^2 + b q_y^2 + c q_x q_y in local coordinates q = R_k^T (p - C_k),
+// thickness 1 (|F| <= |grad F| / 2), patch radius `patch`; voxels on a sheet
+// hold I - n n^T with n its unit normal (the last sheet wins). Other voxels
+// hold, with probability `noise`, m (I - q q^T) with q uniform on the sphere
+// and m ~ U(0, 0.5), drawn from (seed, voxel).
+constexpr int kMaxSheets = 32;
+struct SheetSet {
+    float R[kMaxSheets][9];  // row-major, columns = local axes in world coordinates
+    float C[kMaxSheets][3];
+    float K[kMaxSheets][3];
+    int n;
+    float patch2;
+};
+
+__global__ __launch_bounds__(256) void k_synth_sheets(float* __restrict__ out, unsigned s0, unsigned s1, unsigned s2,
+                                                      const SheetSet S, double noise, unsigned long long seed) {
+    const size_t nv = (size_t)s0 * s1 * s2;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += (size_t)gridDim.x * blockDim.x) {
+        const float px = (float)(i % s2), py = (float)((i / s2) % s1), pz = (float)(i / ((size_t)s1 * s2));
+        float n0 = 0, n1 = 0, n2 = 0, mag = 0;
+        for (int k = 0; k < S.n; k++) {
+            const float dx = px - S.C[k][0], dy = py - S.C[k][1], dz = pz - S.C[k][2];
+            const float* R = S.R[k];
+            const float qx = dx * R[0] + dy * R[3] + dz * R[6];
+            const float qy = dx * R[1] + dy * R[4] + dz * R[7];
+            const float qz = dx * R[2] + dy * R[5] + dz * R[8];
+            const float a = S.K[k][0], b = S.K[k][1], c = S.K[k][2];
+            const float f = qz - (a * qx * qx + b * qy * qy + c * qx * qy);
+            const float gx = -(2.0f * a * qx + c * qy), gy = -(2.0f * b * qy + c * qx);
+            const float gn = sqrtf(gx * gx + gy * gy + 1.0f);
+            if (fabsf(f) <= 0.5f * gn && qx * qx + qy * qy <= S.patch2) {
+                const float ig = 1.0f / gn;
+                n0 = (R[0] * gx + R[1] * gy + R[2]) * ig;
+                n1 = (R[3] * gx + R[4] * gy + R[5]) * ig;
+                n2 = (R[6] * gx + R[7] * gy + R[8]) * ig;
+                mag = 1.0f;
+            }
+        }
+        if (mag == 0.0f && u01(seed, i, 0) < noise) {
+            const double z = 2.0 * u01(seed, i, 1) - 1.0, ph = 6.283185307179586 * u01(seed, i, 2);
+            const double r = sqrt(fmax(0.0, 1.0 - z * z));
+            n0 = (float)(r * cos(ph)); n1 = (float)(r * sin(ph)); n2 = (float)z;
+            mag = (float)(0.5 * u01(seed, i, 3));
+        }
+        float* o = out + i * 6;
+        o[0] = mag * (1.0f - n0 * n0); o[1] = -mag * n0 * n1; o[2] = -mag * n0 * n2;
+        o[3] = mag * (1.0f - n1 * n1); o[4] = -mag * n1 * n2; o[5] = mag * (1.0f - n2 * n2);
+    }
+}
+
 }  // namespace th
 
 using namespace th;
@@ -80,6 +131,27 @@ int tira_synth_eig_tensors_async(float* out6, size_t n, unsigned long long seed,
     k_synth_eig<<<grid, 256, 0, stream()>>>(out6, n, seed, std::log((double)lo), std::log((double)hi),
                                             degenerate_frac, std::log((double)gap_lo), std::log((double)gap_hi),
                                             lambdas);
+    TH_HIP(hipGetLastError());
+    return TIRA_OK;
+}
+
+int tira_synth_sheets_async(float* out6, unsigned s0, unsigned s1, unsigned s2, int n_sheets, const float* params,
+                            float patch, float noise_frac, unsigned long long seed) {
+    TH_ARG(out6 && (n_sheets == 0 || params), "NULL pointer");
+    TH_ARG(n_sheets >= 0 && n_sheets <= kMaxSheets, "at most %d sheets", kMaxSheets);
+    TH_ARG(noise_frac >= 0.0f && noise_frac <= 1.0f, "noise fraction must be in [0, 1]");
+    SheetSet S{};
+    S.n = n_sheets;
+    S.patch2 = patch * patch;
+    for (int k = 0; k < n_sheets; k++) {
+        const float* q = params + 15 * k;
+        for (int j = 0; j < 9; j++) S.R[k][j] = q[j];
+        for (int j = 0; j < 3; j++) { S.C[k][j] = q[9 + j]; S.K[k][j] = q[12 + j]; }
+    }
+    const size_t nv = (size_t)s0 * s1 * s2;
+    if (!nv) return TIRA_OK;
+    const unsigned grid = (unsigned)std::min<size_t>(ceil_div(nv, 256), 65536);
+    k_synth_sheets<<<grid, 256, 0, stream()>>>(out6, s0, s1, s2, S, (double)noise_frac, seed);
     TH_HIP(hipGetLastError());
     return TIRA_OK;
 }

@@ -132,6 +132,36 @@ def tubes_sheets_volume(shape=(128, 128, 128), n_straight=10, n_helix=10, tube_r
     return vol
 
 
+def sheet_params(shape=(256, 256, 256), n_sheets=12, seed=SEED_CFG4):
+    """Parameters of the curved plate sheets (15 floats each: R row-major with
+    the local axes as columns, centre (x, y, z) in the middle 60% of the
+    volume, curvatures (a, b, c) in U(-0.01, 0.01): radii of 50+ voxels) and
+    the patch radius (0.35 of the smallest edge)."""
+    rng = np.random.default_rng(seed)
+    s0, s1, s2 = shape
+    ext = np.array([s2, s1, s0], np.float64)
+    R = random_rotations(n_sheets, rng)
+    C = rng.uniform(0.2, 0.8, (n_sheets, 3)) * ext
+    K = rng.uniform(-0.01, 0.01, (n_sheets, 3))
+    p = np.concatenate([R.reshape(n_sheets, 9), C, K], axis=1).astype(np.float32)
+    return np.ascontiguousarray(p), float(0.35 * min(shape))
+
+
+def curved_sheets_volume_device(shape=(256, 256, 256), n_sheets=12, noise_frac=0.02, seed=SEED_CFG4):
+    """Plate-dominant volume generated on the device (tira_synth_sheets_async):
+    `n_sheets` curved sheets (see sheet_params), thickness 1, tensor I - n n^T
+    with n the surface normal, plus plate noise m (I - q q^T) (q uniform on the
+    sphere, m ~ U(0, 0.5)) on `noise_frac` of the voxels. Returns a
+    (s0, s1, s2, 6) packed DeviceArray."""
+    from ._lib import DeviceArray, call
+    p, patch = sheet_params(shape, n_sheets, seed)
+    out = DeviceArray(tuple(shape) + (6,))
+    call("tira_synth_sheets_async", out.ptr, *[int(s) for s in shape], n_sheets, p.ctypes.data, patch,
+         float(noise_frac), seed)
+    call("tira_hip_synchronize")
+    return out
+
+
 def random_rotations(n, rng):
     """Haar-uniform rotations from normalised Gaussian quaternions."""
     q = rng.normal(size=(n, 4))
