@@ -47,7 +47,7 @@ def test_dropin_results_match_oracle(tmp_path, hip, oracle):
     ow, oh, od = (int(v) for v in g[:3])
     got = np.array(g[3:], np.float32).reshape(od, oh, ow)
     W, H, D = 20, 18, 16
-    src = ((np.arange(W * H * D, dtype=np.uint64) * 2654435761) % (1 << 32) % 1000).astype(np.float32) / 1000
+    src = ((np.arange(W * H * D, dtype=np.uint64) * 2654435761) % 1000).astype(np.float32) / 1000  # size_t product
     ref = oracle.gaussian_convolve3(src.reshape(D, H, W), 1.0, 1.0, 1.0)
     assert got.shape == ref.shape
     assert np.abs(got - ref).max() <= 1e-5
