@@ -47,6 +47,14 @@ TH_HD void sort3(T& x, T& y, T& z) {
     if (y > z) { t = y; y = z; z = t; }
     if (x > y) { t = x; x = y; y = t; }
 }
+#ifdef __HIP_DEVICE_COMPILE__
+// min3 / med3 / max3 (one instruction each on the device)
+template <>
+TH_HD void sort3<float>(float& x, float& y, float& z) {
+    const float lo = fminf(fminf(x, y), z), hi = fmaxf(fmaxf(x, y), z), mid = __builtin_amdgcn_fmed3f(x, y, z);
+    x = lo; y = mid; z = hi;
+}
+#endif
 
 // Eigenvalues (ascending) of | a b d ; b c e ; d e f |. `diag_eps` is the
 // off-diagonal energy below which the diagonal is returned sorted (the
@@ -106,20 +114,20 @@ TH_HD void normalize3_eps(T* v) {  // normalize3, eigen.h:56-65
     }
 }
 
-// Largest cofactor row of (A - lambda I) (unnormalised); returns its squared norm.
+// Largest cofactor row of (A - lambda I) (unnormalised); returns its squared
+// norm. adj(A - lambda I) is symmetric (6 distinct cofactors), and for a
+// simple eigenvalue it is k v v^T, so the row with the largest diagonal
+// entry |k| v_i^2 is the row with the largest norm.
 template <typename T>
 TH_HD T cofactor_row(T a, T b, T c, T d, T e, T f, T lam, T* v) {
     const T m00 = a - lam, m11 = c - lam, m22 = f - lam;
-    const T c00 = m11 * m22 - e * e, c01 = e * d - b * m22, c02 = b * e - m11 * d;
-    const T c10 = d * e - b * m22, c11 = m00 * m22 - d * d, c12 = b * d - m00 * e;
-    const T c20 = b * e - d * m11, c21 = d * b - m00 * e, c22 = m00 * m11 - b * b;
-    const T r0 = c00 * c00 + c01 * c01 + c02 * c02;
-    const T r1 = c10 * c10 + c11 * c11 + c12 * c12;
-    const T r2 = c20 * c20 + c21 * c21 + c22 * c22;
-    if (r0 >= r1 && r0 >= r2) { v[0] = c00; v[1] = c01; v[2] = c02; return r0; }
-    if (r1 >= r0 && r1 >= r2) { v[0] = c10; v[1] = c11; v[2] = c12; return r1; }
-    v[0] = c20; v[1] = c21; v[2] = c22;
-    return r2;
+    const T c00 = m11 * m22 - e * e, c11 = m00 * m22 - d * d, c22 = m00 * m11 - b * b;
+    const T c01 = d * e - b * m22, c02 = b * e - d * m11, c12 = b * d - m00 * e;
+    const T a0 = d_abs(c00), a1 = d_abs(c11), a2 = d_abs(c22);
+    if (a0 >= a1 && a0 >= a2) { v[0] = c00; v[1] = c01; v[2] = c02; }
+    else if (a1 >= a2) { v[0] = c01; v[1] = c11; v[2] = c12; }
+    else { v[0] = c02; v[1] = c12; v[2] = c22; }
+    return v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
 }
 
 // Null vector of (A - lambda I) from its largest cofactor row (eigen.h:247-288).
@@ -340,14 +348,15 @@ TH_HD void eig3_solve(T a, T b, T c, T d, T e, T f, T diag_eps, T* lam, T* e0, T
         const T iv = d_rsqrt(rr);
         v[0] *= iv; v[1] *= iv; v[2] *= iv;
     }
-    // orthonormal complement {U, W} of v
-    T t[3] = {T(0), T(0), T(0)};
+    // orthonormal complement {U, W} of v: U = v x e_k for the axis k of the
+    // smallest |v_k|
+    T U[3], W[3];
     {
         const T ax = d_abs(v[0]), ay = d_abs(v[1]), az = d_abs(v[2]);
-        if (ax <= ay && ax <= az) t[0] = T(1); else if (ay <= az) t[1] = T(1); else t[2] = T(1);
+        if (ax <= ay && ax <= az) { U[0] = T(0); U[1] = v[2]; U[2] = -v[1]; }
+        else if (ay <= az) { U[0] = -v[2]; U[1] = T(0); U[2] = v[0]; }
+        else { U[0] = v[1]; U[1] = -v[0]; U[2] = T(0); }
     }
-    T U[3], W[3];
-    cross3(v, t, U);
     const T iu = d_rsqrt(U[0] * U[0] + U[1] * U[1] + U[2] * U[2]);
     U[0] *= iu; U[1] *= iu; U[2] *= iu;
     cross3(v, U, W);

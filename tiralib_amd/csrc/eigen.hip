@@ -122,7 +122,7 @@ __device__ __forceinline__ EigTile eig_tile_at(const EigBatchArgs& args, unsigne
 // (The kernel is VALU-bound near the HBM rate: a persistent variant that
 // prefetches the next tile into registers lost more to the occupancy its
 // extra registers cost than it gained in overlap.)
-template <int LAYOUT, bool VECS>
+template <int LAYOUT, bool VECS, bool REFINE>
 __global__ __launch_bounds__(kEigThreads) void k_eig3(const EigBatchArgs args, unsigned ntiles, unsigned flags,
                                                       unsigned long long* __restrict__ n_refined) {
     constexpr int W = (LAYOUT == TIRA_LAYOUT_MAT9) ? 9 : 6;
@@ -153,7 +153,6 @@ __global__ __launch_bounds__(kEigThreads) void k_eig3(const EigBatchArgs args, u
     __syncthreads();
 
     const bool scale_inv = (flags & TIRA_EIG_SCALE_INVARIANT) != 0;
-    const bool refine = (flags & TIRA_EIG_REFINE_FP64) != 0;
     const float deps = scale_inv ? 1e-30f : 1e-6f;
 
     // ---- FP32 pass
@@ -167,7 +166,7 @@ __global__ __launch_bounds__(kEigThreads) void k_eig3(const EigBatchArgs args, u
         if (scale_inv) ex = prescale6(a, b, c, d, e, f);
         float lam[3], v0[3], v2[3];
         eig3_solve<float>(a, b, c, d, e, f, deps, lam, VECS ? v0 : nullptr, v2);
-        if (refine && eig3_near_degenerate(lam[0], lam[1], lam[2])) {
+        if (REFINE && eig3_near_degenerate(lam[0], lam[1], lam[2])) {
             const int slot = atomicAdd(&s_count, 1);
             s_list[slot] = (short)i;
         }
@@ -182,7 +181,7 @@ __global__ __launch_bounds__(kEigThreads) void k_eig3(const EigBatchArgs args, u
     __syncthreads();
 
     // ---- FP64 re-solve of the compacted near-degenerate list
-    const int nflag = s_count;
+    const int nflag = REFINE ? s_count : 0;
     for (int k = tid; k < nflag; k += kEigThreads) {
         const int i = s_list[k];
         float af, bf, cf, df, ef, ff;
@@ -267,7 +266,8 @@ __global__ __launch_bounds__(256) void k_evecs2polar(const float* __restrict__ m
 template <int LAYOUT, bool VECS>
 static void launch_eig3_args(const EigBatchArgs& a, unsigned tiles, unsigned flags, unsigned long long* n_refined,
                              hipStream_t st) {
-    k_eig3<LAYOUT, VECS><<<tiles, kEigThreads, 0, st>>>(a, tiles, flags, n_refined);
+    if (flags & TIRA_EIG_REFINE_FP64) k_eig3<LAYOUT, VECS, true><<<tiles, kEigThreads, 0, st>>>(a, tiles, flags, n_refined);
+    else k_eig3<LAYOUT, VECS, false><<<tiles, kEigThreads, 0, st>>>(a, tiles, flags, n_refined);
 }
 
 static int launch_eig3_table(const EigBatchArgs& a, unsigned tiles, int layout, bool vecs, unsigned flags,
