@@ -167,7 +167,7 @@ def _frob6(x):
     return float(np.sqrt((x[:, [0, 3, 5]] ** 2).sum() + 2 * (x[:, [1, 2, 4]] ** 2).sum()))
 
 
-def bench_mlp(steps, check=True, log=None):
+def bench_mlp(steps, check=True, log=None, keep_net=False):
     """Learned plate surrogate on the 256^3 curved-sheet volume (12 sheets, 2%
     plate noise, seed 1004), sigma 4, w 25: the BF16 MLP trained on the
     device (4M samples from the FP64 64-sample integrator), then stick + plate
@@ -268,7 +268,62 @@ def bench_mlp(steps, check=True, log=None):
             res[key]["rel_frob_err_vs_fp64_oracle"] = _frob6(got - ref) / _frob6(ref)
         res["error_check_sample"] = (f"{len(recv)} receivers (48 on sheets, 16 uniform) in the 24^3 core of a 48^3 "
                                      f"crop; FP64 64-sample oracle, {time.perf_counter() - t:.1f} s")
+    if keep_net:
+        return res, net
     net.close()
+    return res
+
+
+def bench_pipeline(net=None, log=None):
+    """Full pipeline on the 512 x 512 x 384 vascular-tree volume (seed 1005,
+    radii 2-8, intensity 1 on 0, N(0, 0.1) noise): structure tensor ->
+    Gaussian sigma 2 (13 taps) -> stick + plate vote sigma 4 (w 25) -> eig3
+    saliency; the closed-form plate and, with `net`, the learned plate with
+    integrator fallback. Per-stage times and roofline fractions."""
+    from tiralib_amd import _lib, api, synth
+    from tiralib_amd._lib import DeviceArray, Event, Stream, call
+    say = log or (lambda *a: None)
+    shape = (384, 512, 512)
+    n = int(np.prod(shape))
+    vol, segs = synth.vessel_volume_device(shape, seed=synth.SEED_CFG5, noise_sigma=0.1)
+    st = Stream().use()
+    e0, e1 = Event(), Event()
+    pairs = in_bounds_pairs(shape, 25)
+    prof = _lib.profile_get
+    res = {"workload": "512x512x384 vascular tree (seed 1005, %d segments, radii 2-8, N(0,0.1) noise): structure "
+                       "tensor -> Gaussian sigma=2 -> stick+plate vote sigma=4 w=25 -> eig3 saliency" % len(segs),
+           "voxels": n, "pairs_per_vote": pairs}
+    modes = [("closed", _lib.PLATE_CLOSED, None)]
+    if net is not None:
+        modes.append(("mlp", _lib.PLATE_MLP, net))
+    sal = DeviceArray(shape + (3,))
+    for name, mode, mlp in modes:
+        if mode == _lib.PLATE_CLOSED:  # warm-up: workspace and offset tables (the learned run reuses both)
+            api.pipeline3(vol, smooth_sigma=2.0, vote_sigma=4.0, w=25, plate_mode=mode, saliency=sal)
+        st.synchronize()
+        call("tira_hip_profile_begin")
+        e0.record(st)
+        api.pipeline3(vol, smooth_sigma=2.0, vote_sigma=4.0, w=25, plate_mode=mode, mlp=mlp, mlp_tol=4e-3,
+                      saliency=sal)
+        e1.record(st)
+        st.synchronize()
+        call("tira_hip_profile_end")
+        ms = e0.elapsed_ms(e1)
+        vote_name = "vote3_mlp" if mode == _lib.PLATE_MLP else "vote3_closed"
+        prep_name = "vote3_prep_mlp" if mode == _lib.PLATE_MLP else "vote3_prep"
+        stages = {k: prof(k)[0] for k in ("structure_tensor", "gauss3_tensor", prep_name, vote_name, "eig3")}
+        hbm = lambda b, t: round(b * n / (t * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)
+        vote_roof = ({"unit": "TFLOP/s BF16 MFMA", "achieved": round(pairs * 17792 / (stages[vote_name] * 1e-3) / 1e12, 1),
+                      "frac": round(pairs * 17792 / (stages[vote_name] * 1e-3) / 2.5e15, 4)}
+                     if mode == _lib.PLATE_MLP else
+                     {"unit": "TFLOP/s FP32", "achieved": round(pairs * 96 / (stages[vote_name] * 1e-3) / 1e12, 1),
+                      "frac": round(pairs * 96 / (stages[vote_name] * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4)})
+        res[name] = {"ms": round(ms, 2), "mvoxel_s": round(n / (ms * 1e-3) / 1e6, 3),
+                     "stages_ms": {k: round(v, 3) for k, v in stages.items()},
+                     "roofline": {"structure_tensor_hbm_frac": hbm(28, stages["structure_tensor"]),
+                                  "gauss3_tensor_hbm_frac": hbm(48, stages["gauss3_tensor"]),
+                                  "eig3_hbm_frac": hbm(36, stages["eig3"]), "vote": vote_roof}}
+        say(f"pipeline ({name} plate): {ms:.0f} ms")
     return res
 
 
@@ -309,6 +364,7 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-mlp", action="store_true")
+    ap.add_argument("--no-pipeline", action="store_true")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); the vote shards by z-slabs with an "
@@ -408,8 +464,14 @@ def main(argv=None):
     if not a.no_batch:
         out["eig3_batch"] = bench_batch(max(3, a.steps // 2), max(1, a.warmup))
         print("bench: batched eig3 + Gaussian done", file=sys.stderr, flush=True)
+    log = lambda m: print("bench: " + m, file=sys.stderr, flush=True)
+    net = None
     if not a.no_mlp:
-        out["mlp_plate"] = bench_mlp(2, log=lambda m: print("bench: " + m, file=sys.stderr, flush=True))
+        out["mlp_plate"], net = bench_mlp(2, log=log, keep_net=True)
+    if not a.no_pipeline:
+        out["pipeline"] = bench_pipeline(net, log=log)
+    if net is not None:
+        net.close()
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(field, sigma, w, samples, a.cpu_seconds)
     print(json.dumps(out))

@@ -56,6 +56,7 @@ typedef enum tira_status {
 /* eig3 flags */
 #define TIRA_EIG_REFINE_FP64   1u  /* FP64 re-solve of voxels with relative eigengap < 1e-4 */
 #define TIRA_EIG_SCALE_INVARIANT 2u /* thresholds relative to the tensor norm (reference: absolute 1e-6) */
+#define TIRA_EIG_SALIENCY 4u        /* write (l2 - l1, l1 - l0, l0): stick, plate, ball saliency */
 
 /* ------------------------------------------------------------------ runtime */
 int tira_hip_abi_version(void);
@@ -225,6 +226,33 @@ int tira_synth_eig_tensors_async(float* out6, size_t n, unsigned long long seed,
  * m (I - q q^T) on a fraction `noise_frac` of the voxels. Non-blocking. */
 int tira_synth_sheets_async(float* out6, unsigned s0, unsigned s1, unsigned s2, int n_sheets, const float* params,
                             float patch, float noise_frac, unsigned long long seed);
+/* Intensity volume of a vascular tree: 1 inside any tapered capsule of
+ * `segments` (device, nseg x 8 floats: p0 xyz, p1 xyz, r0, r1; x fastest),
+ * 0 outside, plus N(0, noise_sigma^2) noise. Non-blocking. */
+int tira_synth_vessels_async(float* out, unsigned s0, unsigned s1, unsigned s2, const float* segments, int nseg,
+                             float noise_sigma, unsigned long long seed);
+
+/* ------------------------------------------------------------------ front end and pipeline
+ * tira::volume::gradient (volume.h:974-1021, forward / backward difference at
+ * the ends of an axis, central inside, divided by the spacing in double)
+ * along x (axis 0, fastest, s2 voxels), y (s1) and z (s0) with the volume's
+ * double spacings (volume.h:22), fused with the
+ * structure tensor g g^T: out6 is s0 x s1 x s2 x 6 packed (xx, xy, xz, yy,
+ * yz, zz). Device pointers, non-blocking. */
+int tira_structure_tensor3_async(const float* intensity, unsigned s0, unsigned s1, unsigned s2, double spacing_x,
+                                 double spacing_y, double spacing_z, float* out6);
+/* The whole 3D pipeline on device buffers: structure tensor -> Gaussian of
+ * the 6 components (sigma smooth_sigma, ceil(6 sigma) odd taps, zero-padded
+ * "same"; 0 skips it) -> stick + plate vote (plate_mode TIRA_PLATE_*; for
+ * TIRA_PLATE_MLP the network `mlp` with its own sigmas and power, voters
+ * whose error check exceeds mlp_tol taking the integrator) -> eig3 saliency
+ * (l2 - l1, l1 - l0, l0) into saliency (n x 3). flags: TIRA_EIG_* for the
+ * eig3 stage. Intermediates live in library workspace (48 B per voxel). */
+int tira_tensorvote3_pipeline_async(const float* intensity, unsigned s0, unsigned s1, unsigned s2, double spacing_x,
+                                    double spacing_y, double spacing_z, float smooth_sigma, float vote_sigma1,
+                                    float vote_sigma2, unsigned w, unsigned power, int stick, int plate,
+                                    unsigned samples, int plate_mode, void* mlp, float mlp_tol, float* saliency,
+                                    unsigned flags);
 
 /* ------------------------------------------------------------------ learned plate vote
  * No reference counterpart: a BF16 MLP (5 -> 64 -> 64 -> 64 -> 6, MFMA, FP32

@@ -343,3 +343,27 @@ def gaussian3_same_packed6_f64(t6, kernel, nthreads=1):
     k = _c32(kernel)
     lib().oracle_gaussian3_same_packed6_f64(t.reshape(-1), s0, s1, s2, k, k.shape[0], out, int(nthreads))
     return out
+
+
+def gradient3(vol, spacing=(1.0, 1.0, 1.0)):
+    """tira::volume::gradient (volume.h:974-1021) along x, y, z of a
+    (s0, s1, s2) float32 volume (x = axis 2 fastest): forward difference at the
+    first voxel of an axis, backward at the last, central inside, each
+    (float difference) / spacing evaluated in double and stored as float32."""
+    v = np.asarray(vol, np.float32)
+    out = []
+    for axis, d in zip((2, 1, 0), spacing):
+        g = np.empty(v.shape, np.float64)
+        a = np.moveaxis(v, axis, 0)
+        gg = np.moveaxis(g, axis, 0)
+        gg[0] = (a[1] - a[0]).astype(np.float64) / d
+        gg[-1] = (a[-1] - a[-2]).astype(np.float64) / d
+        gg[1:-1] = (a[2:] - a[:-2]).astype(np.float64) / (2.0 * d)
+        out.append(g.astype(np.float32))
+    return out
+
+
+def structure_tensor3(vol, spacing=(1.0, 1.0, 1.0)):
+    """g g^T of gradient3 as packed-6 (xx, xy, xz, yy, yz, zz), float32."""
+    gx, gy, gz = gradient3(vol, spacing)
+    return np.stack([gx * gx, gx * gy, gx * gz, gy * gy, gy * gz, gz * gz], axis=-1)

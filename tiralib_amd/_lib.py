@@ -31,6 +31,7 @@ PLATE_MLP = 3
 
 EIG_REFINE_FP64 = 1
 EIG_SCALE_INVARIANT = 2
+EIG_SALIENCY = 4
 
 MLP_NPARAM = 12288
 MLP_NBINS = 2048
@@ -103,6 +104,10 @@ SIGNATURES = {
     "tira_batch_gauss_eig3_destroy": [_vp],
     "tira_synth_eig_tensors_async": [_vp, _sz, C.c_ulonglong, _f, _f, _f, _f, _f, _vp],
     "tira_synth_sheets_async": [_vp, _u, _u, _u, _i, _vp, _f, _f, C.c_ulonglong],
+    "tira_synth_vessels_async": [_vp, _u, _u, _u, _vp, _i, _f, C.c_ulonglong],
+    "tira_structure_tensor3_async": [_vp, _u, _u, _u, C.c_double, C.c_double, C.c_double, _vp],
+    "tira_tensorvote3_pipeline_async": [_vp, _u, _u, _u, C.c_double, C.c_double, C.c_double, _f, _f, _f, _u, _u,
+                                        _i, _i, _u, _i, _vp, _f, _vp, _u],
     "tira_plate_mlp_create": [C.POINTER(_vp), _f, _f, _u, _u, C.c_ulonglong],
     "tira_plate_mlp_destroy": [_vp],
     "tira_plate_mlp_train": [_vp, _sz, _u, _u, _f, _f, C.c_ulonglong, C.POINTER(_f), C.POINTER(_f)],

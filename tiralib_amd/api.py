@@ -407,3 +407,31 @@ class PlateMLP:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------------ front end and pipeline
+def structure_tensor3(intensity, spacing=(1.0, 1.0, 1.0), out=None):
+    """Gradient (volume.h:974-1021 edge rules) and structure tensor g g^T of a
+    device (s0, s1, s2) volume: returns a (s0, s1, s2, 6) packed DeviceArray."""
+    shape = intensity.shape
+    if out is None:
+        out = DeviceArray(tuple(shape) + (6,))
+    call("tira_structure_tensor3_async", intensity.ptr, *[int(s) for s in shape], *[float(x) for x in spacing],
+         out.ptr)
+    return out
+
+
+def pipeline3(intensity, smooth_sigma=2.0, vote_sigma=4.0, w=None, power=1, stick=True, plate=True, samples=20,
+              plate_mode=_lib.PLATE_AUTO, mlp=None, mlp_tol=4e-3, spacing=(1.0, 1.0, 1.0), saliency=None,
+              flags=_lib.EIG_SCALE_INVARIANT):
+    """Structure tensor -> Gaussian -> stick + plate vote -> eig3 saliency
+    (l2 - l1, l1 - l0, l0) of a device intensity volume (non-blocking)."""
+    shape = intensity.shape
+    if w is None:
+        w = 2 * int(round(3 * vote_sigma)) + 1
+    if saliency is None:
+        saliency = DeviceArray(tuple(shape) + (3,))
+    call("tira_tensorvote3_pipeline_async", intensity.ptr, *[int(s) for s in shape], *[float(x) for x in spacing],
+         float(smooth_sigma), float(vote_sigma), 0.0, int(w), int(power), int(stick), int(plate), int(samples),
+         int(plate_mode), mlp.handle if mlp is not None else None, float(mlp_tol), saliency.ptr, int(flags))
+    return saliency

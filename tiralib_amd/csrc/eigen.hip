@@ -66,6 +66,13 @@ __device__ __forceinline__ void global_tensor(const float* g, size_t i, float& a
     }
 }
 
+// Eigenvalues l0 <= l1 <= l2, or the saliency maps (l2 - l1, l1 - l0, l0):
+// stick, plate and ball saliency of a voted field.
+__device__ __forceinline__ void store_eig(float* o, float l0, float l1, float l2, bool saliency) {
+    if (saliency) { o[0] = l2 - l1; o[1] = l1 - l0; o[2] = l0; }
+    else { o[0] = l0; o[1] = l1; o[2] = l2; }
+}
+
 // Segment table: one launch covers up to kEigMaxSeg independent tensor
 // arrays. It travels in the kernel arguments (no upload, capturable); tile t
 // belongs to the last segment whose tile0 <= t.
@@ -153,6 +160,7 @@ __global__ __launch_bounds__(kEigThreads) void k_eig3(const EigBatchArgs args, u
     __syncthreads();
 
     const bool scale_inv = (flags & TIRA_EIG_SCALE_INVARIANT) != 0;
+    const bool saliency = (flags & TIRA_EIG_SALIENCY) != 0;
     const float deps = scale_inv ? 1e-30f : 1e-6f;
 
     // ---- FP32 pass
@@ -174,9 +182,7 @@ __global__ __launch_bounds__(kEigThreads) void k_eig3(const EigBatchArgs args, u
             if (cur.v0) { cur.v0[i * 3 + 0] = v0[0]; cur.v0[i * 3 + 1] = v0[1]; cur.v0[i * 3 + 2] = v0[2]; }
             if (cur.v2) { cur.v2[i * 3 + 0] = v2[0]; cur.v2[i * 3 + 1] = v2[1]; cur.v2[i * 3 + 2] = v2[2]; }
         }
-        s_out[i * 3 + 0] = ldexpf(lam[0], ex);
-        s_out[i * 3 + 1] = ldexpf(lam[1], ex);
-        s_out[i * 3 + 2] = ldexpf(lam[2], ex);
+        store_eig(s_out + i * 3, ldexpf(lam[0], ex), ldexpf(lam[1], ex), ldexpf(lam[2], ex), saliency);
     }
     __syncthreads();
 
@@ -190,9 +196,8 @@ __global__ __launch_bounds__(kEigThreads) void k_eig3(const EigBatchArgs args, u
         if (scale_inv) ex = prescale6(af, bf, cf, df, ef, ff);
         double lam[3], v0[3], v2[3];
         eig3_solve<double>(af, bf, cf, df, ef, ff, scale_inv ? 1e-30 : 1e-6, lam, VECS ? v0 : nullptr, v2);
-        s_out[i * 3 + 0] = ldexpf((float)lam[0], ex);
-        s_out[i * 3 + 1] = ldexpf((float)lam[1], ex);
-        s_out[i * 3 + 2] = ldexpf((float)lam[2], ex);
+        store_eig(s_out + i * 3, ldexpf((float)lam[0], ex), ldexpf((float)lam[1], ex), ldexpf((float)lam[2], ex),
+                  saliency);
         if (VECS) {
             if (cur.v0) { cur.v0[i * 3 + 0] = (float)v0[0]; cur.v0[i * 3 + 1] = (float)v0[1]; cur.v0[i * 3 + 2] = (float)v0[2]; }
             if (cur.v2) { cur.v2[i * 3 + 0] = (float)v2[0]; cur.v2[i * 3 + 1] = (float)v2[1]; cur.v2[i * 3 + 2] = (float)v2[2]; }

@@ -116,6 +116,40 @@ __global__ __launch_bounds__(256) void k_synth_sheets(float* __restrict__ out, u
     }
 }
 
+// Vascular tree intensity volume: 1 inside any tapered capsule segment
+// (p0, p1, r0, r1), 0 outside, plus Gaussian noise N(0, sigma^2) drawn from
+// (seed, voxel) by Box-Muller. seg: nseg x 8 floats (p0 xyz, p1 xyz, r0, r1),
+// coordinates (x, y, z) = (axis 2, axis 1, axis 0) of the volume.
+__global__ __launch_bounds__(256) void k_synth_vessels(float* __restrict__ out, unsigned s0, unsigned s1, unsigned s2,
+                                                       const float* __restrict__ seg, int nseg, double sigma,
+                                                       unsigned long long seed) {
+    const size_t nv = (size_t)s0 * s1 * s2;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += (size_t)gridDim.x * blockDim.x) {
+        const float px = (float)(i % s2), py = (float)((i / s2) % s1), pz = (float)(i / ((size_t)s1 * s2));
+        float v = 0.0f;
+        for (int k = 0; k < nseg && v == 0.0f; k++) {
+            const float* q = seg + 8 * k;
+            const float rm = fmaxf(q[6], q[7]);
+            if (px < fminf(q[0], q[3]) - rm || px > fmaxf(q[0], q[3]) + rm || py < fminf(q[1], q[4]) - rm ||
+                py > fmaxf(q[1], q[4]) + rm || pz < fminf(q[2], q[5]) - rm || pz > fmaxf(q[2], q[5]) + rm)
+                continue;
+            const float ux = q[3] - q[0], uy = q[4] - q[1], uz = q[5] - q[2];
+            const float wx = px - q[0], wy = py - q[1], wz = pz - q[2];
+            const float uu = ux * ux + uy * uy + uz * uz;
+            float t = uu > 0.0f ? (wx * ux + wy * uy + wz * uz) / uu : 0.0f;
+            t = fminf(1.0f, fmaxf(0.0f, t));
+            const float dx = wx - t * ux, dy = wy - t * uy, dz = wz - t * uz;
+            const float r = q[6] + t * (q[7] - q[6]);
+            if (dx * dx + dy * dy + dz * dz <= r * r) v = 1.0f;
+        }
+        if (sigma > 0.0) {
+            const double g = sqrt(-2.0 * log(u01(seed, i, 0))) * cos(6.283185307179586 * u01(seed, i, 1));
+            v += (float)(sigma * g);
+        }
+        out[i] = v;
+    }
+}
+
 }  // namespace th
 
 using namespace th;
@@ -152,6 +186,19 @@ int tira_synth_sheets_async(float* out6, unsigned s0, unsigned s1, unsigned s2, 
     if (!nv) return TIRA_OK;
     const unsigned grid = (unsigned)std::min<size_t>(ceil_div(nv, 256), 65536);
     k_synth_sheets<<<grid, 256, 0, stream()>>>(out6, s0, s1, s2, S, (double)noise_frac, seed);
+    TH_HIP(hipGetLastError());
+    return TIRA_OK;
+}
+
+int tira_synth_vessels_async(float* out, unsigned s0, unsigned s1, unsigned s2, const float* segments, int nseg,
+                             float noise_sigma, unsigned long long seed) {
+    TH_ARG(out && (nseg == 0 || segments), "NULL pointer");
+    TH_ARG(nseg >= 0 && noise_sigma >= 0.0f, "bad arguments");
+    TH_ARG(is_device(out) && (nseg == 0 || is_device(segments)), "the vessel generator takes device pointers");
+    const size_t nv = (size_t)s0 * s1 * s2;
+    if (!nv) return TIRA_OK;
+    const unsigned grid = (unsigned)std::min<size_t>(ceil_div(nv, 256), 65536);
+    k_synth_vessels<<<grid, 256, 0, stream()>>>(out, s0, s1, s2, segments, nseg, (double)noise_sigma, seed);
     TH_HIP(hipGetLastError());
     return TIRA_OK;
 }
