@@ -128,12 +128,16 @@ def bench_batch(steps, warmup, check=True):
     }
     # eig3 alone over all 48 volumes in one launch (packed-6 in, 3 eigenvalues
     # out, 36 B per tensor), FP32 deflation and with the FP64 re-solve
+    import ctypes as C
     ev = [DeviceArray(tuple(s) + (3,)) for s in shapes]
-    counts = [int(np.prod(s)) for s in shapes]
+    counts = np.ascontiguousarray([int(np.prod(s)) for s in shapes], np.uint64)
+    tin = (C.c_void_p * len(vols))(*[v.ptr for v in vols])
+    tout = (C.c_void_p * len(ev))(*[e.ptr for e in ev])
     for name, flags in (("eig3_fp32", _lib.EIG_SCALE_INVARIANT),
                         ("eig3_fp64_refine", _lib.EIG_SCALE_INVARIANT | _lib.EIG_REFINE_FP64)):
         def run_eig():
-            api.eig3_batch(vols, counts=counts, evals=ev, flags=flags)
+            call("tira_eig3_batch_async", C.cast(tin, C.c_void_p), counts.ctypes.data, len(vols),
+                 _lib.LAYOUT_PACKED6, C.cast(tout, C.c_void_p), None, None, flags, None)
         run_eig()
         st.synchronize()
         e0.record(st)

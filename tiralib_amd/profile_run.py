@@ -4,9 +4,15 @@
     python -m tiralib_amd.profile_run [--steps 5]
 
 Runs a 1 GiB plain copy (the counter calibration reference: 1 GiB read,
-1 GiB written) three times, then `steps` iterations of the 3D vote workload of
-bench.py (128^3 tubes + sheets, sigma 2, w 13, N 20) and of the batched
-eig3 + Gaussian launch. Prints one line per stage.
+1 GiB written) three times, then `steps` iterations of: the 3D vote workload
+of bench.py (128^3 tubes + sheets, sigma 2, w 13, N 20); the batched eig3 +
+Gaussian launch and the one-launch eig3 of its 48 volumes; the learned plate
+vote on a 96^3 curved-sheet volume (sigma 4, w 25; untrained weights with
+the error check off: the kernel's work does not depend on the weights, and
+training's ~10^5 small dispatches are left out of counter runs);
+and the full pipeline (structure tensor, Gaussian sigma 2, vote sigma 4,
+eig3 saliency) on a 192 x 256 x 256 vascular-tree volume. Prints one line per
+stage.
 """
 from __future__ import annotations
 
@@ -25,6 +31,8 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--skip-mlp", action="store_true")
+    ap.add_argument("--skip-pipeline", action="store_true")
     a = ap.parse_args(argv)
     from tiralib_amd import _lib, api, synth
     from tiralib_amd._lib import DeviceArray, call
@@ -60,8 +68,30 @@ def main(argv=None):
                 call("tira_eig3_async", v.ptr, int(np.prod(s)), _lib.LAYOUT_PACKED6, e.ptr, None, None,
                      _lib.EIG_SCALE_INVARIANT, None)
         call("tira_hip_synchronize")
+        counts = [int(np.prod(s)) for s in shapes]
+        for _ in range(a.steps):
+            api.eig3_batch(vols, counts=counts, evals=evs, flags=_lib.EIG_SCALE_INVARIANT)
+        call("tira_hip_synchronize")
         b.close()
-        print(f"batch: {a.steps} runs of 48 volumes ({sum(int(np.prod(s)) for s in shapes)} voxels)", flush=True)
+        print(f"batch: {a.steps} runs of 48 volumes ({sum(counts)} voxels)", flush=True)
+    if not a.skip_mlp:
+        mshape = (96, 96, 96)
+        mvol = synth.curved_sheets_volume_device(mshape, 4, 0.02, seed=synth.SEED_CFG4)
+        net = api.PlateMLP(4.0, 0.0, 1, 64, seed=synth.SEED_CFG4)
+        mout = DeviceArray((int(np.prod(mshape)), 6))
+        for _ in range(a.steps):
+            net.vote(mvol, mout, mshape, 25, stick=True, tol=1e30, in_layout=_lib.LAYOUT_PACKED6)
+        call("tira_hip_synchronize")
+        net.close()
+        print(f"mlp: {a.steps} learned-plate votes of 96^3, sigma 4, w 25", flush=True)
+    if not a.skip_pipeline:
+        pshape = (192, 256, 256)
+        pvol, _ = synth.vessel_volume_device(pshape, seed=synth.SEED_CFG5, noise_sigma=0.1)
+        sal = DeviceArray(pshape + (3,))
+        for _ in range(a.steps):
+            api.pipeline3(pvol, smooth_sigma=2.0, vote_sigma=4.0, w=25, plate_mode=_lib.PLATE_CLOSED, saliency=sal)
+        call("tira_hip_synchronize")
+        print(f"pipeline: {a.steps} runs on 192 x 256 x 256", flush=True)
 
 
 if __name__ == "__main__":
