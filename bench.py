@@ -31,6 +31,7 @@ PEAK_HBM_GBS = 8000.0
 STICK_PAIR_FLOP = 31         # t = q.d, eta, r = q - 2 t d, eta * s * r r^T
 PLATE_CLOSED_PAIR_FLOP = 65  # s = n.d, 4 polynomials in 1 - s^2, alpha I + beta nn^T + (h d^T + d h^T)
 # Reference formulation (SURVEY counting rule): stick 33, N-sample plate 41 N + 12.
+ATV3_PAIR_FLOP = 106         # a = e2.v, b = e0.v, six coefficients, 3 symmetric rank-2 updates + trace term
 
 
 def profiled_traffic(kernel):
@@ -331,6 +332,94 @@ def bench_pipeline(net=None, log=None):
     return res
 
 
+def bench_atv(d_in, field, shape, sigma, w, steps, warmup, cpu_seconds=None):
+    """3D analytical tensor voting (tensorvote.h:2635-2739) on the same resident
+    volume, and 2D ATV of a 256^2 field (sigma 5, the reference module's window
+    rule): Mvoxel/s, FLOP roofline of the gather, optional bounded CPU port."""
+    from tiralib_amd import _lib
+    from tiralib_amd._lib import DeviceArray, Event, Stream, call
+    n = int(np.prod(shape))
+    d_out = DeviceArray((n, 6))
+    st = Stream().use()
+
+    def step():
+        call("tira_tensorvote3_atv_async", d_in.ptr, _lib.LAYOUT_MAT9, d_out.ptr, _lib.LAYOUT_PACKED6, *shape,
+             sigma, w)
+
+    for _ in range(warmup):
+        step()
+    st.synchronize()
+    e0, e1 = Event(), Event()
+    call("tira_hip_profile_begin")
+    e0.record(st)
+    for _ in range(steps):
+        step()
+    e1.record(st)
+    st.synchronize()
+    call("tira_hip_profile_end")
+    ms = e0.elapsed_ms(e1) / steps
+    k_ms, k_cnt = _lib.profile_get("vote3_atv")
+    p_ms, _ = _lib.profile_get("vote3_atv_prep")
+    pairs = in_bounds_pairs(shape, w)
+    achieved = pairs * ATV3_PAIR_FLOP / (k_ms * 1e-3) / 1e12
+    res = {
+        "workload": f"3D ATV {shape[0]}x{shape[1]}x{shape[2]} (tubes+sheets, seed 1002), sigma={sigma}, w={w}, "
+                    f"device-resident, packed-6 output",
+        "value": round(n / (ms * 1e-3) / 1e6, 3), "unit": "Mvoxel/s", "ms_per_step": round(ms, 4),
+        "stages_ms": {"vote3_atv_prep": round(p_ms, 4), "vote3_atv": round(k_ms, 4)},
+        "roofline": {"kernel": "vote3_atv", "bound": "valu", "achieved": round(achieved, 3),
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                     "frac_of_unpacked_peak": round(achieved / PEAK_FP32_UNPACKED_TFLOPS, 4),
+                     "traffic": profiled_traffic("k_vote3_atv"), "flop_per_pair": ATV3_PAIR_FLOP,
+                     "avg_launch_ms": round(k_ms, 4), "launches": k_cnt},
+    }
+    d_out.free()
+    # 2D: 256^2 mixed field as the reference's test builds it, window 2 int(3 sigma) + 1
+    rng = np.random.default_rng(1001)
+    H = W = 256
+    ang = rng.uniform(0, np.pi, (H, W))
+    l1 = rng.uniform(0.5, 1.0, (H, W))
+    l0 = rng.uniform(0.0, 0.5, (H, W))
+    f2 = np.zeros((H, W, 4), np.float32)
+    f2[..., 0] = l0 + (l1 - l0) * np.cos(ang) ** 2
+    f2[..., 1] = f2[..., 2] = (l1 - l0) * np.cos(ang) * np.sin(ang)
+    f2[..., 3] = l0 + (l1 - l0) * np.sin(ang) ** 2
+    d2, o2 = DeviceArray.from_host(f2), DeviceArray((H, W, 4))
+    w2 = 2 * int(3 * 5.0) + 1
+    for _ in range(warmup):
+        call("tira_tensorvote2", o2.ptr, d2.ptr, 5.0, 0.0, 1, w2, H, W, 1, 1, 10, 1)
+    call("tira_hip_synchronize")
+    t = time.perf_counter()
+    for _ in range(steps):
+        call("tira_tensorvote2", o2.ptr, d2.ptr, 5.0, 0.0, 1, w2, H, W, 1, 1, 10, 1)
+    call("tira_hip_synchronize")
+    ms2 = (time.perf_counter() - t) * 1e3 / steps
+    res["atv2d"] = {"workload": f"2D ATV 256x256 mixed stick/ball field, sigma=5, w={w2}, device pointers, "
+                                f"synchronous entry", "value": round(H * W / (ms2 * 1e-3) / 1e6, 3),
+                    "unit": "Mpixel/s", "ms_per_call": round(ms2, 4)}
+    d2.free(); o2.free()
+    if cpu_seconds:
+        from oracle import oracle as O
+        t = time.perf_counter()
+        O.tensorvote3_atv(field, sigma, w, recv=np.array([n // 2]), nthreads=1)
+        t_fixed = time.perf_counter() - t
+        k = 64
+        while True:
+            recv = np.sort(rng.choice(n, k, replace=False))
+            t = time.perf_counter()
+            O.tensorvote3_atv(field, sigma, w, recv=recv, nthreads=1)
+            dt = time.perf_counter() - t
+            if dt > cpu_seconds or k >= n:
+                break
+            k = min(n, int(k * max(2.0, cpu_seconds / max(dt, 1e-3))))
+        per = max(dt - t_fixed, 1e-9) / k
+        total = t_fixed + per * n
+        res["cpu_baseline"] = {"value": round(n / total / 1e6, 6), "unit": "Mvoxel/s", "cores": 1, "kind": "port",
+                               "sample": f"eigen-decomposition of all {n} voxels timed in full; ATV window of {k} "
+                                         f"uniformly sampled receivers extrapolated to {n}; {total:.1f} s per step"}
+    return res
+
+
 def calibrate():
     """Achievable rates on this GPU: FP32 FMA loop (unpacked VALU) and a
     16-byte-vector device copy (HBM), each timed with HIP events."""
@@ -369,6 +458,7 @@ def main(argv=None):
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-mlp", action="store_true")
     ap.add_argument("--no-pipeline", action="store_true")
+    ap.add_argument("--no-atv", action="store_true")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); the vote shards by z-slabs with an "
@@ -469,6 +559,10 @@ def main(argv=None):
         out["eig3_batch"] = bench_batch(max(3, a.steps // 2), max(1, a.warmup))
         print("bench: batched eig3 + Gaussian done", file=sys.stderr, flush=True)
     log = lambda m: print("bench: " + m, file=sys.stderr, flush=True)
+    if not a.no_atv:
+        out["atv"] = bench_atv(d_in, field, shape, sigma, w, max(3, a.steps // 2), max(1, a.warmup),
+                               None if a.no_cpu_baseline else min(a.cpu_seconds, 6.0))
+        log("analytical tensor voting done")
     net = None
     if not a.no_mlp:
         out["mlp_plate"], net = bench_mlp(2, log=log, keep_net=True)

@@ -6,8 +6,9 @@
 // libtira_hip.so (tira_hip.h):
 //   eigen.h:768-1012    evals2_symmetric, evecs2polar_symmetric,
 //                       evals3_symmetric, evecs3spherical_symmetric
-//   tensorvote.h:2125-2305  tensorvote_tk, tensorvote, tensorvote_prebuilt (2D)
-//   tensorvote.h:2741-2873  _tensorvote3_tk, tensorvote3 (3D)
+//   tensorvote.h:2029-2305  tensorvote_atv, tensorvote_tk, tensorvote,
+//                           tensorvote_prebuilt (2D)
+//   tensorvote.h:2695-2873  _tensorvote3_atv, _tensorvote3_tk, tensorvote3 (3D)
 //   filter.h:304-534    gaussian_convolve2, gaussian_filter3d
 //   tensorvote.h:1223-1280  build_neighbors2d (tira::cpu in the reference;
 //                       the library builds the same table on the host)
@@ -122,6 +123,13 @@ static void tensorvote_tk(Type* t_out, const Type* lambdas, const Type* thetas, 
 }
 
 template <typename Type>
+static void tensorvote_atv(Type* t_out, const Type* lambdas, const Type* thetas, Type sigma, const unsigned w,
+                           const unsigned s0, const unsigned s1) {
+    static_assert(hip_detail::is_float<Type>::value, "libtira_hip serves float tensors");
+    hip_detail::check(tira_tensorvote2_atv(t_out, lambdas, thetas, sigma, w, s0, s1), "tensorvote_atv");
+}
+
+template <typename Type>
 static void tensorvote(Type* t_out, const Type* t_in, const Type sigma1, const Type sigma2, const int power,
                        const unsigned w, const unsigned shape0, const unsigned shape1, const bool stick = true,
                        const bool plate = true, const unsigned samples = 10, const bool is_atv = false,
@@ -152,6 +160,13 @@ static inline void _tensorvote3_tk(float* output_field, const float* lambdas, co
     hip_detail::check(tira_tensorvote3_tk(output_field, lambdas, evec2, evec0, s0, s1, s2, sigma1, sigma2, power, w,
                                           STICK, PLATE, samples),
                       "_tensorvote3_tk");
+}
+
+static inline void _tensorvote3_atv(float* output_field, const float* lambdas, const float* evec1,
+                                    const float* evec2, unsigned int s0, unsigned int s1, unsigned int s2,
+                                    float sigma, unsigned int w) {
+    hip_detail::check(tira_tensorvote3_atv(output_field, lambdas, evec1, evec2, s0, s1, s2, sigma, w),
+                      "_tensorvote3_atv");
 }
 
 template <typename Type>

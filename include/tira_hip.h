@@ -38,7 +38,7 @@ typedef enum tira_status {
     TIRA_ERR_INVALID_ARG = 1,   /* bad shape / sigma / window (pybind checks, tensorvote_tk.cu:73-83, 162-178) */
     TIRA_ERR_HIP = 2,           /* a HIP runtime call failed */
     TIRA_ERR_OUT_OF_MEMORY = 3,
-    TIRA_ERR_UNSUPPORTED = 4,   /* valid request this build does not serve (ATV) */
+    TIRA_ERR_UNSUPPORTED = 4,   /* valid request this build does not serve */
     TIRA_ERR_NO_DEVICE = 5
 } tira_status;
 
@@ -136,8 +136,9 @@ int tira_build_neighbors2d(int w, float sigma1, float sigma2, unsigned power, un
                            void* out, int capacity, int* count);
 /* tira::cuda::tensorvote<T>(t_out, t_in, sigma1, sigma2, power, w, shape0, shape1,
  * stick, plate, samples, is_atv), tensorvote.h:2200-2226. t_in / t_out:
- * shape0 x shape1 x 4 floats [a b b c], shape1 fastest. is_atv != 0 returns
- * TIRA_ERR_UNSUPPORTED. */
+ * shape0 x shape1 x 4 floats [a b b c], shape1 fastest. is_atv != 0: analytical
+ * tensor voting (tensorvote.h:1993-2066) with sigma1 and half-window w / 2;
+ * sigma2, power, stick, plate and samples do not apply, as in the reference. */
 int tira_tensorvote2(float* t_out, const float* t_in, float sigma1, float sigma2, int power, unsigned w,
                      unsigned shape0, unsigned shape1, int stick, int plate, unsigned samples, int is_atv);
 /* tira::cuda::tensorvote_prebuilt<T>(t_out, t_in, host_nb, nb_count, sigma1,
@@ -152,12 +153,18 @@ int tira_tensorvote2_prebuilt(float* t_out, const float* t_in, const void* host_
 int tira_tensorvote2_tk(float* t_out, const float* lambdas, const float* thetas, float sigma1, float sigma2,
                         unsigned power, unsigned w, unsigned s0, unsigned s1, int stick, int plate,
                         unsigned samples);
+/* tira::cuda::tensorvote_atv<T>(t_out, lambdas, thetas, sigma, w, s0, s1),
+ * tensorvote.h:2029-2066: 2D analytical tensor voting from eigenpairs
+ * (lambdas / thetas s0 x s1 x 2 as evals2 / evecs2polar return them). */
+int tira_tensorvote2_atv(float* t_out, const float* lambdas, const float* thetas, float sigma, unsigned w,
+                         unsigned s0, unsigned s1);
 
 /* ------------------------------------------------------------------ 3D voting */
 /* tira::cuda::tensorvote3(input_field, output_field, s0, s1, s2, sigma, sigma2, w,
  * power, device, stick, plate, samples, is_atv), tensorvote.h:2816-2873.
- * in / out: s0*s1*s2 x 9 floats (glm::mat3, s2 fastest). is_atv != 0 returns
- * TIRA_ERR_UNSUPPORTED. */
+ * in / out: s0*s1*s2 x 9 floats (glm::mat3, s2 fastest). is_atv != 0:
+ * analytical tensor voting (tensorvote.h:2635-2739) with sigma and half-window
+ * w / 2; sigma2, power, stick, plate and samples do not apply. */
 int tira_tensorvote3(const float* input_field, float* output_field, unsigned s0, unsigned s1, unsigned s2,
                      float sigma, float sigma2, unsigned w, unsigned power, int device, int stick, int plate,
                      unsigned samples, int is_atv);
@@ -172,6 +179,16 @@ int tira_tensorvote3_tk(float* output_field, const float* lambdas, const float* 
 int tira_tensorvote3_async(const float* input_field, int in_layout, float* output_field, int out_layout,
                            unsigned s0, unsigned s1, unsigned s2, float sigma1, float sigma2, unsigned w,
                            unsigned power, int stick, int plate, unsigned samples, int plate_mode);
+/* tira::cuda::_tensorvote3_atv(out, lambdas, evec1, evec2, s0, s1, s2, sigma, w),
+ * tensorvote.h:2695-2739: 3D analytical tensor voting from eigenpairs
+ * (lambdas n x 3 ascending, evec1 middle and evec2 largest eigenvectors n x 3,
+ * orthonormal as evecs3spherical_symmetric returns them). */
+int tira_tensorvote3_atv(float* output_field, const float* lambdas, const float* evec1, const float* evec2,
+                         unsigned s0, unsigned s1, unsigned s2, float sigma, unsigned w);
+/* Native 3D analytical tensor voting on device buffers, non-blocking on the
+ * library stream (layouts TIRA_LAYOUT_*). */
+int tira_tensorvote3_atv_async(const float* input_field, int in_layout, float* output_field, int out_layout,
+                               unsigned s0, unsigned s1, unsigned s2, float sigma, unsigned w);
 
 /* ------------------------------------------------------------------ Gaussian filter */
 /* tira::cuda::gaussian_filter3d<float>(src, width, height, depth, sigma_w, sigma_h,

@@ -89,6 +89,17 @@ def lib():
         _sig(L, "oracle_gaussian_convolve3", [_f32p, _u, _u, _u, _f, _f, _f, _f32p,
                                               C.POINTER(_u), C.POINTER(_u), C.POINTER(_u)])
         _sig(L, "oracle_gaussian3_same_packed6_f64", [_f32p, _u, _u, _u, _f32p, _u, _f64p, _i])
+        _sig(L, "oracle_atv2_f32", [_f] * 9 + [_f32p])
+        _sig(L, "oracle_atv2_f64", [_d] * 9 + [_f64p])
+        _sig(L, "oracle_atv3_f32", [_f, _f, _f, _f32p, _f32p, _f, _f, _f, _f, _f32p])
+        _sig(L, "oracle_atv3_f64", [_d, _d, _d, _f64p, _f64p, _d, _d, _d, _d, _f64p])
+        _sig(L, "oracle_tensorvote2_atv_f32", [_f32p, _f32p, _f32p, _f, _u, _u, _u, _i])
+        _sig(L, "oracle_tensorvote2_atv_full_f32", [_f32p, _f32p, _f, _u, _u, _u, _i])
+        _sig(L, "oracle_tensorvote2_atv_f64", [_f64p, _f32p, _d, _u, _u, _u, _i])
+        _sig(L, "oracle_tensorvote3_atv_eig_f32", [_f32p, _f32p, _f32p, _f32p, _u, _u, _u, _f, _u,
+                                                   C.c_void_p, _sz, _i])
+        _sig(L, "oracle_tensorvote3_atv_f32", [_f32p, _f32p, _u, _u, _u, _f, _u, C.c_void_p, _sz, _i])
+        _sig(L, "oracle_tensorvote3_atv_f64", [_f32p, _f64p, _u, _u, _u, _d, _u, C.c_void_p, _sz, _i])
         _lib = L
     return _lib
 
@@ -297,6 +308,90 @@ def plate3_numerical_f32(d, c1, c2, power, n, samples):
     o = np.empty(6, np.float32)
     lib().oracle_plate3_numerical_f32(_c32(d), float(c1), float(c2), int(power), _c32(n), int(samples), o)
     return o
+
+
+# ---------------------------------------------------------------- analytical tensor voting
+def atv2_pair(l0, l1, ev0, ev1, r, sigma, f64=False):
+    """One 2D ATV vote (tensorvote.h:224-253): (a, b, c) at offset r = (rx, ry)."""
+    if f64:
+        o = np.empty(3)
+        lib().oracle_atv2_f64(float(l0), float(l1), float(ev0[0]), float(ev0[1]), float(ev1[0]), float(ev1[1]),
+                              float(r[0]), float(r[1]), float(sigma), o)
+    else:
+        o = np.empty(3, np.float32)
+        lib().oracle_atv2_f32(float(l0), float(l1), float(ev0[0]), float(ev0[1]), float(ev1[0]), float(ev1[1]),
+                              float(r[0]), float(r[1]), float(sigma), o)
+    return o
+
+
+def atv3_pair(lam, e1, e2, r, sigma, f64=False):
+    """One 3D ATV vote (tensorvote.h:688-760): (00, 01, 11, 02, 12, 22)."""
+    if f64:
+        o = np.empty(6)
+        lib().oracle_atv3_f64(float(lam[0]), float(lam[1]), float(lam[2]), np.ascontiguousarray(e1, np.float64),
+                              np.ascontiguousarray(e2, np.float64), float(r[0]), float(r[1]), float(r[2]), float(sigma), o)
+    else:
+        o = np.empty(6, np.float32)
+        lib().oracle_atv3_f32(float(lam[0]), float(lam[1]), float(lam[2]), _c32(e1), _c32(e2), float(r[0]),
+                              float(r[1]), float(r[2]), float(sigma), o)
+    return o
+
+
+def tensorvote2_atv(field, sigma, w, nthreads=1):
+    """cpu::tensorvote with is_atv (FP32, reference-faithful). field: (H, W, 4)."""
+    f = _c32(field)
+    H, W = f.shape[0], f.shape[1]
+    out = np.zeros((H, W, 4), np.float32)
+    lib().oracle_tensorvote2_atv_full_f32(out, f.reshape(H, W, 4), float(sigma), int(w), H, W, int(nthreads))
+    return out
+
+
+def tensorvote2_atv_eig(lambdas, thetas, sigma, w, nthreads=1):
+    """ATV window from eigenpairs (lambdas, thetas: (H, W, 2))."""
+    lam = _c32(lambdas)
+    H, W = lam.shape[0], lam.shape[1]
+    out = np.zeros((H, W, 4), np.float32)
+    lib().oracle_tensorvote2_atv_f32(out, lam.reshape(-1), _c32(thetas).reshape(-1), float(sigma), int(w), H, W,
+                                     int(nthreads))
+    return out
+
+
+def tensorvote2_atv_f64(field, sigma, w, nthreads=1):
+    f = _c32(field)
+    H, W = f.shape[0], f.shape[1]
+    out = np.zeros((H, W, 4), np.float64)
+    lib().oracle_tensorvote2_atv_f64(out, f.reshape(H, W, 4), float(sigma), int(w), H, W, int(nthreads))
+    return out
+
+
+def tensorvote3_atv(vol9, sigma, w, recv=None, nthreads=1):
+    """cpu::tensorvote3 with is_atv (FP32, reference-faithful). vol9: (s0, s1, s2, 9)."""
+    v = _c32(vol9)
+    s0, s1, s2 = v.shape[:3]
+    rp, nr, keep = _recv(recv)
+    out = np.zeros(((nr if recv is not None else s0 * s1 * s2), 9), np.float32)
+    lib().oracle_tensorvote3_atv_f32(v.reshape(-1), out, s0, s1, s2, float(sigma), int(w), rp, nr, int(nthreads))
+    return out if recv is not None else out.reshape(s0, s1, s2, 9)
+
+
+def tensorvote3_atv_eig(lambdas, evec1, evec2, shape, sigma, w, recv=None, nthreads=1):
+    """ATV window from eigenpairs (the inputs of _tensorvote3_atv)."""
+    s0, s1, s2 = shape
+    rp, nr, keep = _recv(recv)
+    out = np.zeros(((nr if recv is not None else s0 * s1 * s2), 9), np.float32)
+    lib().oracle_tensorvote3_atv_eig_f32(_c32(lambdas).reshape(-1), _c32(evec1).reshape(-1),
+                                         _c32(evec2).reshape(-1), out, s0, s1, s2, float(sigma), int(w), rp, nr,
+                                         int(nthreads))
+    return out if recv is not None else out.reshape(s0, s1, s2, 9)
+
+
+def tensorvote3_atv_f64(vol9, sigma, w, recv=None, nthreads=1):
+    v = _c32(vol9)
+    s0, s1, s2 = v.shape[:3]
+    rp, nr, keep = _recv(recv)
+    out = np.zeros(((nr if recv is not None else s0 * s1 * s2), 9), np.float64)
+    lib().oracle_tensorvote3_atv_f64(v.reshape(-1), out, s0, s1, s2, float(sigma), int(w), rp, nr, int(nthreads))
+    return out if recv is not None else out.reshape(s0, s1, s2, 9)
 
 
 # ---------------------------------------------------------------- Gaussian

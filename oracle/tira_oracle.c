@@ -1215,3 +1215,276 @@ void oracle_gaussian3_same_packed6_f64(const float* in6, unsigned s0, unsigned s
                 }
     free(t1); free(t2);
 }
+
+/* ======================================================================
+ * Analytical tensor voting (ATV), 2D and 3D (tensorvote.h:23-760,
+ * 255-306, 1298-1316, 1380-1399, 1443-1495, 1819-1859)
+ *
+ * One generic body per precision: T = float is the reference-faithful FP32
+ * restatement (operation order kept; the unqualified exp of the reference's
+ * host code resolves to the double overload, so it is evaluated in double
+ * and rounded), T = double is the FP64 route used to state error.
+ * ====================================================================== */
+#define O_ATV_DEFINE(T, SFX, SQRT)                                                                       \
+    /* atv_attenuation, tensorvote.h:34-40 */                                                            \
+    static T o_atv_atten_##SFX(T dist, T sigma) {                                                        \
+        if (sigma == (T)0) return (dist > (T)O_TV_EPS) ? (T)0 : (T)1;                                    \
+        return (T)exp((double)(-(dist * dist) / (sigma * sigma)));                                      \
+    }                                                                                                    \
+    /* atv (2D) with atv_rotation, atv_normalized_T, atv_vTvT, atv_TvvT, atv_H:                       \
+     * tensorvote.h:54-60, 73-79, 100-106, 125-138, 159-253. V = (a, b, c). */                          \
+    void oracle_atv2_##SFX(T l0, T l1, T ev0x, T ev0y, T ev1x, T ev1y, T rx, T ry, T sigma, T* V) {      \
+        const T dist = SQRT(rx * rx + ry * ry);                                                          \
+        const T c = o_atv_atten_##SFX(dist, sigma);                                                      \
+        T dx, dy;                                                                                        \
+        if (dist == 0) { dx = 0; dy = 0; }                                                               \
+        else { dx = rx / dist; dy = ry / dist; }                                                         \
+        const T R_a = 1 - 2 * dx * dx, R_b = -2 * dx * dy, R_c = 1 - 2 * dy * dy;                        \
+        const T T1_a = ev1x * ev1x, T1_b = ev1x * ev1y, T1_c = ev1y * ev1y;                              \
+        const T T0_a = ev0x * ev0x + T1_a, T0_b = ev0x * ev0y + T1_b, T0_c = ev0y * ev0y + T1_c;         \
+        const T v0 = dx * (dx * T0_a + dy * T0_b) + dy * (dx * T0_b + dy * T0_c);                       \
+        const T v1 = dx * (dx * T1_a + dy * T1_b) + dy * (dx * T1_b + dy * T1_c);                       \
+        const T t0x = T0_a * dx + T0_b * dy, t0y = T0_b * dx + T0_c * dy;                                \
+        const T t1x = T1_a * dx + T1_b * dy, t1y = T1_b * dx + T1_c * dy;                                \
+        const T s1 = l1 - l0, inv3 = (T)1.0 / (T)3.0;                                                    \
+        T H_a = s1 * (T1_a - inv3 * (T1_a * v1 + 2 * (t1x * t1x)));                                      \
+        T H_b = s1 * (T1_b - inv3 * (T1_b * v1 + 2 * (t1x * t1y)));                                      \
+        T H_c = s1 * (T1_c - inv3 * (T1_c * v1 + 2 * (t1y * t1y)));                                      \
+        const T s0 = l0, inv4 = (T)1.0 / (T)4.0;                                                         \
+        H_a += s0 * (T0_a - inv4 * (T0_a * v0 + 2 * (t0x * t0x)));                                       \
+        H_b += s0 * (T0_b - inv4 * (T0_b * v0 + 2 * (t0x * t0y)));                                       \
+        H_c += s0 * (T0_c - inv4 * (T0_c * v0 + 2 * (t0y * t0y)));                                       \
+        const T Ra2 = R_a * R_a, Rb2 = R_b * R_b, Rc2 = R_c * R_c;                                       \
+        const T Rab = R_a * R_b, Rbc = R_b * R_c, Rac = R_a * R_c;                                       \
+        V[0] = c * (Ra2 * H_a + 2 * Rab * H_b + Rb2 * H_c);                                              \
+        V[1] = c * (Rb2 * H_b + Rab * H_a + Rac * H_b + Rbc * H_c);                                      \
+        V[2] = c * (Rb2 * H_a + 2 * Rbc * H_b + Rc2 * H_c);                                              \
+    }                                                                                                    \
+    /* atv (3D) with atv_rotation, atv_normalized_T, atv_vTvT, atv_TvvT, atv_H:                       \
+     * tensorvote.h:454-464, 485-493, 518-537, 563-578, 602-683, 688-760.                             \
+     * e1 middle, e2 largest eigenvector. V = (a, b, c, d, e, f) = (00, 01, 11, 02, 12, 22). */        \
+    void oracle_atv3_##SFX(T l0, T l1, T l2, const T* e1, const T* e2, T rx, T ry, T rz, T sigma, T* V) { \
+        const T dist = SQRT(rx * rx + ry * ry + rz * rz);                                                \
+        const T c = o_atv_atten_##SFX(dist, sigma);                                                      \
+        T dx, dy, dz;                                                                                    \
+        if (dist == (T)0) { dx = dy = dz = (T)0; }                                                       \
+        else { dx = rx / dist; dy = ry / dist; dz = rz / dist; }                                         \
+        const T two = (T)2;                                                                              \
+        const T R00 = (T)1 - two * dx * dx, R01 = (T)0 - two * dx * dy, R11 = (T)1 - two * dy * dy;      \
+        const T R02 = (T)0 - two * dx * dz, R12 = (T)0 - two * dy * dz, R22 = (T)1 - two * dz * dz;      \
+        T Tk[3][6];                                                                                      \
+        Tk[0][0] = e2[0] * e2[0]; Tk[0][1] = e2[0] * e2[1]; Tk[0][2] = e2[1] * e2[1];                    \
+        Tk[0][3] = e2[0] * e2[2]; Tk[0][4] = e2[1] * e2[2]; Tk[0][5] = e2[2] * e2[2];                    \
+        Tk[1][0] = e1[0] * e1[0]; Tk[1][1] = e1[0] * e1[1]; Tk[1][2] = e1[1] * e1[1];                    \
+        Tk[1][3] = e1[0] * e1[2]; Tk[1][4] = e1[1] * e1[2]; Tk[1][5] = e1[2] * e1[2];                    \
+        for (int k = 0; k < 6; k++) Tk[1][k] += Tk[0][k];                                                \
+        Tk[2][0] = 1; Tk[2][1] = 0; Tk[2][2] = 1; Tk[2][3] = 0; Tk[2][4] = 0; Tk[2][5] = 1;              \
+        const T s[3] = {l2 - l1, l1 - l0, l0};                                                           \
+        const T inv[3] = {(T)1 / (T)3, (T)1 / (T)4, (T)1 / (T)5};                                        \
+        T term[3][6];                                                                                    \
+        for (int k = 0; k < 3; k++) {                                                                    \
+            const T* t = Tk[k];                                                                          \
+            const T tvx = t[0] * dx + t[1] * dy + t[3] * dz;                                             \
+            const T tvy = t[1] * dx + t[2] * dy + t[4] * dz;                                             \
+            const T tvz = t[3] * dx + t[4] * dy + t[5] * dz;                                             \
+            const T vTv = dx * tvx + dy * tvy + dz * tvz;                                                \
+            const T tt[6] = {tvx * tvx, tvx * tvy, tvy * tvy, tvx * tvz, tvy * tvz, tvz * tvz};          \
+            for (int j = 0; j < 6; j++) term[k][j] = s[k] * (t[j] - inv[k] * (t[j] * vTv + two * tt[j])); \
+        }                                                                                                \
+        T Hh[6];                                                                                         \
+        for (int j = 0; j < 6; j++) Hh[j] = term[0][j] + term[1][j] + term[2][j];                        \
+        const T H00 = Hh[0], H01 = Hh[1], H02 = Hh[3], H11 = Hh[2], H12 = Hh[4], H22 = Hh[5];            \
+        const T H10 = H01, H20 = H02, H21 = H12, R10 = R01, R20 = R02, R21 = R12;                        \
+        const T m00 = R00 * H00 + R01 * H10 + R02 * H20, m01 = R00 * H01 + R01 * H11 + R02 * H21;        \
+        const T m02 = R00 * H02 + R01 * H12 + R02 * H22, m10 = R10 * H00 + R11 * H10 + R12 * H20;        \
+        const T m11 = R10 * H01 + R11 * H11 + R12 * H21, m12 = R10 * H02 + R11 * H12 + R12 * H22;        \
+        const T m20 = R20 * H00 + R21 * H10 + R22 * H20, m21 = R20 * H01 + R21 * H11 + R22 * H21;        \
+        const T m22 = R20 * H02 + R21 * H12 + R22 * H22;                                                 \
+        V[0] = c * (m00 * R00 + m01 * R10 + m02 * R20);                                                  \
+        V[1] = c * (m00 * R01 + m01 * R11 + m02 * R21);                                                  \
+        V[2] = c * (m10 * R01 + m11 * R11 + m12 * R21);                                                  \
+        V[3] = c * (m00 * R02 + m01 * R12 + m02 * R22);                                                  \
+        V[4] = c * (m10 * R02 + m11 * R12 + m12 * R22);                                                  \
+        V[5] = c * (m20 * R02 + m21 * R12 + m22 * R22);                                                  \
+    }
+
+O_ATV_DEFINE(float, f32, sqrtf)
+O_ATV_DEFINE(double, f64, sqrt)
+
+/* atv_window (tensorvote.h:255-306) over every pixel, as cpu::tensorvote_atv
+ * (1298-1316) and the CUDA kernel (1998-2013) write it: [a b b c]. */
+void oracle_tensorvote2_atv_f32(float* t_out, const float* lam, const float* th, float sigma, unsigned w,
+                                unsigned S0, unsigned S1, int nthreads) {
+    const int hw = (int)w / 2, s0 = (int)S0, s1 = (int)S1;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int x0 = 0; x0 < s0; x0++)
+        for (int x1 = 0; x1 < s1; x1++) {
+            float oa = 0, ob = 0, oc = 0;
+            for (int v = -hw; v <= hw; v++) {
+                const int r0 = x0 + v;
+                if (r0 < 0 || r0 >= s0) continue;
+                for (int u = -hw; u <= hw; u++) {
+                    const int r1 = x1 + u;
+                    if (r1 < 0 || r1 >= s1) continue;
+                    const size_t idx = (size_t)r0 * s1 * 2 + (size_t)r1 * 2;
+                    const float t0 = th[idx], t1 = th[idx + 1];
+                    float V[3];
+                    oracle_atv2_f32(lam[idx], lam[idx + 1], cosf(t0), sinf(t0), cosf(t1), sinf(t1), (float)u,
+                                    (float)v, sigma, V);
+                    oa += V[0]; ob += V[1]; oc += V[2];
+                }
+            }
+            float* o = t_out + ((size_t)x0 * s1 + x1) * 4;
+            o[0] = oa; o[1] = ob; o[2] = ob; o[3] = oc;
+        }
+}
+
+/* cpu::tensorvote with is_atv (tensorvote.h:1380-1399): evals2 + evecs2polar + ATV window. */
+void oracle_tensorvote2_atv_full_f32(float* t_out, const float* t_in, float sigma, unsigned w, unsigned S0,
+                                     unsigned S1, int nthreads) {
+    const size_t n = (size_t)S0 * S1;
+    float* lam = (float*)malloc(n * 2 * sizeof(float));
+    float* th = (float*)malloc(n * 2 * sizeof(float));
+    oracle_evals2_symmetric(t_in, n, lam);
+    oracle_evecs2polar_symmetric(t_in, lam, n, th);
+    oracle_tensorvote2_atv_f32(t_out, lam, th, sigma, w, S0, S1, nthreads);
+    free(lam); free(th);
+}
+
+/* FP64 2D ATV: closed-form FP64 eigenpairs, FP64 pair formula, same window. */
+void oracle_tensorvote2_atv_f64(double* t_out, const float* t_in, double sigma, unsigned w, unsigned S0,
+                                unsigned S1, int nthreads) {
+    const size_t n = (size_t)S0 * S1;
+    double* e = (double*)malloc(n * 6 * sizeof(double)); /* l0, l1, e0x, e0y, e1x, e1y */
+    for (size_t i = 0; i < n; i++) {
+        const double a = t_in[i * 4], b = t_in[i * 4 + 1], c = t_in[i * 4 + 3];
+        const double m = 0.5 * (a + c), r = hypot(0.5 * (a - c), b);
+        const double ang = 0.5 * atan2(2 * b, a - c); /* eigenvector of the larger eigenvalue */
+        double* o = e + i * 6;
+        o[0] = m - r; o[1] = m + r;
+        o[4] = cos(ang); o[5] = sin(ang);
+        o[2] = -o[5]; o[3] = o[4];
+    }
+    const int hw = (int)w / 2, s0 = (int)S0, s1 = (int)S1;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int x0 = 0; x0 < s0; x0++)
+        for (int x1 = 0; x1 < s1; x1++) {
+            double acc[3] = {0, 0, 0};
+            for (int v = -hw; v <= hw; v++) {
+                const int r0 = x0 + v;
+                if (r0 < 0 || r0 >= s0) continue;
+                for (int u = -hw; u <= hw; u++) {
+                    const int r1 = x1 + u;
+                    if (r1 < 0 || r1 >= s1) continue;
+                    const double* p = e + ((size_t)r0 * s1 + r1) * 6;
+                    double V[3];
+                    oracle_atv2_f64(p[0], p[1], p[2], p[3], p[4], p[5], u, v, sigma, V);
+                    acc[0] += V[0]; acc[1] += V[1]; acc[2] += V[2];
+                }
+            }
+            double* o = t_out + ((size_t)x0 * s1 + x1) * 4;
+            o[0] = acc[0]; o[1] = acc[1]; o[2] = acc[1]; o[3] = acc[2];
+        }
+    free(e);
+}
+
+/* atv_window3 (tensorvote.h:1443-1482) from eigenpairs: L n x 3 ascending,
+ * E1 middle and E2 largest eigenvectors n x 3 (the inputs of _tensorvote3_atv,
+ * 2695-2739). Output glm::mat3 (9 floats, symmetric). */
+void oracle_tensorvote3_atv_eig_f32(const float* L, const float* E1, const float* E2, float* out9, unsigned s0,
+                                    unsigned s1, unsigned s2, float sigma, unsigned w, const int64_t* recv,
+                                    size_t nrecv, int nthreads) {
+    const int hw = (int)w / 2;
+    const size_t nvox = (size_t)s0 * s1 * s2, nr = recv ? nrecv : nvox;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads > 0 ? nthreads : 1)
+    for (long long i = 0; i < (long long)nr; i++) {
+        const size_t vx = recv ? (size_t)recv[i] : (size_t)i;
+        const int x2 = (int)(vx % s2), x1 = (int)((vx / s2) % s1), x0 = (int)(vx / ((size_t)s1 * s2));
+        float M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int dz = -hw; dz <= hw; ++dz) {
+            const int r0 = x0 + dz;
+            if (r0 < 0 || r0 >= (int)s0) continue;
+            for (int dy = -hw; dy <= hw; ++dy) {
+                const int r1 = x1 + dy;
+                if (r1 < 0 || r1 >= (int)s1) continue;
+                for (int dx = -hw; dx <= hw; ++dx) {
+                    const int r2 = x2 + dx;
+                    if (r2 < 0 || r2 >= (int)s2) continue;
+                    const size_t b = ((size_t)r0 * s1 + r1) * s2 + r2;
+                    float V[6];
+                    oracle_atv3_f32(L[b * 3], L[b * 3 + 1], L[b * 3 + 2], E1 + b * 3, E2 + b * 3, (float)dx,
+                                    (float)dy, (float)dz, sigma, V);
+                    /* Receiver[c][r] += ..., column-major flat: 0=00 1=01 2=02 3=10 4=11 5=12 6=20 7=21 8=22 */
+                    M[0] += V[0]; M[1] += V[1]; M[2] += V[3];
+                    M[3] += V[1]; M[4] += V[2]; M[5] += V[4];
+                    M[6] += V[3]; M[7] += V[4]; M[8] += V[5];
+                }
+            }
+        }
+        memcpy(out9 + (size_t)i * 9, M, sizeof(M));
+    }
+}
+
+/* cpu::tensorvote3 with is_atv (tensorvote.h:1819-1852): evals3, evecs3spherical,
+ * middle / largest eigenvectors back to cartesian, ATV window. */
+void oracle_tensorvote3_atv_f32(const float* in9, float* out9, unsigned s0, unsigned s1, unsigned s2, float sigma,
+                                unsigned w, const int64_t* recv, size_t nrecv, int nthreads) {
+    const size_t n = (size_t)s0 * s1 * s2;
+    float* L = (float*)malloc(n * 3 * sizeof(float));
+    float* Th = (float*)malloc(n * 6 * sizeof(float));
+    float* E1 = (float*)malloc(n * 3 * sizeof(float));
+    float* E2 = (float*)malloc(n * 3 * sizeof(float));
+    oracle_evals3_symmetric(in9, n, L);
+    oracle_evecs3spherical_symmetric(in9, L, n, Th);
+    for (size_t i = 0; i < n; i++) {
+        oracle_spherical_to_cartesian(Th[i * 6 + 4], Th[i * 6 + 5], E2 + i * 3);
+        oracle_spherical_to_cartesian(Th[i * 6 + 2], Th[i * 6 + 3], E1 + i * 3);
+    }
+    oracle_tensorvote3_atv_eig_f32(L, E1, E2, out9, s0, s1, s2, sigma, w, recv, nrecv, nthreads);
+    free(L); free(Th); free(E1); free(E2);
+}
+
+/* FP64 3D ATV: Jacobi eigenpairs, e1 = e2 x e0, FP64 pair formula, same window. */
+void oracle_tensorvote3_atv_f64(const float* in9, double* out9, unsigned s0, unsigned s1, unsigned s2,
+                                double sigma, unsigned w, const int64_t* recv, size_t nrecv, int nthreads) {
+    const size_t n = (size_t)s0 * s1 * s2;
+    double* L = (double*)malloc(n * 3 * sizeof(double));
+    double* E0 = (double*)malloc(n * 3 * sizeof(double));
+    double* E1 = (double*)malloc(n * 3 * sizeof(double));
+    double* E2 = (double*)malloc(n * 3 * sizeof(double));
+    oracle_eig3_f64(in9, n, L, E0, E2);
+    for (size_t i = 0; i < n; i++) {
+        const double* a = E2 + i * 3; const double* b = E0 + i * 3;
+        double* c = E1 + i * 3;
+        c[0] = a[1] * b[2] - a[2] * b[1]; c[1] = a[2] * b[0] - a[0] * b[2]; c[2] = a[0] * b[1] - a[1] * b[0];
+    }
+    const int hw = (int)w / 2;
+    const size_t nr = recv ? nrecv : n;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads > 0 ? nthreads : 1)
+    for (long long i = 0; i < (long long)nr; i++) {
+        const size_t vx = recv ? (size_t)recv[i] : (size_t)i;
+        const int x2 = (int)(vx % s2), x1 = (int)((vx / s2) % s1), x0 = (int)(vx / ((size_t)s1 * s2));
+        double M[6] = {0, 0, 0, 0, 0, 0};
+        for (int dz = -hw; dz <= hw; ++dz) {
+            const int r0 = x0 + dz;
+            if (r0 < 0 || r0 >= (int)s0) continue;
+            for (int dy = -hw; dy <= hw; ++dy) {
+                const int r1 = x1 + dy;
+                if (r1 < 0 || r1 >= (int)s1) continue;
+                for (int dx = -hw; dx <= hw; ++dx) {
+                    const int r2 = x2 + dx;
+                    if (r2 < 0 || r2 >= (int)s2) continue;
+                    const size_t b = ((size_t)r0 * s1 + r1) * s2 + r2;
+                    double V[6];
+                    oracle_atv3_f64(L[b * 3], L[b * 3 + 1], L[b * 3 + 2], E1 + b * 3, E2 + b * 3, dx, dy, dz,
+                                    sigma, V);
+                    for (int k = 0; k < 6; k++) M[k] += V[k];
+                }
+            }
+        }
+        double* o = out9 + (size_t)i * 9;
+        o[0] = M[0]; o[1] = M[1]; o[2] = M[3]; o[3] = M[1]; o[4] = M[2]; o[5] = M[4];
+        o[6] = M[3]; o[7] = M[4]; o[8] = M[5];
+    }
+    free(L); free(E0); free(E1); free(E2);
+}

@@ -135,6 +135,33 @@ void oracle_gaussian_convolve3(const float* in, unsigned sx, unsigned sy, unsign
 void oracle_gaussian3_same_packed6_f64(const float* in6, unsigned s0, unsigned s1, unsigned s2,
                                        const float* kernel, unsigned K, double* out6, int nthreads);
 
+/* ---------------- analytical tensor voting (tensorvote.h:23-760) ---------------- */
+/* Single pair: 2D V = (a, b, c); 3D V = (00, 01, 11, 02, 12, 22). rx.. = receiver
+ * offset from the voter; e1 middle, e2 largest eigenvector (3D). */
+void oracle_atv2_f32(float l0, float l1, float ev0x, float ev0y, float ev1x, float ev1y, float rx, float ry,
+                     float sigma, float* V);
+void oracle_atv2_f64(double l0, double l1, double ev0x, double ev0y, double ev1x, double ev1y, double rx,
+                     double ry, double sigma, double* V);
+void oracle_atv3_f32(float l0, float l1, float l2, const float* e1, const float* e2, float rx, float ry, float rz,
+                     float sigma, float* V);
+void oracle_atv3_f64(double l0, double l1, double l2, const double* e1, const double* e2, double rx, double ry,
+                     double rz, double sigma, double* V);
+/* 2D ATV window from eigenpairs (lambdas n x 2, thetas n x 2) / from the field. */
+void oracle_tensorvote2_atv_f32(float* t_out, const float* lambdas, const float* thetas, float sigma, unsigned w,
+                                unsigned s0, unsigned s1, int nthreads);
+void oracle_tensorvote2_atv_full_f32(float* t_out, const float* t_in, float sigma, unsigned w, unsigned s0,
+                                     unsigned s1, int nthreads);
+void oracle_tensorvote2_atv_f64(double* t_out, const float* t_in, double sigma, unsigned w, unsigned s0,
+                                unsigned s1, int nthreads);
+/* 3D ATV window from eigenpairs (L n x 3, E1 middle, E2 largest) / from the field. */
+void oracle_tensorvote3_atv_eig_f32(const float* L, const float* E1, const float* E2, float* out9, unsigned s0,
+                                    unsigned s1, unsigned s2, float sigma, unsigned w, const int64_t* recv,
+                                    size_t nrecv, int nthreads);
+void oracle_tensorvote3_atv_f32(const float* in9, float* out9, unsigned s0, unsigned s1, unsigned s2, float sigma,
+                                unsigned w, const int64_t* recv, size_t nrecv, int nthreads);
+void oracle_tensorvote3_atv_f64(const float* in9, double* out9, unsigned s0, unsigned s1, unsigned s2,
+                                double sigma, unsigned w, const int64_t* recv, size_t nrecv, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -60,6 +60,17 @@ def test_dropin_results_match_oracle(tmp_path, hip, oracle):
     up = [0, 1, 2, 4, 5, 8]
     assert np.linalg.norm(out[:, up] - ref[:, up]) <= 1e-4 * np.linalg.norm(ref[:, up])
 
+    vol = np.zeros((S * S * S, 9), np.float32)
+    vol[::5, 0], vol[::5, 4], vol[::5, 8] = 1.0, 0.5, 0.25
+    out = np.array(_run(exe, "atv3"), np.float32).reshape(-1, 9)
+    ref = oracle.tensorvote3_atv(vol.reshape(S, S, S, 9), 1.5, 7).reshape(-1, 9)
+    assert np.linalg.norm(out - ref) <= 1e-5 * np.linalg.norm(ref)
+
+    a2 = np.array(_run(exe, "atv2"), np.float32)
+    fin, fout = a2[:10 * 13 * 4].reshape(10, 13, 4), a2[10 * 13 * 4:].reshape(10, 13, 4)
+    ref = oracle.tensorvote2_atv(fin, 2.0, 9)
+    assert np.abs(fout - ref).max() <= 1e-5 * np.abs(ref).max()
+
     nb = _run(exe, "nb2")
     count = int(nb[0])
     rows = np.array(nb[1:], np.float64).reshape(count, 4)

@@ -1,0 +1,190 @@
+"""GPU parity of analytical tensor voting (ATV, tensorvote.h:23-760,
+1993-2066, 2635-2739) against the CPU oracle.
+
+Bar: relative Frobenius error <= 1e-4 (the vote bar) against the
+reference-faithful FP32 restatement on identical seeded inputs, and, where the
+FP64 route is evaluated, GPU vs FP64 <= max(1e-5, 1.5 x the FP32 restatement's
+own error): the kernels evaluate the same vote in a collapsed closed form, so
+only FP32 rounding and the eigen-decomposition route differ, and on fields of
+exactly degenerate tensors (tubes t t^T, sheets I - n n^T) the FP32
+restatement is itself ~1e-5 from FP64. 2D fields: max-abs / max-ref <= 1e-5;
+their generator is the reference's own (test_cuda_vs_python.py:48-66), whose
+bar is 1%."""
+import numpy as np
+import pytest
+
+from tiralib_amd import _lib, api, synth
+from tiralib_amd import tensorvote_atv as atv_mod
+from tiralib_amd._lib import DeviceArray
+from tests.conftest import rel_frob
+from tests.test_oracle_atv import _mixed_field
+
+pytestmark = pytest.mark.gpu
+
+NT = 8
+TOL = 1e-5       # 2D max-abs bar; FP64 floor for 3D
+TOL_VOTE = 1e-4  # 3D relative Frobenius vs the FP32 restatement
+
+
+def _field3(shape, seed, degenerate=0.05):
+    t6, _ = synth.eig_tensors(int(np.prod(shape)), seed=seed, lo=0.1, hi=10.0, degenerate_frac=degenerate)
+    return synth.mat9_from_packed6(t6).reshape(*shape, 9)
+
+
+def _check3(gpu, ref, ref64=None, tol=TOL_VOTE):
+    e = rel_frob(np.asarray(gpu).reshape(-1, 9), np.asarray(ref).reshape(-1, 9))
+    msg = f"GPU vs FP32 oracle {e:.3e}"
+    if ref64 is not None:
+        e64 = rel_frob(np.asarray(gpu).reshape(-1, 9), ref64.reshape(-1, 9))
+        r64 = rel_frob(np.asarray(ref).reshape(-1, 9), ref64.reshape(-1, 9))
+        msg += f"; GPU vs FP64 {e64:.3e}; oracle FP32 vs FP64 {r64:.3e}"
+        assert e64 <= max(TOL, 1.5 * r64), msg
+    print(msg)
+    assert e <= tol, msg
+
+
+@pytest.mark.parametrize("H", [16, 32, 64])
+def test_atv2_mixed_field_module(hip, oracle, H):
+    """tensorvote_atv.tensorvote_atv(field, sigma=5) as cuda_wrappers.atv_vote2_cuda calls it."""
+    f = _mixed_field(H, H)
+    gpu = atv_mod.tensorvote_atv(f, 5.0)
+    ref = oracle.tensorvote2_atv(f, 5.0, 31, nthreads=NT)
+    e = np.abs(gpu - ref).max() / np.abs(ref).max()
+    print(f"max-abs / max-ref {e:.3e}")
+    assert e <= TOL
+    assert np.array_equal(gpu[..., 1], gpu[..., 2])
+
+
+def test_atv2_module_rules(hip):
+    f = _mixed_field(20, 24, seed=3)
+    # window rule: w = 2 int(3 sigma) + 1, at least 3, even -> +1
+    a = atv_mod.tensorvote_atv(f, 1.7)
+    b = api.tensorvote(f, 1.7, 0.0, 1, 2 * int(3 * 1.7) + 1, is_atv=True)
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(atv_mod.tensorvote_atv(f, 2.0, w=6), atv_mod.tensorvote_atv(f, 2.0, w=7))
+    np.testing.assert_array_equal(atv_mod.tensorvote_atv(f, 0.2), atv_mod.tensorvote_atv(f, 0.2, w=3))
+    # stick / plate / samples do not change the analytical vote
+    np.testing.assert_array_equal(atv_mod.tensorvote_atv(f, 2.0, stick=False, plate=False, samples=3),
+                                  atv_mod.tensorvote_atv(f, 2.0))
+    with pytest.raises(RuntimeError, match="sigma must be > 0"):
+        atv_mod.tensorvote_atv(f, 0.0)
+    with pytest.raises(RuntimeError, match="shape"):
+        atv_mod.tensorvote_atv(f[..., :3], 1.0)
+
+
+def test_atv2_stick_field_and_f64(hip, oracle):
+    rng = np.random.default_rng(5)
+    H, W = 40, 56
+    ang = rng.uniform(0, np.pi, (H, W))
+    l1 = rng.uniform(0.5, 1.0, (H, W))
+    f = np.zeros((H, W, 4), np.float32)
+    f[..., 0] = l1 * np.cos(ang) ** 2
+    f[..., 1] = f[..., 2] = l1 * np.cos(ang) * np.sin(ang)
+    f[..., 3] = l1 * np.sin(ang) ** 2
+    gpu = api.tensorvote(f, 3.0, 0.0, 1, 19, is_atv=True)
+    ref = oracle.tensorvote2_atv(f, 3.0, 19, nthreads=NT)
+    ref64 = oracle.tensorvote2_atv_f64(f, 3.0, 19, nthreads=NT)
+    assert np.abs(gpu - ref).max() <= TOL * np.abs(ref).max()
+    assert np.abs(gpu - ref64).max() <= TOL * np.abs(ref64).max()
+
+
+def test_atv2_eigenpair_entry(hip, oracle):
+    f = _mixed_field(24, 30, seed=9)
+    m4 = f.reshape(-1, 4)
+    lam = oracle.evals2(m4)
+    th = oracle.evecs2polar(m4, lam)
+    gpu = api.tensorvote_atv(lam.reshape(24, 30, 2), th.reshape(24, 30, 2), (24, 30), 2.5, 15)
+    ref = oracle.tensorvote2_atv_eig(lam.reshape(24, 30, 2), th.reshape(24, 30, 2), 2.5, 15)
+    assert np.abs(gpu - ref).max() <= TOL * np.abs(ref).max()
+
+
+def test_atv2_sigma0_is_identity(hip):
+    """sigma = 0: only the voter itself votes, with its own tensor."""
+    f = _mixed_field(17, 19, seed=4)
+    gpu = api.tensorvote(f, 0.0, 0.0, 1, 9, is_atv=True)
+    np.testing.assert_allclose(gpu, f, rtol=0, atol=2e-6)
+
+
+@pytest.mark.parametrize("shape,sigma,w", [((20, 22, 24), 1.5, 7), ((13, 70, 65), 1.0, 5), ((16, 16, 40), 2.0, 13)])
+def test_atv3_random_field(hip, oracle, shape, sigma, w):
+    f = _field3(shape, seed=41)
+    gpu = api.tensorvote3(f, sigma, 0.0, w, 1, is_atv=True)
+    ref = oracle.tensorvote3_atv(f, sigma, w, nthreads=NT)
+    ref64 = oracle.tensorvote3_atv_f64(f, sigma, w, nthreads=NT) if np.prod(shape) <= 12000 else None
+    _check3(gpu, ref, ref64)
+    g = gpu.reshape(-1, 3, 3)
+    assert np.array_equal(g, np.transpose(g, (0, 2, 1)))
+
+
+def test_atv3_tubes_sheets(hip, oracle):
+    f = synth.tubes_sheets_volume((32, 32, 32), n_straight=3, n_helix=2, n_sheets=2, seed=5)
+    gpu = api.tensorvote3(f, 2.0, 0.0, 13, 1, is_atv=True)
+    ref = oracle.tensorvote3_atv(f, 2.0, 13, nthreads=NT)
+    ref64 = oracle.tensorvote3_atv_f64(f, 2.0, 13, nthreads=NT)
+    _check3(gpu, ref, ref64)
+
+
+def test_atv3_flags_do_not_apply(hip):
+    """is_atv ignores sigma2, power, stick, plate and samples (tensorvote.h:2860-2861)."""
+    f = _field3((9, 10, 11), seed=43)
+    a = api.tensorvote3(f, 1.2, 0.0, 5, 1, is_atv=True)
+    b = api.tensorvote3(f, 1.2, 0.8, 5, 3, stick=False, plate=False, samples=2, is_atv=True)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_atv3_eigenpair_entry(hip, oracle):
+    shape = (10, 12, 14)
+    m9 = _field3(shape, seed=44).reshape(-1, 9)
+    lam = oracle.evals3(m9)
+    ang = oracle.evecs3spherical(m9, lam)
+    th, ph = ang[:, 0::2].astype(np.float64), ang[:, 1::2].astype(np.float64)
+    cart = np.stack([np.cos(th) * np.sin(ph), np.sin(th) * np.sin(ph), np.cos(ph)], -1).astype(np.float32)
+    e1, e2 = np.ascontiguousarray(cart[:, 1]), np.ascontiguousarray(cart[:, 2])
+    gpu = api.tensorvote3_atv(lam, e1, e2, shape, 1.5, 9)
+    ref = oracle.tensorvote3_atv_eig(lam, e1, e2, shape, 1.5, 9, nthreads=NT)
+    _check3(gpu, ref)
+
+
+def test_atv3_async_layouts_and_window(hip):
+    shape = (12, 13, 70)
+    f = _field3(shape, seed=45)
+    m9 = f.reshape(-1, 9)
+    p6 = np.ascontiguousarray(m9[:, [0, 1, 2, 4, 5, 8]])
+    host = api.tensorvote3(f, 1.4, 0.0, 7, 1, is_atv=True).reshape(-1, 9)
+    d9, d6 = DeviceArray.from_host(m9), DeviceArray.from_host(p6)
+    o9, o6 = DeviceArray(m9.shape), DeviceArray(p6.shape)
+    try:
+        api.tensorvote3_atv_async(d6, o6, shape, 1.4, 7, in_layout=_lib.LAYOUT_PACKED6,
+                                  out_layout=_lib.LAYOUT_PACKED6)
+        api.tensorvote3_atv_async(d9, o9, shape, 1.4, 6)  # even w: half-window 3, as w = 7
+        _lib.call("tira_hip_synchronize")
+        np.testing.assert_array_equal(o6.to_host(), host[:, [0, 1, 2, 4, 5, 8]])
+        np.testing.assert_array_equal(o9.to_host(), host)
+    finally:
+        for d in (d9, d6, o9, o6):
+            d.free()
+
+
+def test_atv3_sigma0_and_single_voter(hip, oracle):
+    f = _field3((6, 7, 8), seed=46, degenerate=0.0)
+    gpu = api.tensorvote3(f, 0.0, 0.0, 5, 1, is_atv=True)
+    np.testing.assert_allclose(gpu, f, rtol=0, atol=2e-5 * np.abs(f).max())
+    # one voter in an empty volume, against the FP64 pair formula
+    s0, s1, s2 = 9, 10, 11
+    vol = np.zeros((s0, s1, s2, 9), np.float32)
+    vol[4, 5, 6] = f[2, 3, 4]
+    gpu = api.tensorvote3(vol, 1.8, 0.0, 7, 1, is_atv=True)
+    lam64 = np.linalg.eigh(f[2, 3, 4].reshape(3, 3).astype(np.float64))
+    ev, Q = lam64
+    worst = 0.0
+    for z in range(s0):
+        for y in range(s1):
+            for x in range(s2):
+                d = (6 - x, 5 - y, 4 - z)
+                if max(map(abs, d)) > 3:
+                    assert not gpu[z, y, x].any()
+                    continue
+                V = oracle.atv3_pair(ev, Q[:, 1], Q[:, 2], d, 1.8, f64=True)
+                g = gpu[z, y, x][[0, 1, 4, 2, 5, 8]]
+                worst = max(worst, np.abs(g - V).max())
+    assert worst <= 1e-5 * np.abs(f[2, 3, 4]).max(), worst

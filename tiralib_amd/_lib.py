@@ -94,6 +94,9 @@ SIGNATURES = {
     "tira_tensorvote3": [_vp, _vp, _u, _u, _u, _f, _f, _u, _u, _i, _i, _i, _u, _i],
     "tira_tensorvote3_tk": [_vp, _vp, _vp, _vp, _u, _u, _u, _f, _f, _u, _u, _i, _i, _u],
     "tira_tensorvote3_async": [_vp, _i, _vp, _i, _u, _u, _u, _f, _f, _u, _u, _i, _i, _u, _i],
+    "tira_tensorvote2_atv": [_vp, _vp, _vp, _f, _u, _u, _u],
+    "tira_tensorvote3_atv": [_vp, _vp, _vp, _vp, _u, _u, _u, _f, _u],
+    "tira_tensorvote3_atv_async": [_vp, _i, _vp, _i, _u, _u, _u, _f, _u],
     "tira_gaussian_filter3d": [_vp, _u, _u, _u, _f, _f, _f, _vp, C.POINTER(_u), C.POINTER(_u), C.POINTER(_u)],
     "tira_gaussian_convolve2": [_vp, _u, _u, _f, _f, _vp, C.POINTER(_u), C.POINTER(_u)],
     "tira_gaussian_taps": [_u, _f, _vp],

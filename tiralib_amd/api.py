@@ -108,8 +108,8 @@ def tensorvote3(input_field, sigma: float, sigma2: float = 0.0, w: int = 0, powe
     """
     f = _f32(input_field)
     s0, s1, s2 = _vol_shape(f)
-    if sigma <= 0:
-        raise ValueError("sigma must be > 0")
+    if sigma < 0 or (sigma == 0 and not is_atv):
+        raise ValueError("sigma must be > 0 (ATV also takes 0: each voxel keeps its own tensor)")
     if w <= 0:
         w = 2 * int(3.0 * max(sigma, sigma2)) + 1
     out = _like(f, (s0, s1, s2, 9))
@@ -137,6 +137,24 @@ def tensorvote3_async(field, out, shape, sigma1, sigma2, w, power, stick=True, p
     s0, s1, s2 = shape
     call("tira_tensorvote3_async", field.ptr, int(in_layout), out.ptr, int(out_layout), s0, s1, s2,
          float(sigma1), float(sigma2), int(w), int(power), int(stick), int(plate), int(samples), int(plate_mode))
+    return out
+
+
+def tensorvote3_atv(lambdas, evec1, evec2, shape, sigma, w):
+    """tira::cuda::_tensorvote3_atv (tensorvote.h:2695-2739): analytical tensor voting
+    from eigenpairs (ascending eigenvalues, middle and largest eigenvectors)."""
+    s0, s1, s2 = shape
+    l, e1, e2 = _f32(lambdas), _f32(evec1), _f32(evec2)
+    out = _like(l, (s0, s1, s2, 9))
+    call("tira_tensorvote3_atv", ptr(out), ptr(l), ptr(e1), ptr(e2), s0, s1, s2, float(sigma), int(w))
+    return out
+
+
+def tensorvote3_atv_async(field, out, shape, sigma, w, in_layout=_lib.LAYOUT_MAT9, out_layout=_lib.LAYOUT_MAT9):
+    """Native non-blocking 3D analytical tensor voting on device arrays (library stream)."""
+    s0, s1, s2 = shape
+    call("tira_tensorvote3_atv_async", field.ptr, int(in_layout), out.ptr, int(out_layout), s0, s1, s2,
+         float(sigma), int(w))
     return out
 
 
@@ -435,3 +453,12 @@ def pipeline3(intensity, smooth_sigma=2.0, vote_sigma=4.0, w=None, power=1, stic
          float(smooth_sigma), float(vote_sigma), 0.0, int(w), int(power), int(stick), int(plate), int(samples),
          int(plate_mode), mlp.handle if mlp is not None else None, float(mlp_tol), saliency.ptr, int(flags))
     return saliency
+
+
+def tensorvote_atv(lambdas, thetas, shape, sigma, w):
+    """tira::cuda::tensorvote_atv (2D, tensorvote.h:2029-2066) from eigenpairs (H, W, 2) each."""
+    H, W = shape
+    l, t = _f32(lambdas), _f32(thetas)
+    out = _like(l, (H, W, 4))
+    call("tira_tensorvote2_atv", ptr(out), ptr(l), ptr(t), float(sigma), int(w), H, W)
+    return out

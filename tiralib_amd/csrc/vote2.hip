@@ -1,6 +1,6 @@
-// vote2.hip -- 2D tensor-kernel (TK) voting on gfx950, replacing
-// tira::cuda::tensorvote / tensorvote_prebuilt / tensorvote_tk
-// (tensorvote.h:2071-2305) and the host table builder
+// vote2.hip -- 2D tensor-kernel (TK) and analytical (ATV) voting on gfx950,
+// replacing tira::cuda::tensorvote / tensorvote_prebuilt / tensorvote_tk /
+// tensorvote_atv (tensorvote.h:1993-2305) and the host table builder
 // cpu::build_neighbors2d (tensorvote.h:1223-1280).
 //
 // Design: one prep kernel turns each pixel into a voter payload
@@ -184,6 +184,139 @@ __global__ __launch_bounds__(256) void k_vote2(const float4* __restrict__ pay, c
     }
 }
 
+// ---------------------------------------------------------------- analytical tensor voting (ATV)
+// tira::atv 2D (tensorvote.h:159-253): c R H R with R = I - 2 v v^T and
+//   H = (l1 - l0)(T1 - (v^T T1 v T1 + 2 T1 v v^T T1) / 3) + l0 (T0 - (v^T T0 v T0 + 2 T0 v v^T T0) / 4),
+//   T1 = e1 e1^T, T0 = e0 e0^T + e1 e1^T = I.
+// With a = e1.v (rho = |v|^2: 1, or 0 at the voter itself) this collapses to
+//   K11 e1 e1^T + K1 (v e1^T + e1 v^T) + Kvv v v^T + KI I,
+//   K11 = c s1 (1 - a^2), K1 = -2 a K11, Kvv = -c l0 / 2 + 4 a^2 K11, KI = c l0 (1 - rho/4).
+struct Atv2 {  // one window offset, 48 bytes
+    int du, dv;
+    float vx, vy, c, nc, ck4, nhalfc, vxx, vyy, vxy2, pad;
+};
+static_assert(sizeof(Atv2) == 48, "ATV offset entry must stay 48 bytes");
+
+// Window of the reference's ATV loops (tensorvote.h:268-279): hw = w / 2,
+// v (axis 0) outer, u (axis 1) inner.
+static void build_atv2_table(unsigned w, float sigma, std::vector<Atv2>& t) {
+    const int hw = (int)w / 2;
+    t.clear();
+    for (int dv = -hw; dv <= hw; dv++)
+        for (int du = -hw; du <= hw; du++) {
+            const double l2 = (double)du * du + (double)dv * dv, L = std::sqrt(l2);
+            double vx = 0.0, vy = 0.0;
+            if (l2 > 0.0) { vx = du / L; vy = dv / L; }
+            const double c = sigma == 0.0f ? (L > 1e-12 ? 0.0 : 1.0) : std::exp(-l2 / ((double)sigma * sigma));
+            const double k4 = l2 > 0.0 ? 0.75 : 1.0;
+            Atv2 e;
+            e.du = du; e.dv = dv;
+            e.vx = (float)vx; e.vy = (float)vy; e.c = (float)c; e.nc = (float)-c;
+            e.ck4 = (float)(c * k4); e.nhalfc = (float)(-0.5 * c);
+            e.vxx = (float)(vx * vx); e.vyy = (float)(vy * vy); e.vxy2 = (float)(2.0 * vx * vy); e.pad = 0.0f;
+            t.push_back(e);
+        }
+}
+
+// ATV payload from the field: (e1, l1 - l0, l0), e1 the eigenvector of the
+// larger eigenvalue (evals2 / evecs2polar, tensorvote.h:1388-1393).
+__global__ __launch_bounds__(256) void k_vote2_atv_prep(const float4* __restrict__ t, size_t n,
+                                                        float4* __restrict__ pay) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 m = t[i];
+    float l0, l1, vx, vy, ux, uy;
+    eig2_vals(m.x, m.y, m.w, l0, l1);
+    eig2_vecs(m.x, m.y, m.w, l0, vx, vy, ux, uy);
+    pay[i] = make_float4(ux, uy, l1 - l0, l0);
+}
+
+// ATV payload from caller eigenpairs (lambdas, thetas n x 2; e1 = angle
+// theta1, with theta0 perpendicular as evecs2polar_symmetric returns it).
+__global__ __launch_bounds__(256) void k_vote2_atv_prep_eig(const float2* __restrict__ lam,
+                                                            const float2* __restrict__ th, size_t n,
+                                                            float4* __restrict__ pay) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float2 l = lam[i];
+    float sn, cs;
+    sincosf(th[i].y, &sn, &cs);
+    pay[i] = make_float4(cs, sn, l.y - l.x, l.x);
+}
+
+__global__ __launch_bounds__(256) void k_vote2_atv(const float4* __restrict__ pay, const Atv2* __restrict__ tab,
+                                                   int nt, int s0, int s1, float* __restrict__ out) {
+    const int x1 = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int x0 = blockIdx.y * 4 + (threadIdx.x >> 6);
+    float xx = 0.0f, xy2 = 0.0f, yy = 0.0f, ii = 0.0f;
+    for (int k = 0; k < nt; k++) {
+        const Atv2 e = tab[k];
+        const int r0 = x0 + e.dv, r1 = x1 + e.du;
+        if ((unsigned)r0 >= (unsigned)s0 || (unsigned)r1 >= (unsigned)s1) continue;
+        const float4 P = pay[(size_t)r0 * s1 + r1];
+        const float a = fmaf(P.x, e.vx, P.y * e.vy);
+        const float al = a * a;
+        const float K11 = P.z * fmaf(e.nc, al, e.c);
+        const float Kvv = fmaf(4.0f * al, K11, P.w * e.nhalfc);
+        const float K1d = (-4.0f * a) * K11;  // 2 K1
+        const float hx = fmaf(K11, P.x, K1d * e.vx), hy = fmaf(K11, P.y, K1d * e.vy);
+        xx = fmaf(P.x, hx, fmaf(Kvv, e.vxx, xx));
+        yy = fmaf(P.y, hy, fmaf(Kvv, e.vyy, yy));
+        xy2 = fmaf(P.x, hy, fmaf(hx, P.y, fmaf(Kvv, e.vxy2, xy2)));
+        ii = fmaf(P.w, e.ck4, ii);
+    }
+    if (x0 < s0 && x1 < s1) {
+        const float b = 0.5f * xy2;
+        reinterpret_cast<float4*>(out)[(size_t)x0 * s1 + x1] = make_float4(xx + ii, b, b, yy + ii);
+    }
+}
+
+static int vote2_atv_run(const float* tin, const float* lam, const float* th, float* out, float sigma, unsigned w,
+                         unsigned s0, unsigned s1, hipStream_t st) {
+    const size_t n = (size_t)s0 * s1;
+    if (!n) return TIRA_OK;
+    float4* pay = nullptr;
+    TH_TRY(ws_get(WS_PAY0, n * 16, (void**)&pay));
+    {
+        ProfScope prof("vote2_atv_prep", st);
+        if (tin) k_vote2_atv_prep<<<ceil_div(n, 256), 256, 0, st>>>((const float4*)tin, n, pay);
+        else k_vote2_atv_prep_eig<<<ceil_div(n, 256), 256, 0, st>>>((const float2*)lam, (const float2*)th, n, pay);
+    }
+    TH_HIP(hipGetLastError());
+    std::vector<Atv2> tab;
+    build_atv2_table(w, sigma, tab);
+    Atv2* dtab = nullptr;
+    TH_TRY(ws_get(WS_TABLE2, tab.size() * sizeof(Atv2), (void**)&dtab));
+    TH_HIP(hipStreamSynchronize(st));  // pageable source, table slot may still be read by earlier work
+    TH_HIP(hipMemcpy(dtab, tab.data(), tab.size() * sizeof(Atv2), hipMemcpyHostToDevice));
+    dim3 grid(ceil_div(s1, 64), ceil_div(s0, 4));
+    {
+        ProfScope prof("vote2_atv", st);
+        k_vote2_atv<<<grid, 256, 0, st>>>(pay, dtab, (int)tab.size(), (int)s0, (int)s1, out);
+    }
+    TH_HIP(hipGetLastError());
+    return TIRA_OK;
+}
+
+static int vote2_atv_entry(float* t_out, const float* t_in, const float* lam, const float* th, float sigma,
+                           unsigned w, unsigned s0, unsigned s1) {
+    TH_ARG(t_out && (t_in || (lam && th)), "NULL pointer");
+    TH_ARG(sigma >= 0.0f, "sigma must be >= 0");
+    TH_ARG(w <= 1025, "window %u exceeds the supported maximum (1025)", w);
+    const size_t n = (size_t)s0 * s1;
+    if (!n) return TIRA_OK;
+    const float *din = nullptr, *dl = nullptr, *dt = nullptr;
+    void* dout;
+    if (t_in) TH_TRY(stage_in(t_in, n * 16, WS_IN0, (const void**)&din));
+    else {
+        TH_TRY(stage_in(lam, n * 8, WS_IN0, (const void**)&dl));
+        TH_TRY(stage_in(th, n * 8, WS_IN1, (const void**)&dt));
+    }
+    TH_TRY(stage_out(t_out, n * 16, WS_OUT0, &dout));
+    TH_TRY(vote2_atv_run(din, dl, dt, (float*)dout, sigma, w, s0, s1, stream()));
+    return finish_out(t_out, dout, n * 16);
+}
+
 // ---------------------------------------------------------------- host driver
 // plate: the reference's kernel adds the plate term when plate && samples > 0
 // (tensorvote.h:2093); the table's V entries are zero otherwise.
@@ -262,10 +395,8 @@ static int vote2_entry(float* t_out, const float* t_in, const float* lam, const 
 // tira::cuda::tensorvote (2D), tensorvote.h:2200-2226
 int tira_tensorvote2(float* t_out, const float* t_in, float sigma1, float sigma2, int power, unsigned w, unsigned s0,
                      unsigned s1, int stick, int plate, unsigned samples, int is_atv) {
-    if (is_atv) {
-        set_error("analytical tensor voting (is_atv) is not provided by this library");
-        return TIRA_ERR_UNSUPPORTED;
-    }
+    // ATV takes sigma1 only; sigma2, power, stick, plate and samples do not apply (tensorvote.h:2211-2212).
+    if (is_atv) return vote2_atv_entry(t_out, t_in, nullptr, nullptr, sigma1, w, s0, s1);
     TH_ARG(w >= 1, "w must be >= 1");
     std::vector<Nb2> nb;
     const bool use_plate = plate && samples > 0;
@@ -298,6 +429,12 @@ int tira_tensorvote2_tk(float* t_out, const float* lambdas, const float* thetas,
         nbc = build_neighbors2d((int)w, sigma1, sigma2, power < 1 ? 1 : power, samples, stick != 0, use_plate, nb);
     return vote2_entry(t_out, nullptr, lambdas, thetas, nb.data(), nbc, sigma1, sigma2, (int)power, w, s0, s1,
                        stick, plate, samples);
+}
+
+// tira::cuda::tensorvote_atv (2D, from eigenpairs), tensorvote.h:2029-2066
+int tira_tensorvote2_atv(float* t_out, const float* lambdas, const float* thetas, float sigma, unsigned w,
+                         unsigned s0, unsigned s1) {
+    return vote2_atv_entry(t_out, nullptr, lambdas, thetas, sigma, w, s0, s1);
 }
 
 }  // extern "C"

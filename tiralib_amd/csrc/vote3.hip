@@ -1,6 +1,6 @@
-// vote3.hip -- 3D tensor voting (stick + numerically integrated plate) on
-// gfx950, replacing tira::cuda::tensorvote3 / _tensorvote3_tk
-// (tensorvote.h:2412-2633, 2741-2873).
+// vote3.hip -- 3D tensor voting (stick + numerically integrated plate, and
+// analytical tensor voting) on gfx950, replacing tira::cuda::tensorvote3 /
+// _tensorvote3_tk / _tensorvote3_atv (tensorvote.h:2412-2873).
 //
 // Pipeline of one call:
 //  1. k_vote3_prep: eigen-decompose every voxel (FP32, FP64 for near-
@@ -35,6 +35,7 @@ namespace th {
 constexpr int kTileX = 64;      // receivers along x per wave
 constexpr int kTileY = 4;       // waves per block (along y)
 constexpr int kThreads = kTileX * kTileY;
+constexpr int kAtvStride = 24;  // floats per ATV offset entry
 
 // ---------------------------------------------------------------- tables
 static double binom(int n, int k) {
@@ -157,6 +158,33 @@ void build_sum_table(int w, float s1, float s2, unsigned power, std::vector<floa
                 t[0] = d[0]; t[1] = d[1]; t[2] = d[2];
                 t[3] = snf * c1; t[4] = snf * c2;
                 t[5] = c1; t[6] = c2; t[7] = 0.0f;
+            }
+}
+
+// ATV table: unit offset v, attenuation c = exp(-|d|^2 / sigma^2) (delta at
+// sigma = 0, atv_attenuation tensorvote.h:34-40) and the c-scaled factors of
+// the collapsed vote (see atv3_pair); offsets in the order dw, dv, du.
+void build_atv3_table(int w, float sigma, std::vector<float>& tab) {
+    const int hw = w / 2;
+    tab.assign((size_t)w * w * w * kAtvStride, 0.0f);
+    size_t k = 0;
+    for (int dw = -hw; dw <= hw; dw++)
+        for (int dv = -hw; dv <= hw; dv++)
+            for (int du = -hw; du <= hw; du++, k++) {
+                const double l2 = (double)du * du + (double)dv * dv + (double)dw * dw;
+                const double L = std::sqrt(l2);
+                double v[3] = {0.0, 0.0, 0.0};
+                if (l2 > 0.0) { v[0] = du / L; v[1] = dv / L; v[2] = dw / L; }
+                double c;
+                if (sigma == 0.0f) c = L > 1e-12 ? 0.0 : 1.0;
+                else c = std::exp(-l2 / ((double)sigma * sigma));
+                const double rho = l2 > 0.0 ? 1.0 : 0.0, k4 = 1.0 - rho / 4.0, k5 = 1.0 - rho / 5.0;
+                const double f[21] = {v[0], v[1], v[2], c, -c, -0.75 * c, -c * k4, 0.25 * c, c * k4, c * k5,
+                                      -3.0 * c, -0.5 * c, -0.4 * c, 3.0 * c, 2.0 * c,
+                                      v[0] * v[0], v[1] * v[1], v[2] * v[2],
+                                      2.0 * v[0] * v[1], 2.0 * v[0] * v[2], 2.0 * v[1] * v[2]};
+                float* t = &tab[k * kAtvStride];
+                for (int i = 0; i < 21; i++) t[i] = (float)f[i];
             }
 }
 
@@ -766,6 +794,154 @@ __global__ __launch_bounds__(kThreads) void k_vote3_mlp(const float4* __restrict
     if (x < g.s2 && y < g.s1 && z < g.s0) write_out(out, out_layout, ((size_t)z * g.s1 + y) * g.s2 + x, acc);
 }
 
+// ---------------------------------------------------------------- analytical tensor voting (ATV)
+// tira::atv (tensorvote.h:688-760) for voter eigenvalues l0 <= l1 <= l2 and an
+// orthonormal eigenframe (e0, e1, e2) at unit offset v with attenuation c:
+//   c R H R,  R = I - 2 v v^T,
+//   H = sum_k s_k (T_k - (v^T T_k v T_k + 2 T_k v v^T T_k) / (k + 2)),
+//   T_1 = e2 e2^T, T_2 = e1 e1^T + e2 e2^T = I - e0 e0^T, T_3 = I,
+//   s = (l2 - l1, l1 - l0, l0).
+// Every T_k is a projector, so with a = e2.v, b = e0.v the vote collapses to
+//   KI I + K00 e0 e0^T + K22 e2 e2^T + Kvv v v^T + K0 (v e0^T + e0 v^T) + K2 (v e2^T + e2 v^T)
+// with (S = c s, rho = |v|^2: 1, or 0 at the voter itself where the vote is c A)
+//   K22 = S1 (1 - a^2)                 K2  = -2 a K22
+//   K00 = -S2 (1 - rho/4 + 3 b^2/4)    K0  = S2 b (1 + 3 b^2 / 2)
+//   KI  = S2 (1 - rho/4 + b^2/4) + S3 (1 - rho/5)
+//   Kvv = -S2 (1/2 + b^2 + 3 b^4) - 2 S3 / 5 + 4 a^2 K22.
+// The per-offset factors of c live in the table below; off-diagonal
+// accumulators carry twice their value so that the symmetric products need
+// one FMA per term.
+__device__ __forceinline__ void atv3_pair(Acc& acc, const float4 A, const float4 B, const float s3,
+                                          const float* __restrict__ t) {
+    const float v0 = t[0], v1 = t[1], v2 = t[2];
+    const float a = fmaf(B.x, v0, fmaf(B.y, v1, B.z * v2));  // e2 . v
+    const float b = fmaf(A.x, v0, fmaf(A.y, v1, A.z * v2));  // e0 . v
+    const float al = a * a, be = b * b;
+    const float s1 = B.w, s2 = A.w;
+    const float K22 = s1 * fmaf(t[4], al, t[3]);
+    const float K00 = s2 * fmaf(t[5], be, t[6]);
+    const float KI = fmaf(s2, fmaf(t[7], be, t[8]), s3 * t[9]);
+    const float P = fmaf(fmaf(t[10], be, t[4]), be, t[11]);
+    const float Kvv = fmaf(4.0f * al, K22, fmaf(s3, t[12], s2 * P));
+    const float K0d = (s2 * b) * fmaf(t[13], be, t[14]);  // 2 K0
+    const float K2d = (-4.0f * a) * K22;                   // 2 K2
+    const float h00 = fmaf(K00, A.x, K0d * v0), h01 = fmaf(K00, A.y, K0d * v1), h02 = fmaf(K00, A.z, K0d * v2);
+    const float h20 = fmaf(K22, B.x, K2d * v0), h21 = fmaf(K22, B.y, K2d * v1), h22 = fmaf(K22, B.z, K2d * v2);
+    acc.xx = fmaf(A.x, h00, fmaf(B.x, h20, fmaf(Kvv, t[15], acc.xx)));
+    acc.yy = fmaf(A.y, h01, fmaf(B.y, h21, fmaf(Kvv, t[16], acc.yy)));
+    acc.zz = fmaf(A.z, h02, fmaf(B.z, h22, fmaf(Kvv, t[17], acc.zz)));
+    acc.xy = fmaf(A.x, h01, fmaf(h00, A.y, fmaf(B.x, h21, fmaf(h20, B.y, fmaf(Kvv, t[18], acc.xy)))));
+    acc.xz = fmaf(A.x, h02, fmaf(h00, A.z, fmaf(B.x, h22, fmaf(h20, B.z, fmaf(Kvv, t[19], acc.xz)))));
+    acc.yz = fmaf(A.y, h02, fmaf(h01, A.z, fmaf(B.y, h22, fmaf(h21, B.z, fmaf(Kvv, t[20], acc.yz)))));
+    acc.ii += KI;
+}
+
+// ATV gather: same tiling, padding and z register blocking as k_vote3_closed.
+// Payloads: pay_a = (e0, l1 - l0), pay_b = (e2, l2 - l1), pay_c = l0.
+template <int BZ>
+__global__ __launch_bounds__(kThreads) void k_vote3_atv(const float4* __restrict__ pay_a,
+                                                        const float4* __restrict__ pay_b,
+                                                        const float* __restrict__ pay_c,
+                                                        const float* __restrict__ tab, VoteGeom g,
+                                                        float* __restrict__ out, int out_layout) {
+    unsigned bx, by, bz;
+    tile_coords(g, BZ, bx, by, bz);
+    const int x = (int)bx * kTileX + (int)(threadIdx.x % kTileX);
+    const int y = (int)by * kTileY + (int)(threadIdx.x / kTileX);
+    const int z0 = (int)bz * BZ;
+    const int r = g.r, w = 2 * g.r + 1;
+    const size_t plane = (size_t)g.P1 * g.P2;
+
+    Acc acc[BZ];
+#pragma unroll
+    for (int j = 0; j < BZ; j++) acc[j] = Acc{0, 0, 0, 0, 0, 0, 0};
+
+    for (int dv = -r; dv <= r; dv++) {
+        for (int du = -r; du <= r; du++) {
+            const size_t col = ((size_t)z0 * g.P1 + (size_t)(y + r + dv)) * g.P2 + (size_t)(x + r + du);
+            const int ebase = (dv + r) * w + (du + r);
+            for (int zz = 0; zz < BZ + 2 * r; zz++) {
+                const size_t pi = col + (size_t)zz * plane;
+                const float4 A = pay_a[pi], B = pay_b[pi];
+                const float s3 = pay_c[pi];
+#pragma unroll
+                for (int j = 0; j < BZ; j++) {
+                    const int dw = zz - r - j;
+                    if (dw >= -r && dw <= r)
+                        atv3_pair(acc[j], A, B, s3, tab + (size_t)((dw + r) * w * w + ebase) * kAtvStride);
+                }
+            }
+        }
+    }
+    if (x < g.s2 && y < g.s1) {
+#pragma unroll
+        for (int j = 0; j < BZ; j++) {
+            const int z = z0 + j;
+            if (z < g.s0) {
+                Acc o = acc[j];
+                o.xy *= 0.5f; o.xz *= 0.5f; o.yz *= 0.5f;
+                write_out(out, out_layout, ((size_t)z * g.s1 + y) * g.s2 + x, o);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void write_atv_payload(size_t p, float l0, float l1, float l2, const float e0[3],
+                                                  const float e2[3], float4* pay_a, float4* pay_b, float* pay_c) {
+    pay_a[p] = make_float4(e0[0], e0[1], e0[2], l1 - l0);
+    pay_b[p] = make_float4(e2[0], e2[1], e2[2], l2 - l1);
+    pay_c[p] = l0;
+}
+
+// ATV payload from the tensor field (one eigen-decomposition per voxel,
+// padded with zero payloads like k_vote3_prep).
+__global__ __launch_bounds__(256) void k_vote3_atv_prep(const float* __restrict__ in, int layout, VoteGeom g,
+                                                        float4* __restrict__ pay_a, float4* __restrict__ pay_b,
+                                                        float* __restrict__ pay_c) {
+    const size_t total = (size_t)g.P0 * g.P1 * g.P2;
+    const float z3[3] = {0, 0, 0};
+    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < total; p += (size_t)gridDim.x * blockDim.x) {
+        const int px = (int)(p % g.P2), py = (int)((p / g.P2) % g.P1), pz = (int)(p / ((size_t)g.P1 * g.P2));
+        const int x = px - g.r, y = py - g.r, z = pz - g.r;
+        if (x < 0 || y < 0 || z < 0 || x >= g.s2 || y >= g.s1 || z >= g.s0) {
+            write_atv_payload(p, 0.0f, 0.0f, 0.0f, z3, z3, pay_a, pay_b, pay_c);
+            continue;
+        }
+        float l0, l1, l2, e0[3], e2[3];
+        voxel_eig(in, layout, ((size_t)z * g.s1 + y) * g.s2 + x, l0, l1, l2, e0, e2);
+        write_atv_payload(p, l0, l1, l2, e0, e2, pay_a, pay_b, pay_c);
+    }
+}
+
+// ATV payload from caller eigenpairs (_tensorvote3_atv inputs: ascending
+// eigenvalues, middle and largest eigenvectors). The frame is completed with
+// e0 = e1 x e2, which is the smallest eigenvector for the orthonormal pairs
+// evecs3spherical_symmetric returns.
+__global__ __launch_bounds__(256) void k_vote3_atv_prep_eig(const float* __restrict__ lam,
+                                                            const float* __restrict__ ev1,
+                                                            const float* __restrict__ ev2, VoteGeom g,
+                                                            float4* __restrict__ pay_a, float4* __restrict__ pay_b,
+                                                            float* __restrict__ pay_c) {
+    const size_t total = (size_t)g.P0 * g.P1 * g.P2;
+    const float z3[3] = {0, 0, 0};
+    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < total; p += (size_t)gridDim.x * blockDim.x) {
+        const int px = (int)(p % g.P2), py = (int)((p / g.P2) % g.P1), pz = (int)(p / ((size_t)g.P1 * g.P2));
+        const int x = px - g.r, y = py - g.r, z = pz - g.r;
+        if (x < 0 || y < 0 || z < 0 || x >= g.s2 || y >= g.s1 || z >= g.s0) {
+            write_atv_payload(p, 0.0f, 0.0f, 0.0f, z3, z3, pay_a, pay_b, pay_c);
+            continue;
+        }
+        const size_t v = ((size_t)z * g.s1 + y) * g.s2 + x;
+        const float* m = ev1 + v * 3;
+        const float e2[3] = {ev2[v * 3], ev2[v * 3 + 1], ev2[v * 3 + 2]};
+        float e0[3] = {m[1] * e2[2] - m[2] * e2[1], m[2] * e2[0] - m[0] * e2[2], m[0] * e2[1] - m[1] * e2[0]};
+        const float nn = e0[0] * e0[0] + e0[1] * e0[1] + e0[2] * e0[2];
+        const float inv = nn > 0.0f ? rsqrtf(nn) : 0.0f;
+        e0[0] *= inv; e0[1] *= inv; e0[2] *= inv;
+        write_atv_payload(p, lam[v * 3], lam[v * 3 + 1], lam[v * 3 + 2], e0, e2, pay_a, pay_b, pay_c);
+    }
+}
+
 __global__ void k_fill_zero(float* __restrict__ out, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         out[i] = 0.0f;
@@ -820,7 +996,9 @@ static int table_for(const TableKey& key, int P, hipStream_t st, char** dtab, fl
     std::vector<float> tab;
     std::vector<float2> cs;
     size_t tab_bytes, cs_bytes = 0;
-    if (key.sum) {
+    if (key.sum == 2) {
+        tab_bytes = (size_t)key.w * key.w * key.w * kAtvStride * 4;
+    } else if (key.sum) {
         tab_bytes = (size_t)key.w * key.w * key.w * kSumStride * 4;
         cs_bytes = (size_t)key.nsamp * 8;
     } else {
@@ -830,9 +1008,11 @@ static int table_for(const TableKey& key, int P, hipStream_t st, char** dtab, fl
     void* buf = nullptr;
     TH_TRY(ws_get(WS_TABLE, cs_off + cs_bytes + 256, &buf));
     *dtab = (char*)buf;
-    *dcs = key.sum ? (float2*)((char*)buf + cs_off) : nullptr;
+    *dcs = key.sum == 1 ? (float2*)((char*)buf + cs_off) : nullptr;
     if (g_tab.key == key && g_tab.device == dev && g_tab.ptr == buf) return TIRA_OK;
-    if (key.sum) {
+    if (key.sum == 2) {
+        build_atv3_table(key.w, key.s1, tab);
+    } else if (key.sum) {
         build_sum_table(key.w, key.s1, key.s2, key.power, tab);
         cs.resize(key.nsamp);
         const float db = (float)M_PI / (float)key.nsamp;  // dbeta of tensorvote.h:2473
@@ -847,7 +1027,7 @@ static int table_for(const TableKey& key, int P, hipStream_t st, char** dtab, fl
     // `st` that may still read the previous table.
     TH_HIP(hipStreamSynchronize(st));
     TH_HIP(hipMemcpy(buf, tab.data(), tab_bytes, hipMemcpyHostToDevice));
-    if (key.sum) TH_HIP(hipMemcpy((char*)buf + cs_off, cs.data(), cs_bytes, hipMemcpyHostToDevice));
+    if (key.sum == 1) TH_HIP(hipMemcpy((char*)buf + cs_off, cs.data(), cs_bytes, hipMemcpyHostToDevice));
     g_tab.key = key;
     g_tab.device = dev;
     g_tab.ptr = buf;
@@ -923,6 +1103,42 @@ int vote3_run(const float* tensors, int in_layout, const float* lam, const float
             case 3: launch_closed<3>(stick, plate, grid, st, ps, pp, t, g, out, out_layout); break;
             default: launch_closed<4>(stick, plate, grid, st, ps, pp, t, g, out, out_layout); break;
         }
+    }
+    TH_HIP(hipGetLastError());
+    return TIRA_OK;
+}
+
+// ATV on device buffers (tensorvote.h:2635-2739 semantics): `tensors` in
+// `in_layout`, or caller eigenpairs lam / ev1 (middle) / ev2 (largest).
+// The window half-width is w / 2, as the reference's loops take it.
+int vote3_atv_run(const float* tensors, int in_layout, const float* lam, const float* ev1, const float* ev2,
+                  float* out, int out_layout, unsigned s0, unsigned s1, unsigned s2, float sigma, unsigned w,
+                  hipStream_t st) {
+    const size_t nvox = (size_t)s0 * s1 * s2;
+    if (nvox == 0) return TIRA_OK;
+    const unsigned wo = 2 * (w / 2) + 1;
+    const VoteGeom g = make_geom(s0, s1, s2, wo, kBZ);
+    const size_t ptotal = (size_t)g.P0 * g.P1 * g.P2;
+    float4 *pa = nullptr, *pb = nullptr;
+    float* pc = nullptr;
+    TH_TRY(ws_get(WS_PAY0, ptotal * 16, (void**)&pa));
+    TH_TRY(ws_get(WS_PAY1, ptotal * 16, (void**)&pb));
+    TH_TRY(ws_get(WS_SCRATCH0, ptotal * 4, (void**)&pc));
+    const unsigned pgrid = (unsigned)std::min<size_t>(ceil_div(ptotal, 256), 65536);
+    {
+        ProfScope prof("vote3_atv_prep", st);
+        if (tensors) k_vote3_atv_prep<<<pgrid, 256, 0, st>>>(tensors, in_layout, g, pa, pb, pc);
+        else k_vote3_atv_prep_eig<<<pgrid, 256, 0, st>>>(lam, ev1, ev2, g, pa, pb, pc);
+    }
+    TH_HIP(hipGetLastError());
+    const TableKey key{2, (int)wo, sigma, 0.0f, 0u, 0u};
+    char* dtab = nullptr;
+    float2* dcs = nullptr;
+    TH_TRY(table_for(key, 1, st, &dtab, &dcs));
+    const unsigned grid = g.nbx * g.nby * g.nbz;
+    {
+        ProfScope prof("vote3_atv", st);
+        k_vote3_atv<kBZ><<<grid, kThreads, 0, st>>>(pa, pb, pc, (const float*)dtab, g, out, out_layout);
     }
     TH_HIP(hipGetLastError());
     return TIRA_OK;
@@ -1045,6 +1261,14 @@ static int check_vote_args(unsigned s0, unsigned s1, unsigned s2, unsigned w, fl
     return TIRA_OK;
 }
 
+static int check_atv_args(unsigned s0, unsigned s1, unsigned s2, unsigned w, float sigma) {
+    TH_ARG(sigma >= 0.0f, "sigma must be >= 0");
+    TH_ARG((size_t)s0 * s1 * s2 < ((size_t)1 << 40), "volume too large");
+    TH_ARG(w <= 129, "window %u exceeds the supported maximum (129)", w);
+    (void)s0; (void)s1; (void)s2;
+    return TIRA_OK;
+}
+
 }  // namespace th
 
 using namespace th;
@@ -1056,13 +1280,18 @@ int tira_tensorvote3(const float* input_field, float* output_field, unsigned s0,
                      unsigned samples, int is_atv) {
     TH_TRY(use_device(device));
     TH_ARG(input_field && output_field, "NULL pointer");
-    if (is_atv) {
-        set_error("analytical tensor voting (is_atv) is not provided by this library");
-        return TIRA_ERR_UNSUPPORTED;
-    }
-    TH_TRY(check_vote_args(s0, s1, s2, w, sigma, sigma2));
     const size_t n = (size_t)s0 * s1 * s2;
     const float* din; void* dout;
+    if (is_atv) {
+        // ATV takes sigma only; stick, plate, sigma2, power and samples do not apply (tensorvote.h:2860-2861).
+        TH_TRY(check_atv_args(s0, s1, s2, w, sigma));
+        TH_TRY(stage_in(input_field, n * 36, WS_IN0, (const void**)&din));
+        TH_TRY(stage_out(output_field, n * 36, WS_OUT0, &dout));
+        TH_TRY(vote3_atv_run(din, TIRA_LAYOUT_MAT9, nullptr, nullptr, nullptr, (float*)dout, TIRA_LAYOUT_MAT9, s0,
+                             s1, s2, sigma, w, stream()));
+        return finish_out(output_field, dout, n * 36);
+    }
+    TH_TRY(check_vote_args(s0, s1, s2, w, sigma, sigma2));
     TH_TRY(stage_in(input_field, n * 36, WS_IN0, (const void**)&din));
     TH_TRY(stage_out(output_field, n * 36, WS_OUT0, &dout));
     TH_TRY(vote3_run(din, TIRA_LAYOUT_MAT9, nullptr, nullptr, nullptr, (float*)dout, TIRA_LAYOUT_MAT9, s0, s1, s2,
@@ -1116,6 +1345,31 @@ int tira_tensorvote3_async(const float* input_field, int in_layout, float* outpu
     TH_TRY(check_vote_args(s0, s1, s2, w, sigma1, sigma2));
     return vote3_run(input_field, in_layout, nullptr, nullptr, nullptr, output_field, out_layout, s0, s1, s2, sigma1,
                      sigma2, w, power, stick != 0, plate != 0, samples, plate_mode, stream());
+}
+
+// _tensorvote3_atv (tensorvote.h:2695-2739): ATV from eigenpairs.
+int tira_tensorvote3_atv(float* output_field, const float* lambdas, const float* evec1, const float* evec2,
+                         unsigned s0, unsigned s1, unsigned s2, float sigma, unsigned w) {
+    TH_ARG(output_field && lambdas && evec1 && evec2, "NULL pointer");
+    TH_TRY(check_atv_args(s0, s1, s2, w, sigma));
+    const size_t n = (size_t)s0 * s1 * s2;
+    const float *dl, *d1, *d2; void* dout;
+    TH_TRY(stage_in(lambdas, n * 12, WS_IN0, (const void**)&dl));
+    TH_TRY(stage_in(evec1, n * 12, WS_IN1, (const void**)&d1));
+    TH_TRY(stage_in(evec2, n * 12, WS_IN2, (const void**)&d2));
+    TH_TRY(stage_out(output_field, n * 36, WS_OUT0, &dout));
+    TH_TRY(vote3_atv_run(nullptr, 0, dl, d1, d2, (float*)dout, TIRA_LAYOUT_MAT9, s0, s1, s2, sigma, w, stream()));
+    return finish_out(output_field, dout, n * 36);
+}
+
+int tira_tensorvote3_atv_async(const float* input_field, int in_layout, float* output_field, int out_layout,
+                               unsigned s0, unsigned s1, unsigned s2, float sigma, unsigned w) {
+    TH_ARG(input_field && output_field, "NULL pointer");
+    TH_ARG(in_layout == TIRA_LAYOUT_MAT9 || in_layout == TIRA_LAYOUT_PACKED6, "bad input layout");
+    TH_ARG(out_layout == TIRA_LAYOUT_MAT9 || out_layout == TIRA_LAYOUT_PACKED6, "bad output layout");
+    TH_TRY(check_atv_args(s0, s1, s2, w, sigma));
+    return vote3_atv_run(input_field, in_layout, nullptr, nullptr, nullptr, output_field, out_layout, s0, s1, s2,
+                         sigma, w, stream());
 }
 
 }  // extern "C"

@@ -5,7 +5,8 @@
 
 Runs a 1 GiB plain copy (the counter calibration reference: 1 GiB read,
 1 GiB written) three times, then `steps` iterations of: the 3D vote workload
-of bench.py (128^3 tubes + sheets, sigma 2, w 13, N 20); the batched eig3 +
+of bench.py (128^3 tubes + sheets, sigma 2, w 13, N 20) and the analytical
+vote (ATV) of the same volume; the batched eig3 +
 Gaussian launch and the one-launch eig3 of its 48 volumes; the learned plate
 vote on a 96^3 curved-sheet volume (sigma 4, w 25; untrained weights with
 the error check off: the kernel's work does not depend on the weights, and
@@ -53,6 +54,11 @@ def main(argv=None):
         call("tira_eig3_async", vote.ptr, n, _lib.LAYOUT_PACKED6, ev.ptr, None, None, _lib.EIG_SCALE_INVARIANT, None)
     call("tira_hip_synchronize")
     print(f"vote3: {a.steps} steps of 128^3 sigma 2 w 13 N 20", flush=True)
+    for _ in range(a.steps):
+        call("tira_tensorvote3_atv_async", field.ptr, _lib.LAYOUT_MAT9, vote.ptr, _lib.LAYOUT_PACKED6, *shape, 2.0,
+             13)
+    call("tira_hip_synchronize")
+    print(f"atv3: {a.steps} analytical votes of 128^3 sigma 2 w 13", flush=True)
     if not a.skip_batch:
         shapes = synth.batch_shapes(48, seed=synth.SEED_CFG3)
         vols = []
