@@ -57,6 +57,8 @@ typedef enum tira_status {
 #define TIRA_EIG_REFINE_FP64   1u  /* FP64 re-solve of voxels with relative eigengap < 1e-4 */
 #define TIRA_EIG_SCALE_INVARIANT 2u /* thresholds relative to the tensor norm (reference: absolute 1e-6) */
 #define TIRA_EIG_SALIENCY 4u        /* write (l2 - l1, l1 - l0, l0): stick, plate, ball saliency */
+#define TIRA_EIG_MAGNITUDE 8u       /* with SALIENCY: rank by |l| (reconstruct.py:14-34):
+                                       (|l|max - |l|mid, |l|mid - |l|min, |l|min) */
 
 /* ------------------------------------------------------------------ runtime */
 int tira_hip_abi_version(void);
@@ -323,6 +325,36 @@ int tira_plate_mlp_refresh_bins(void* handle);
 int tira_tensorvote3_mlp(void* mlp, const float* input_field, int in_layout, float* output_field, int out_layout,
                          unsigned s0, unsigned s1, unsigned s2, unsigned w, int stick, float tol,
                          unsigned long long* n_fallback);
+
+/* ------------------------------------------------------------------ saliency, threshold, thinning */
+/* Stick saliency |l|max - |l|min of n 2x2 tensors [a b b c] (reconstruct.py:14-34),
+ * device pointers, non-blocking. (3D: tira_eig3_async with TIRA_EIG_SALIENCY |
+ * TIRA_EIG_MAGNITUDE.) */
+int tira_stick_saliency2_async(const float* field4, size_t n, float* saliency);
+/* Otsu threshold (reconstruct.py:37-63) of values[i * stride], i < n, nonnegative
+ * float32: 256-bin histogram on [0, max] with numpy.histogram's float32 binning,
+ * between-class variance in FP64, threshold = the winning bin's centre, written to
+ * *threshold (device). Non-blocking, no host round trip. */
+int tira_otsu_threshold_async(const float* values, size_t n, int stride, float* threshold);
+/* mask[i] = values[i * stride] > *threshold (reconstruct.py:87), device pointers. */
+int tira_threshold_async(const float* values, size_t n, int stride, const float* threshold, unsigned char* mask);
+/* lee(in, out, x, y, z) / computeThinImage, thinning/lee_thinning.h:600-706:
+ * Lee et al. 1994 3D thinning of an int volume (x fastest; nonzero = object),
+ * bit-exact, output 0/1. Host or device pointers; blocking. *iterations (may
+ * be NULL) receives the number of outer passes. */
+int tira_lee_thin3(const int* in, int* out, unsigned x, unsigned y, unsigned z, unsigned* iterations);
+/* The same on a device byte mask of s0 x s1 x s2 (s2 fastest), in place. */
+int tira_lee_thin3_u8(unsigned char* mask, unsigned s0, unsigned s1, unsigned s2, unsigned* iterations);
+/* Neighbourhood tests of the thinning for codes first .. first+count-1 (27-bit
+ * code = low 13 bits | centre bit 13 | high bits << 14; bit 9(dz+1)+3(dy+1)+(dx+1)):
+ * flags bit 0 Euler invariant (lee_thinning.h:288-299), bit 1 simple (530-579),
+ * bit 2 end point (96-103). Device output, non-blocking. */
+int tira_thin3_classify_async(unsigned first, unsigned count, unsigned char* flags);
+/* Voted field (device, layout) -> saliency3 (n x 3 magnitude-ranked saliency,
+ * device) -> Otsu threshold of the stick saliency (*threshold, device) ->
+ * skeleton (device bytes) = Lee thinning of saliency > threshold. Blocking. */
+int tira_skeleton3(const float* voted, int layout, unsigned s0, unsigned s1, unsigned s2, float* saliency3,
+                   float* threshold, unsigned char* skeleton, unsigned* iterations);
 
 #ifdef __cplusplus
 }

@@ -100,6 +100,14 @@ def lib():
                                                    C.c_void_p, _sz, _i])
         _sig(L, "oracle_tensorvote3_atv_f32", [_f32p, _f32p, _u, _u, _u, _f, _u, C.c_void_p, _sz, _i])
         _sig(L, "oracle_tensorvote3_atv_f64", [_f32p, _f64p, _u, _u, _u, _d, _u, C.c_void_p, _sz, _i])
+        _sig(L, "oracle_otsu_threshold_f32", [_f32p, _sz], _f)
+        _sig(L, "oracle_thin_euler_lut_sum", [_u], _i)
+        _sig(L, "oracle_thin_delta_euler", [_u], _i)
+        _sig(L, "oracle_thin_simple_octree", [_u], _i)
+        _sig(L, "oracle_thin_simple_bfs", [_u], _i)
+        _sig(L, "oracle_thin_exhaustive", [C.POINTER(C.c_ulonglong), _i])
+        _sig(L, "oracle_lee_thin3", [C.c_void_p, _u, _u, _u], _i)
+        _sig(L, "oracle_thin_classify", [_u, _u, C.c_void_p, _i])
         _lib = L
     return _lib
 
@@ -392,6 +400,34 @@ def tensorvote3_atv_f64(vol9, sigma, w, recv=None, nthreads=1):
     out = np.zeros(((nr if recv is not None else s0 * s1 * s2), 9), np.float64)
     lib().oracle_tensorvote3_atv_f64(v.reshape(-1), out, s0, s1, s2, float(sigma), int(w), rp, nr, int(nthreads))
     return out if recv is not None else out.reshape(s0, s1, s2, 9)
+
+
+# ---------------------------------------------------------------- saliency threshold, Lee thinning
+def otsu_threshold(values):
+    """Otsu threshold of float32 values (reconstruct.py:37-63 on a float32 array)."""
+    v = _c32(values).reshape(-1)
+    return lib().oracle_otsu_threshold_f32(v, v.size)
+
+
+def lee_thin3(vol):
+    """computeThinImage (lee_thinning.h:600-680) of a (z, y, x) volume; nonzero = object.
+    Returns (uint8 0/1 volume, outer iterations)."""
+    v = np.ascontiguousarray(np.asarray(vol) != 0, dtype=np.uint8)
+    s0, s1, s2 = v.shape
+    it = lib().oracle_lee_thin3(v.ctypes.data, s0, s1, s2)
+    return v, it
+
+
+def thin_exhaustive(nthreads=8):
+    out = (C.c_ulonglong * 2)()
+    lib().oracle_thin_exhaustive(out, int(nthreads))
+    return int(out[0]), int(out[1])
+
+
+def thin_classify(first, count, nthreads=8):
+    f = np.empty(count, np.uint8)
+    lib().oracle_thin_classify(int(first), int(count), f.ctypes.data, int(nthreads))
+    return f
 
 
 # ---------------------------------------------------------------- Gaussian

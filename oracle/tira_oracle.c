@@ -1488,3 +1488,277 @@ void oracle_tensorvote3_atv_f64(const float* in9, double* out9, unsigned s0, uns
     }
     free(L); free(E0); free(E1); free(E2);
 }
+
+/* ======================================================================
+ * Saliency threshold (reconstruct.py:14-63) and Lee 3D thinning
+ * (thinning/lee_thinning.h:112-139, 185-598, 600-706)
+ *
+ * A 3x3x3 neighbourhood is a 27-bit code, bit n = voxel
+ * (dx, dy, dz) with n = 9 (dz + 1) + 3 (dy + 1) + (dx + 1), the order of
+ * get_neighborhood (lee_thinning.h:52-91); bit 13 is the centre.
+ * ====================================================================== */
+
+/* Otsu threshold of nonnegative float32 values, reconstruct.py:37-63 with
+ * numpy.histogram(values, 256, range=(0, max)) semantics for float32 input:
+ * float32 edges (float64 linspace rounded), bin index from the float64
+ * quotient with the +-1 edge corrections, float32 bin centres, float64
+ * cumulative sums, first maximum of the between-class variance. */
+float oracle_otsu_threshold_f32(const float* v, size_t n) {
+    if (n == 0) return 0.0f;
+    float vmax = v[0];
+    for (size_t i = 1; i < n; i++) if (v[i] > vmax) vmax = v[i];
+    if (!(vmax > 0.0f)) return 0.0f;
+    float edges[257];
+    const double step = (double)vmax / 256.0;
+    for (int i = 0; i < 256; i++) edges[i] = (float)((double)i * step);
+    edges[256] = vmax;
+    double hist[256];
+    for (int i = 0; i < 256; i++) hist[i] = 0.0;
+    for (size_t k = 0; k < n; k++) {
+        const float a = v[k];
+        if (!(a >= 0.0f && a <= vmax)) continue;
+        long idx = (long)(((double)a / (double)vmax) * 256.0);
+        if (idx == 256) idx = 255;
+        if (a < edges[idx]) idx--;
+        if (idx != 255 && a >= edges[idx + 1]) idx++;
+        hist[idx] += 1.0;
+    }
+    double total = 0.0;
+    for (int i = 0; i < 256; i++) total += hist[i];
+    if (total == 0.0) return 0.0f;
+    float centers[256];
+    for (int i = 0; i < 256; i++) centers[i] = (edges[i] + edges[i + 1]) / 2.0f;
+    double cum[256], w0[256], acc = 0.0, wacc = 0.0;
+    for (int i = 0; i < 256; i++) {
+        acc += hist[i] * (double)centers[i];
+        wacc += hist[i];
+        cum[i] = acc; w0[i] = wacc;
+    }
+    const double mu_total = cum[255];
+    int best = 0;
+    double bv = 0.0;
+    for (int i = 0; i < 256; i++) {
+        const double w1 = total - w0[i];
+        const double mu0 = w0[i] > 0 ? cum[i] / w0[i] : 0.0;
+        const double mu1 = w1 > 0 ? (mu_total - cum[i]) / w1 : 0.0;
+        const double d = mu0 - mu1;
+        const double b = w0[i] * w1 * (d * d);
+        if (i == 0 || b > bv) { bv = b; best = i; }
+    }
+    return centers[best];
+}
+
+/* Euler characteristic LUT of Lee et al. 1994 for 26-connectivity, indexed by
+ * a 2x2x2 octant code (lee_thinning.h:112-139); only odd entries are used. */
+static const signed char o_euler_lut[256] = {
+    0, 1, 0, -1, 0, -1, 0, 1, 0, -3, 0, -1, 0, -1, 0, 1, 0, -1, 0, 1, 0, 1, 0, -1, 0, 3, 0, 1, 0, 1, 0, -1,
+    0, -3, 0, -1, 0, 3, 0, 1, 0, 1, 0, -1, 0, 3, 0, 1, 0, -1, 0, 1, 0, 1, 0, -1, 0, 3, 0, 1, 0, 1, 0, -1,
+    0, -3, 0, 3, 0, -1, 0, 1, 0, 1, 0, 3, 0, -1, 0, 1, 0, -1, 0, 1, 0, 1, 0, -1, 0, 3, 0, 1, 0, 1, 0, -1,
+    0, 1, 0, 3, 0, 3, 0, 1, 0, 5, 0, 3, 0, 3, 0, 1, 0, -1, 0, 1, 0, 1, 0, -1, 0, 3, 0, 1, 0, 1, 0, -1,
+    0, -7, 0, -1, 0, -1, 0, 1, 0, -3, 0, -1, 0, -1, 0, 1, 0, -1, 0, 1, 0, 1, 0, -1, 0, 3, 0, 1, 0, 1, 0, -1,
+    0, -3, 0, -1, 0, 3, 0, 1, 0, 1, 0, -1, 0, 3, 0, 1, 0, -1, 0, 1, 0, 1, 0, -1, 0, 3, 0, 1, 0, 1, 0, -1,
+    0, -3, 0, 3, 0, -1, 0, 1, 0, 1, 0, 3, 0, -1, 0, 1, 0, -1, 0, 1, 0, 1, 0, -1, 0, 3, 0, 1, 0, 1, 0, -1,
+    0, 1, 0, 3, 0, 3, 0, 1, 0, 5, 0, 3, 0, 3, 0, 1, 0, -1, 0, 1, 0, 1, 0, -1, 0, 3, 0, 1, 0, 1, 0, -1};
+
+/* Octant index of the Euler test: for each octant the neighbourhood bits
+ * that set code bits 7..1 (bit 0, the centre, is always set), in the order
+ * of index_octant_SWU, SEU, NWU, NEU, SWB, SEB, NWB, NEB (lee_thinning.h:185-283). */
+static const unsigned char o_octant_bits[8][7] = {
+    {24, 25, 15, 16, 21, 22, 12}, {26, 23, 17, 14, 25, 22, 16},
+    {18, 21, 9, 12, 19, 22, 10}, {20, 23, 19, 22, 11, 14, 10},
+    {6, 15, 7, 16, 3, 12, 4}, {8, 7, 17, 16, 5, 4, 14},
+    {0, 9, 3, 12, 1, 10, 4}, {2, 1, 11, 10, 5, 4, 14}};
+
+/* is_euler_invariant (lee_thinning.h:288-299): sum of the octant LUT values. */
+int oracle_thin_euler_lut_sum(unsigned code) {
+    int s = 0;
+    for (int o = 0; o < 8; o++) {
+        unsigned idx = 1;
+        for (int k = 0; k < 7; k++)
+            if (code >> o_octant_bits[o][k] & 1u) idx |= 128u >> k;
+        s += o_euler_lut[idx];
+    }
+    return s;
+}
+
+/* Independent route: change of the Euler characteristic V - E + F - C of the
+ * union of closed unit cubes when the centre cube is removed. Only cells of
+ * the centre cube that no other set voxel's cube contains disappear. */
+int oracle_thin_delta_euler(unsigned code) {
+    int d = -1; /* the centre cube itself */
+    for (int sx = -1; sx <= 1; sx++)
+        for (int sy = -1; sy <= 1; sy++)
+            for (int sz = -1; sz <= 1; sz++) {
+                /* a cell of the centre cube is identified by which axes it sits on a face of:
+                 * s = 0 means the cell spans that axis, +-1 the side. Voxels sharing the cell
+                 * are those offset by 0 or s on each axis. */
+                const int dim = (sx == 0) + (sy == 0) + (sz == 0);
+                if (dim == 3) continue;
+                int covered = 0;
+                for (int ox = (sx < 0 ? sx : 0); ox <= (sx > 0 ? sx : 0) && !covered; ox++)
+                    for (int oy = (sy < 0 ? sy : 0); oy <= (sy > 0 ? sy : 0) && !covered; oy++)
+                        for (int oz = (sz < 0 ? sz : 0); oz <= (sz > 0 ? sz : 0) && !covered; oz++) {
+                            if (!ox && !oy && !oz) continue;
+                            const int n = 9 * (oz + 1) + 3 * (oy + 1) + (ox + 1);
+                            if (code >> n & 1u) covered = 1;
+                        }
+                if (!covered) d += (dim % 2 == 0) ? 1 : -1;
+            }
+    return d;
+}
+
+/* Octree labelling (lee_thinning.h:304-518) restated as a flood fill over
+ * the octant membership it encodes: two neighbours are connected when an
+ * octant (2x2x2 block with the centre) holds both. is_simple_point
+ * (lee_thinning.h:530-579): at most one component among the 26 neighbours. */
+static void o_octree_fill(int oct, int label, int* cube) {
+    for (int n = 0; n < 27; n++) {
+        if (n == 13) continue;
+        const int dx = n % 3 - 1, dy = (n / 3) % 3 - 1, dz = n / 9 - 1;
+        const int ox = oct & 1, oy = (oct >> 1) & 1, oz = (oct >> 2) & 1; /* 0: minus side, 1: plus side */
+        if ((ox ? dx < 0 : dx > 0) || (oy ? dy < 0 : dy > 0) || (oz ? dz < 0 : dz > 0)) continue;
+        const int c = n < 13 ? n : n - 1;
+        if (cube[c] != 1) continue;
+        cube[c] = label;
+        for (int o2 = 0; o2 < 8; o2++) {
+            if (o2 == oct) continue;
+            const int px = o2 & 1, py = (o2 >> 1) & 1, pz = (o2 >> 2) & 1;
+            if ((px ? dx < 0 : dx > 0) || (py ? dy < 0 : dy > 0) || (pz ? dz < 0 : dz > 0)) continue;
+            o_octree_fill(o2, label, cube);
+        }
+    }
+}
+
+int oracle_thin_simple_octree(unsigned code) {
+    int cube[26];
+    for (int n = 0, c = 0; n < 27; n++)
+        if (n != 13) cube[c++] = (int)(code >> n & 1u);
+    int label = 2;
+    for (int c = 0; c < 26; c++) {
+        if (cube[c] != 1) continue;
+        const int n = c < 13 ? c : c + 1;
+        const int dx = n % 3 - 1, dy = (n / 3) % 3 - 1, dz = n / 9 - 1;
+        const int oct = (dx > 0) | ((dy > 0) << 1) | ((dz > 0) << 2); /* an octant holding it */
+        o_octree_fill(oct, label, cube);
+        label++;
+        if (label - 2 >= 2) return 0;
+    }
+    return 1;
+}
+
+/* Independent route: breadth-first 26-connected components. */
+int oracle_thin_simple_bfs(unsigned code) {
+    unsigned obj = code & ~(1u << 13) & 0x7FFFFFFu;
+    int comps = 0;
+    while (obj) {
+        int stack[27], sp = 0;
+        int s = __builtin_ctz(obj);
+        obj &= ~(1u << s);
+        stack[sp++] = s;
+        while (sp) {
+            const int n = stack[--sp];
+            const int x = n % 3, y = (n / 3) % 3, z = n / 9;
+            for (int m = 0; m < 27; m++) {
+                if (!(obj >> m & 1u)) continue;
+                const int mx = m % 3, my = (m / 3) % 3, mz = m / 9;
+                if (abs(mx - x) <= 1 && abs(my - y) <= 1 && abs(mz - z) <= 1) {
+                    obj &= ~(1u << m);
+                    stack[sp++] = m;
+                }
+            }
+        }
+        if (++comps >= 2) return 0;
+    }
+    return 1;
+}
+
+/* Exhaustive comparison over all 2^26 neighbourhoods (centre set):
+ * out[0] = #codes where (LUT sum == 0) != (delta Euler == 0),
+ * out[1] = #codes where octree and BFS simple tests differ. */
+void oracle_thin_exhaustive(unsigned long long* out, int nthreads) {
+    unsigned long long m0 = 0, m1 = 0;
+#pragma omp parallel for schedule(static, 65536) reduction(+ : m0, m1) num_threads(nthreads > 0 ? nthreads : 1)
+    for (long long k = 0; k < (1ll << 26); k++) {
+        const unsigned low = (unsigned)(k & 0x1FFF), high = (unsigned)(k >> 13);
+        const unsigned code = low | (1u << 13) | (high << 14);
+        m0 += (oracle_thin_euler_lut_sum(code) == 0) != (oracle_thin_delta_euler(code) == 0);
+        m1 += oracle_thin_simple_octree(code) != oracle_thin_simple_bfs(code);
+    }
+    out[0] = m0; out[1] = m1;
+}
+
+static unsigned o_thin_code(const unsigned char* v, int s0, int s1, int s2, int x, int y, int z) {
+    unsigned code = 0;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                const int X = x + dx, Y = y + dy, Z = z + dz;
+                if (X < 0 || Y < 0 || Z < 0 || X >= s2 || Y >= s1 || Z >= s0) continue;
+                if (v[((size_t)Z * s1 + Y) * s2 + X] == 1) code |= 1u << (9 * (dz + 1) + 3 * (dy + 1) + dx + 1);
+            }
+    return code;
+}
+
+/* computeThinImage (lee_thinning.h:600-680) on a 0/1 volume, s2 (x) fastest,
+ * s1 = y, s0 = z. Border directions 1..6 = N (y-1), S (y+1), E (x+1),
+ * W (x-1), U (z+1), B (z-1). Candidates of one direction are collected on
+ * the unchanged volume, then re-checked for simplicity and deleted one by
+ * one in raster order. Returns the number of outer iterations. */
+int oracle_lee_thin3(unsigned char* v, unsigned S0, unsigned S1, unsigned S2) {
+    const int s0 = (int)S0, s1 = (int)S1, s2 = (int)S2;
+    const size_t n = (size_t)s0 * s1 * s2;
+    for (size_t i = 0; i < n; i++) v[i] = v[i] != 0; /* lee(): binarise */
+    size_t cap = 1024, cnt = 0;
+    size_t* list = (size_t*)malloc(cap * sizeof(size_t));
+    static const int dirs[6][3] = {{0, -1, 0}, {0, 1, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 0, 1}, {0, 0, -1}};
+    int unchanged = 0, iterations = 0;
+    while (unchanged < 6) {
+        unchanged = 0;
+        iterations++;
+        for (int b = 0; b < 6; b++) {
+            int nochange = 1;
+            cnt = 0;
+            for (int z = 0; z < s0; z++)
+                for (int y = 0; y < s1; y++)
+                    for (int x = 0; x < s2; x++) {
+                        const size_t i = ((size_t)z * s1 + y) * s2 + x;
+                        if (v[i] != 1) continue;
+                        const int X = x + dirs[b][0], Y = y + dirs[b][1], Z = z + dirs[b][2];
+                        const int nb = (X < 0 || Y < 0 || Z < 0 || X >= s2 || Y >= s1 || Z >= s0)
+                                           ? 0 : v[((size_t)Z * s1 + Y) * s2 + X];
+                        if (nb > 0) continue;
+                        const unsigned code = o_thin_code(v, s0, s1, s2, x, y, z);
+                        if (__builtin_popcount(code) - 1 == 1) continue; /* is_endpoint */
+                        if (oracle_thin_euler_lut_sum(code) != 0) continue;
+                        if (!oracle_thin_simple_octree(code)) continue;
+                        if (cnt == cap) { cap *= 2; list = (size_t*)realloc(list, cap * sizeof(size_t)); }
+                        list[cnt++] = i;
+                    }
+            for (size_t k = 0; k < cnt; k++) {
+                const size_t i = list[k];
+                const int x = (int)(i % s2), y = (int)((i / s2) % s1), z = (int)(i / ((size_t)s1 * s2));
+                if (oracle_thin_simple_octree(o_thin_code(v, s0, s1, s2, x, y, z))) {
+                    v[i] = 0;
+                    nochange = 0;
+                }
+            }
+            if (nochange) unchanged++;
+        }
+    }
+    free(list);
+    return iterations;
+}
+
+/* Flags of codes first .. first+count-1, enumerated with the centre set
+ * (code = low 13 bits | 1 << 13 | high << 14): bit 0 Euler invariant (LUT),
+ * bit 1 simple (octree), bit 2 end point. */
+void oracle_thin_classify(unsigned first, unsigned count, unsigned char* flags, int nthreads) {
+#pragma omp parallel for schedule(static, 65536) num_threads(nthreads > 0 ? nthreads : 1)
+    for (long long k = 0; k < (long long)count; k++) {
+        const unsigned e = first + (unsigned)k;
+        const unsigned code = (e & 0x1FFFu) | (1u << 13) | ((e >> 13) << 14);
+        flags[k] = (unsigned char)((oracle_thin_euler_lut_sum(code) == 0 ? 1 : 0) |
+                                   (oracle_thin_simple_octree(code) ? 2 : 0) |
+                                   (__builtin_popcount(code) == 2 ? 4 : 0));
+    }
+}

@@ -162,6 +162,20 @@ void oracle_tensorvote3_atv_f32(const float* in9, float* out9, unsigned s0, unsi
 void oracle_tensorvote3_atv_f64(const float* in9, double* out9, unsigned s0, unsigned s1, unsigned s2,
                                 double sigma, unsigned w, const int64_t* recv, size_t nrecv, int nthreads);
 
+/* ---------------- saliency threshold and Lee thinning ---------------- */
+/* Otsu threshold (reconstruct.py:37-63, numpy.histogram semantics for float32 values). */
+float oracle_otsu_threshold_f32(const float* values, size_t n);
+/* 27-bit neighbourhood codes (bit 9(dz+1)+3(dy+1)+(dx+1)); Euler LUT sum, the
+ * independent Euler change, octree and BFS simple-point tests. */
+int oracle_thin_euler_lut_sum(unsigned code);
+int oracle_thin_delta_euler(unsigned code);
+int oracle_thin_simple_octree(unsigned code);
+int oracle_thin_simple_bfs(unsigned code);
+void oracle_thin_exhaustive(unsigned long long* out2, int nthreads);
+/* computeThinImage (lee_thinning.h:600-680) on a 0/1 byte volume in place. */
+int oracle_lee_thin3(unsigned char* vol, unsigned s0, unsigned s1, unsigned s2);
+void oracle_thin_classify(unsigned first, unsigned count, unsigned char* flags, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -32,6 +32,7 @@ PLATE_MLP = 3
 EIG_REFINE_FP64 = 1
 EIG_SCALE_INVARIANT = 2
 EIG_SALIENCY = 4
+EIG_MAGNITUDE = 8
 
 MLP_NPARAM = 12288
 MLP_NBINS = 2048
@@ -95,6 +96,13 @@ SIGNATURES = {
     "tira_tensorvote3_tk": [_vp, _vp, _vp, _vp, _u, _u, _u, _f, _f, _u, _u, _i, _i, _u],
     "tira_tensorvote3_async": [_vp, _i, _vp, _i, _u, _u, _u, _f, _f, _u, _u, _i, _i, _u, _i],
     "tira_tensorvote2_atv": [_vp, _vp, _vp, _f, _u, _u, _u],
+    "tira_stick_saliency2_async": [_vp, _sz, _vp],
+    "tira_otsu_threshold_async": [_vp, _sz, _i, _vp],
+    "tira_threshold_async": [_vp, _sz, _i, _vp, _vp],
+    "tira_lee_thin3": [_vp, _vp, _u, _u, _u, _vp],
+    "tira_lee_thin3_u8": [_vp, _u, _u, _u, _vp],
+    "tira_thin3_classify_async": [_u, _u, _vp],
+    "tira_skeleton3": [_vp, _i, _u, _u, _u, _vp, _vp, _vp, _vp],
     "tira_tensorvote3_atv": [_vp, _vp, _vp, _vp, _u, _u, _u, _f, _u],
     "tira_tensorvote3_atv_async": [_vp, _i, _vp, _i, _u, _u, _u, _f, _u],
     "tira_gaussian_filter3d": [_vp, _u, _u, _u, _f, _f, _f, _vp, C.POINTER(_u), C.POINTER(_u), C.POINTER(_u)],

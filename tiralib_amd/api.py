@@ -462,3 +462,70 @@ def tensorvote_atv(lambdas, thetas, shape, sigma, w):
     out = _like(l, (H, W, 4))
     call("tira_tensorvote2_atv", ptr(out), ptr(l), ptr(t), float(sigma), int(w), H, W)
     return out
+
+
+# ------------------------------------------------------------------ saliency, threshold, thinning
+def _dev(a):
+    """(device array, owned) for a host array or an existing DeviceArray."""
+    if isinstance(a, DeviceArray):
+        return a, False
+    return DeviceArray.from_host(np.ascontiguousarray(a)), True
+
+
+def stick_saliency2(field):
+    """Stick saliency |l|max - |l|min of an (H, W, 4) or (H, W, 2, 2) field (reconstruct.py:14-34), float32."""
+    f = _f32(field)
+    H, W = f.shape[0], f.shape[1]
+    f = f.reshape(H, W, 4)
+    d, o = DeviceArray.from_host(f), DeviceArray((H, W))
+    try:
+        call("tira_stick_saliency2_async", d.ptr, H * W, o.ptr)
+        call("tira_hip_synchronize")
+        return o.to_host()
+    finally:
+        d.free(); o.free()
+
+
+def otsu_threshold(values, stride=1, n=None):
+    """Otsu threshold (reconstruct.py:37-63) of values[::stride] (host array or DeviceArray)."""
+    d, own = _dev(np.asarray(values, np.float32) if not isinstance(values, DeviceArray) else values)
+    if n is None:
+        n = int(np.prod(d.shape)) // int(stride)
+    t = DeviceArray((1,))
+    try:
+        call("tira_otsu_threshold_async", d.ptr, int(n), int(stride), t.ptr)
+        call("tira_hip_synchronize")
+        return float(t.to_host()[0])
+    finally:
+        t.free()
+        if own:
+            d.free()
+
+
+def saliency_threshold2(T, threshold="otsu"):
+    """reconstruct.saliency_threshold(T, fill=False) on the GPU: bool mask (H, W)."""
+    s = stick_saliency2(T)
+    thr = otsu_threshold(s) if threshold == "otsu" else float(threshold)
+    return s > thr
+
+
+def lee_thin3(volume):
+    """tira::lee / computeThinImage (lee_thinning.h:600-706) of a (z, y, x) volume.
+    Returns (int32 0/1 volume, outer iterations)."""
+    v = np.ascontiguousarray(volume, dtype=np.int32)
+    s0, s1, s2 = v.shape
+    out = np.empty_like(v)
+    it = C.c_uint()
+    call("tira_lee_thin3", ptr(v), ptr(out), s2, s1, s0, C.byref(it))
+    return out, int(it.value)
+
+
+def skeleton3(voted, shape, layout=_lib.LAYOUT_PACKED6):
+    """Voted field (DeviceArray) -> magnitude-ranked saliency, Otsu threshold of the
+    stick saliency, Lee-thinned mask. Returns dict of DeviceArrays and numbers."""
+    s0, s1, s2 = shape
+    n = s0 * s1 * s2
+    sal, thr, sk = DeviceArray((n, 3)), DeviceArray((1,)), DeviceArray((s0, s1, s2), np.uint8)
+    it = C.c_uint()
+    call("tira_skeleton3", voted.ptr, int(layout), s0, s1, s2, sal.ptr, thr.ptr, sk.ptr, C.byref(it))
+    return {"saliency": sal, "threshold": float(thr.to_host()[0]), "skeleton": sk, "iterations": int(it.value)}

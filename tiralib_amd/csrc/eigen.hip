@@ -66,10 +66,17 @@ __device__ __forceinline__ void global_tensor(const float* g, size_t i, float& a
     }
 }
 
-// Eigenvalues l0 <= l1 <= l2, or the saliency maps (l2 - l1, l1 - l0, l0):
-// stick, plate and ball saliency of a voted field.
-__device__ __forceinline__ void store_eig(float* o, float l0, float l1, float l2, bool saliency) {
-    if (saliency) { o[0] = l2 - l1; o[1] = l1 - l0; o[2] = l0; }
+// Eigenvalues l0 <= l1 <= l2 (mode 0), the saliency maps (l2 - l1, l1 - l0,
+// l0) -- stick, plate and ball saliency of a voted field (mode 1) -- or the
+// same with the eigenvalues ranked by magnitude, m0 <= m1 <= m2 of |l|
+// (mode 2, the ranking of reconstruct.py:14-34).
+__device__ __forceinline__ void store_eig(float* o, float l0, float l1, float l2, int mode) {
+    if (mode == 2) {
+        const float a = fabsf(l0), b = fabsf(l1), c = fabsf(l2);
+        const float m2 = fmaxf(a, fmaxf(b, c)), m0 = fminf(a, fminf(b, c));
+        const float m1 = __builtin_amdgcn_fmed3f(a, b, c);
+        o[0] = m2 - m1; o[1] = m1 - m0; o[2] = m0;
+    } else if (mode == 1) { o[0] = l2 - l1; o[1] = l1 - l0; o[2] = l0; }
     else { o[0] = l0; o[1] = l1; o[2] = l2; }
 }
 
@@ -160,7 +167,7 @@ __global__ __launch_bounds__(kEigThreads) void k_eig3(const EigBatchArgs args, u
     __syncthreads();
 
     const bool scale_inv = (flags & TIRA_EIG_SCALE_INVARIANT) != 0;
-    const bool saliency = (flags & TIRA_EIG_SALIENCY) != 0;
+    const int saliency = (flags & TIRA_EIG_SALIENCY) ? ((flags & TIRA_EIG_MAGNITUDE) ? 2 : 1) : 0;
     const float deps = scale_inv ? 1e-30f : 1e-6f;
 
     // ---- FP32 pass
