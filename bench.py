@@ -44,6 +44,47 @@ def profiled_traffic(kernel):
         return None
 
 
+SPECTRAL_KERNELS = ("spec_fwd_x", "spec_fwd_y", "spec_fwd_z", "spec_mul", "spec_inv_z", "spec_inv_y", "spec_inv_x")
+
+
+def spectral_bytes(shape, w, in_bytes=36, out_bytes=24, nf=30):
+    """Algorithmic HBM bytes per launch of each kernel of the spectral vote:
+    every array a pass reads or writes, once (zero padding that the passes
+    skip is not counted). Features: nf real channels packed two per complex
+    volume of FFT lengths L0 x L1 x L2 (8 B per complex value); kernel spectra
+    6 nf octant values (4 B); payload 32 B per voxel; output packed-6."""
+    from tiralib_amd import api
+    n0, n1, n2 = shape
+    L0, L1, L2 = (api.spectral_length(n, w) for n in shape)
+    h0, h1, h2 = L0 // 2 + 1, L1 // 2 + 1, L2 // 2 + 1
+    nz, N, c = (nf + 1) // 2, n0 * n1 * n2, 8
+    return {
+        "vote3_prep": N * (in_bytes + 32),
+        "spec_fwd_x": N * 32 + nz * n0 * n1 * L2 * c,
+        "spec_fwd_y": nz * n0 * n1 * L2 * c + nz * n0 * L1 * L2 * c,
+        "spec_fwd_z": nz * n0 * L1 * L2 * c + nz * L0 * L1 * L2 * c,
+        "spec_mul": nz * L0 * L1 * L2 * c + 6 * nf * h0 * h1 * h2 * 4 + 3 * L0 * L1 * L2 * c,
+        "spec_inv_z": 3 * L0 * L1 * L2 * c + 3 * n0 * L1 * L2 * c,
+        "spec_inv_y": 3 * n0 * L1 * L2 * c + 3 * n0 * n1 * L2 * c,
+        "spec_inv_x": 3 * n0 * n1 * L2 * c + N * out_bytes,
+    }
+
+
+def spectral_roofline(shape, w, stage_ms, in_bytes=36):
+    """Per-kernel achieved HBM bandwidth of the spectral vote and the dominant kernel."""
+    b = spectral_bytes(shape, w, in_bytes)
+    per = {k: {"ms": round(stage_ms[k], 4), "bytes": int(b[k]),
+               "gbs": round(b[k] / (stage_ms[k] * 1e-3) / 1e9, 1),
+               "frac": round(b[k] / (stage_ms[k] * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+           for k in SPECTRAL_KERNELS if stage_ms.get(k)}
+    tot_b = sum(b[k] for k in per)
+    tot_ms = sum(stage_ms[k] for k in per)
+    dom = max(per, key=lambda k: per[k]["ms"]) if per else None
+    return per, dom, {"bytes": int(tot_b), "ms": round(tot_ms, 4),
+                      "gbs": round(tot_b / (tot_ms * 1e-3) / 1e9, 1) if tot_ms else None,
+                      "frac": round(tot_b / (tot_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4) if tot_ms else None}
+
+
 def in_bounds_pairs(shape, w):
     r = (w - 1) // 2
     p = 1
@@ -192,7 +233,7 @@ def bench_mlp(steps, check=True, log=None, keep_net=False):
     train_s = time.perf_counter() - t
     be = net.bin_errors()
     say(f"mlp: trained in {train_s:.1f} s, validation error {va:.2e}")
-    outs = {k: DeviceArray((n, 6)) for k in ("mlp", "closed", "sum20", "sum64")}
+    outs = {k: DeviceArray((n, 6)) for k in ("mlp", "closed", "sum20", "sum64", "spectral")}
     nfall = net.vote(vol, outs["mlp"], shape, w, stick=True, tol=tol, in_layout=_lib.LAYOUT_PACKED6,
                      count_fallback=True)
     st = Stream().use()
@@ -216,6 +257,9 @@ def bench_mlp(steps, check=True, log=None, keep_net=False):
     say(f"mlp: {ms['mlp']:.0f} ms per vote")
     trad(_lib.PLATE_CLOSED, 20, outs["closed"])
     ms["closed"] = timed(lambda: trad(_lib.PLATE_CLOSED, 20, outs["closed"]), steps)
+    trad(_lib.VOTE_SPECTRAL, 20, outs["spectral"])  # builds the kernel spectra once
+    ms["spectral"] = timed(lambda: trad(_lib.VOTE_SPECTRAL, 20, outs["spectral"]), steps)
+    say(f"spectral (exact integral as an FFT convolution): {ms['spectral']:.1f} ms")
     ms["sum20"] = timed(lambda: trad(_lib.PLATE_SUM, 20, outs["sum20"]), 1)
     say(f"integrated plate N=20: {ms['sum20']:.0f} ms")
     ms["sum64"] = timed(lambda: trad(_lib.PLATE_SUM, 64, outs["sum64"]), 1)
@@ -235,6 +279,9 @@ def bench_mlp(steps, check=True, log=None, keep_net=False):
         "integrated_sum20": {"ms": round(ms["sum20"], 2), "mvoxel_s": round(n / (ms["sum20"] * 1e-3) / 1e6, 3)},
         "integrated_sum64": {"ms": round(ms["sum64"], 2), "mvoxel_s": round(n / (ms["sum64"] * 1e-3) / 1e6, 3)},
         "closed_form": {"ms": round(ms["closed"], 2), "mvoxel_s": round(n / (ms["closed"] * 1e-3) / 1e6, 3)},
+        "spectral": {"ms": round(ms["spectral"], 3), "mvoxel_s": round(n / (ms["spectral"] * 1e-3) / 1e6, 1)},
+        "speedup_spectral_vs_mlp": round(ms["mlp"] / ms["spectral"], 1),
+        "speedup_spectral_vs_sum20": round(ms["sum20"] / ms["spectral"], 1),
         "speedup_mlp_vs_sum20": round(ms["sum20"] / ms["mlp"], 3),
         "speedup_mlp_vs_sum64": round(ms["sum64"] / ms["mlp"], 3),
         "speedup_closed_vs_mlp": round(ms["mlp"] / ms["closed"], 3),
@@ -266,10 +313,10 @@ def bench_mlp(steps, check=True, log=None, keep_net=False):
                                  nthreads=min(16, os.cpu_count() or 1))
         ref = ref9[:, [0, 1, 2, 4, 5, 8]]
         gidx = ((pick[:, 0] + z - 24) * shape[1] + (pick[:, 1] + y - 24)) * shape[2] + (pick[:, 2] + x - 24)
-        for k in ("mlp", "closed", "sum20", "sum64"):
+        for k in ("mlp", "closed", "sum20", "sum64", "spectral"):
             got = outs[k].to_host().reshape(-1, 6)[gidx]
             key = {"mlp": "mlp", "closed": "closed_form", "sum20": "integrated_sum20",
-                   "sum64": "integrated_sum64"}[k]
+                   "sum64": "integrated_sum64", "spectral": "spectral"}[k]
             res[key]["rel_frob_err_vs_fp64_oracle"] = _frob6(got - ref) / _frob6(ref)
         res["error_check_sample"] = (f"{len(recv)} receivers (48 on sheets, 16 uniform) in the 24^3 core of a 48^3 "
                                      f"crop; FP64 64-sample oracle, {time.perf_counter() - t:.1f} s")
@@ -283,8 +330,9 @@ def bench_pipeline(net=None, log=None):
     """Full pipeline on the 512 x 512 x 384 vascular-tree volume (seed 1005,
     radii 2-8, intensity 1 on 0, N(0, 0.1) noise): structure tensor ->
     Gaussian sigma 2 (13 taps) -> stick + plate vote sigma 4 (w 25) -> eig3
-    saliency; the closed-form plate and, with `net`, the learned plate with
-    integrator fallback. Per-stage times and roofline fractions."""
+    saliency; the spectral vote (AUTO), the closed-form gather and, with
+    `net`, the learned plate with integrator fallback. Per-stage times and
+    roofline fractions; the skeleton stage after the spectral run.""" 
     from tiralib_amd import _lib, api, synth
     from tiralib_amd._lib import DeviceArray, Event, Stream, call
     say = log or (lambda *a: None)
@@ -298,12 +346,13 @@ def bench_pipeline(net=None, log=None):
     res = {"workload": "512x512x384 vascular tree (seed 1005, %d segments, radii 2-8, N(0,0.1) noise): structure "
                        "tensor -> Gaussian sigma=2 -> stick+plate vote sigma=4 w=25 -> eig3 saliency" % len(segs),
            "voxels": n, "pairs_per_vote": pairs}
-    modes = [("closed", _lib.PLATE_CLOSED, None)]
+    modes = [("spectral", _lib.VOTE_SPECTRAL, None), ("closed", _lib.PLATE_CLOSED, None)]
     if net is not None:
         modes.append(("mlp", _lib.PLATE_MLP, net))
     sal = DeviceArray(shape + (3,))
+    hbm = lambda b, t: round(b * n / (t * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)
     for name, mode, mlp in modes:
-        if mode == _lib.PLATE_CLOSED:  # warm-up: workspace and offset tables (the learned run reuses both)
+        if mode != _lib.PLATE_MLP:  # warm-up: workspace, offset tables / kernel spectra
             api.pipeline3(vol, smooth_sigma=2.0, vote_sigma=4.0, w=25, plate_mode=mode, saliency=sal)
         st.synchronize()
         call("tira_hip_profile_begin")
@@ -314,24 +363,36 @@ def bench_pipeline(net=None, log=None):
         st.synchronize()
         call("tira_hip_profile_end")
         ms = e0.elapsed_ms(e1)
-        vote_name = "vote3_mlp" if mode == _lib.PLATE_MLP else "vote3_closed"
+        if mode == _lib.VOTE_SPECTRAL:
+            vote_stages = list(SPECTRAL_KERNELS)
+        else:
+            vote_stages = ["vote3_mlp" if mode == _lib.PLATE_MLP else "vote3_closed"]
         prep_name = "vote3_prep_mlp" if mode == _lib.PLATE_MLP else "vote3_prep"
-        stages = {k: prof(k)[0] for k in ("structure_tensor", "gauss3_tensor", prep_name, vote_name, "eig3")}
-        hbm = lambda b, t: round(b * n / (t * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)
-        vote_roof = ({"unit": "TFLOP/s BF16 MFMA", "achieved": round(pairs * 17792 / (stages[vote_name] * 1e-3) / 1e12, 1),
-                      "frac": round(pairs * 17792 / (stages[vote_name] * 1e-3) / 2.5e15, 4)}
-                     if mode == _lib.PLATE_MLP else
-                     {"unit": "TFLOP/s FP32", "achieved": round(pairs * 96 / (stages[vote_name] * 1e-3) / 1e12, 1),
-                      "frac": round(pairs * 96 / (stages[vote_name] * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4)})
-        res[name] = {"ms": round(ms, 2), "mvoxel_s": round(n / (ms * 1e-3) / 1e6, 3),
+        stages = {k: prof(k)[0] for k in ["structure_tensor", "gauss3_tensor", prep_name, *vote_stages, "eig3"]}
+        vote_ms = sum(stages[k] for k in vote_stages)
+        if mode == _lib.VOTE_SPECTRAL:
+            per, dom, tot = spectral_roofline(shape, 25, stages, in_bytes=24)
+            vote_roof = {"unit": "GB/s HBM", "bound": "hbm", "dominant_kernel": dom, "achieved": per[dom]["gbs"],
+                         "frac": per[dom]["frac"], "all_kernels": tot, "per_kernel": per}
+        elif mode == _lib.PLATE_MLP:
+            vote_roof = {"unit": "TFLOP/s BF16 MFMA", "achieved": round(pairs * 17792 / (vote_ms * 1e-3) / 1e12, 1),
+                         "frac": round(pairs * 17792 / (vote_ms * 1e-3) / 2.5e15, 4)}
+        else:
+            vote_roof = {"unit": "TFLOP/s FP32", "achieved": round(pairs * 96 / (vote_ms * 1e-3) / 1e12, 1),
+                         "frac": round(pairs * 96 / (vote_ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4)}
+        res[name] = {"ms": round(ms, 2), "mvoxel_s": round(n / (ms * 1e-3) / 1e6, 3), "vote_ms": round(vote_ms, 3),
                      "stages_ms": {k: round(v, 3) for k, v in stages.items()},
                      "roofline": {"structure_tensor_hbm_frac": hbm(28, stages["structure_tensor"]),
                                   "gauss3_tensor_hbm_frac": hbm(48, stages["gauss3_tensor"]),
                                   "eig3_hbm_frac": hbm(36, stages["eig3"]), "vote": vote_roof}}
-        say(f"pipeline ({name} plate): {ms:.0f} ms")
-        if name == "closed":
+        say(f"pipeline ({name}): {ms:.0f} ms")
+        if mode == _lib.VOTE_SPECTRAL:
+            s_spec = sal.to_host()
             res["skeleton"] = bench_skeleton(sal, shape, st)
             say("skeleton (Otsu + Lee thinning) done")
+        elif mode == _lib.PLATE_CLOSED:
+            s_cl = sal.to_host()
+            res["spectral_vs_closed_saliency_rel_err"] = float(np.linalg.norm(s_spec - s_cl) / np.linalg.norm(s_cl))
     sal.free()
     vol.free()
     return res
@@ -375,6 +436,53 @@ def bench_skeleton(sal, shape, st):
             "ms": round(ms, 2), "otsu_threshold_mask_ms": round(t_mask, 3), "otsu_ms": round(otsu_ms, 3),
             "lee_thin_ms": round(t_thin, 2), "threshold": t, "mask_voxels": fg, "skeleton_voxels": sk,
             "outer_iterations": int(it.value), "mvoxel_s": round(n / (ms * 1e-3) / 1e6, 1)}
+
+
+def bench_headline_check(d_in, d_vote, field, shape, sigma, w, samples, st, nrecv=600, steps=5):
+    """The headline vote's error against the FP64 oracle on sampled receivers,
+    and the direct closed-form gather (the per-pair FP32 VALU kernel) timed
+    and checked on the same field."""
+    from oracle import oracle as O
+    from tiralib_amd import _lib
+    from tiralib_amd._lib import DeviceArray, Event, call
+    n = int(np.prod(shape))
+    head = d_vote.to_host()
+    d_g = DeviceArray((n, 6))
+    e0, e1 = Event(), Event()
+    try:
+        def gather():
+            call("tira_tensorvote3_async", d_in.ptr, _lib.LAYOUT_MAT9, d_g.ptr, _lib.LAYOUT_PACKED6, *shape, sigma,
+                 0.0, w, 1, 1, 1, samples, _lib.PLATE_CLOSED)
+        gather()
+        st.synchronize()
+        call("tira_hip_profile_begin")
+        e0.record(st)
+        for _ in range(steps):
+            gather()
+        e1.record(st)
+        st.synchronize()
+        call("tira_hip_profile_end")
+        g_ms = _lib.profile_get("vote3_closed")[0]
+        g = d_g.to_host()
+    finally:
+        d_g.free()
+    rng = np.random.default_rng(42)
+    recv = np.sort(rng.choice(n, nrecv, replace=False))
+    t = time.perf_counter()
+    ref = O.tensorvote3_f64(field, sigma, 0.0, 1, w, samples=samples, recv=recv,
+                            nthreads=min(16, os.cpu_count() or 1))[:, [0, 1, 2, 4, 5, 8]]
+    pairs = in_bounds_pairs(shape, w)
+    tflops = pairs * (STICK_PAIR_FLOP + PLATE_CLOSED_PAIR_FLOP) / (g_ms * 1e-3) / 1e12
+    return {
+        "headline_rel_frob_err_vs_fp64_oracle": _frob6(head[recv] - ref) / _frob6(ref),
+        "gather_rel_frob_err_vs_fp64_oracle": _frob6(g[recv] - ref) / _frob6(ref),
+        "headline_vs_gather_rel_frob_full_field": _frob6(head - g) / _frob6(g),
+        "error_sample": f"{nrecv} uniformly sampled receivers, FP64 oracle with {samples} plate samples "
+                        f"({time.perf_counter() - t:.1f} s); upper-triangle components",
+        "gather": {"kernel": "vote3_closed", "ms": round(g_ms, 4), "mvoxel_s_vote_only": round(n / g_ms / 1e3, 1),
+                   "tflops": round(tflops, 2), "frac_of_fp32_peak": round(tflops / PEAK_FP32_TFLOPS, 4),
+                   "frac_of_unpacked_peak": round(tflops / PEAK_FP32_UNPACKED_TFLOPS, 4)},
+    }
 
 
 def bench_atv(d_in, field, shape, sigma, w, steps, warmup, cpu_seconds=None):
@@ -498,7 +606,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--plate-mode", choices=["auto", "sum", "closed"], default="auto")
+    ap.add_argument("--plate-mode", choices=["auto", "sum", "closed", "spectral"], default="auto")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-mlp", action="store_true")
@@ -516,7 +624,8 @@ def main(argv=None):
     _lib.load()
     assert _lib.device_count() >= 1, "no GPU visible"
     shape, sigma, w, samples, power = (128, 128, 128), 2.0, 13, 20, 1
-    mode = {"auto": _lib.PLATE_AUTO, "sum": _lib.PLATE_SUM, "closed": _lib.PLATE_CLOSED}[a.plate_mode]
+    mode = {"auto": _lib.PLATE_AUTO, "sum": _lib.PLATE_SUM, "closed": _lib.PLATE_CLOSED,
+            "spectral": _lib.VOTE_SPECTRAL}[a.plate_mode]
     field = synth.tubes_sheets_volume(shape, seed=synth.SEED_CFG2)
     n = int(np.prod(shape))
     d_in = DeviceArray.from_host(field)
@@ -548,18 +657,42 @@ def main(argv=None):
     ms = e0.elapsed_ms(e1) / a.steps
 
     prof = _lib.profile_get
-
-    vote_name = "vote3_sum" if (mode == _lib.PLATE_SUM) else "vote3_closed"
-    vote_ms, vote_cnt = prof(vote_name)
     prep_ms, _ = prof("vote3_prep")
     eig_ms, _ = prof("eig3")
     pairs = in_bounds_pairs(shape, w)
-    if vote_name == "vote3_closed":
-        pair_flop = STICK_PAIR_FLOP + PLATE_CLOSED_PAIR_FLOP
+    spectral = prof("spec_mul")[1] > 0
+    if spectral:
+        stage_ms = {k: prof(k)[0] for k in SPECTRAL_KERNELS}
+        per, dom, tot = spectral_roofline(shape, w, stage_ms)
+        vote_ms = sum(stage_ms.values())
+        roof = {
+            "kernel": dom, "bound": "hbm", "achieved": per[dom]["gbs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": per[dom]["frac"], "traffic": profiled_traffic(dom),
+            "traffic_source": "profiles/traffic.json (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per launch)",
+            "algorithmic_bytes": per[dom]["bytes"], "avg_launch_ms": per[dom]["ms"],
+            "spectral_vote_all_kernels": tot, "per_kernel": per,
+            "pair_equivalent_tflops": round(pairs * (STICK_PAIR_FLOP + PLATE_CLOSED_PAIR_FLOP) / (vote_ms * 1e-3) / 1e12, 1),
+        }
+        stages = {"vote3_prep": round(prep_ms, 4), **{k: round(v, 4) for k, v in stage_ms.items()},
+                  "eig3": round(eig_ms, 4)}
+        vote_name = "spectral"
     else:
-        pair_flop = 33 + 41 * samples + 12
-    achieved = pairs * pair_flop / (vote_ms * 1e-3) / 1e12
-    ref_equiv = pairs * (33 + 41 * samples + 12) / (vote_ms * 1e-3) / 1e12
+        vote_name = "vote3_sum" if (mode == _lib.PLATE_SUM) else "vote3_closed"
+        vote_ms, vote_cnt = prof(vote_name)
+        if vote_name == "vote3_closed":
+            pair_flop = STICK_PAIR_FLOP + PLATE_CLOSED_PAIR_FLOP
+        else:
+            pair_flop = 33 + 41 * samples + 12
+        achieved = pairs * pair_flop / (vote_ms * 1e-3) / 1e12
+        roof = {
+            "kernel": vote_name, "bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+            "traffic": profiled_traffic("k_" + vote_name),
+            "traffic_source": "profiles/traffic.json (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per launch)",
+            "flop_per_pair": pair_flop, "avg_launch_ms": round(vote_ms, 4), "launches": vote_cnt,
+            "frac_of_unpacked_peak": round(achieved / PEAK_FP32_UNPACKED_TFLOPS, 4),
+        }
+        stages = {"vote3_prep": round(prep_ms, 4), vote_name: round(vote_ms, 4), "eig3": round(eig_ms, 4)}
 
     out = {
         "metric": METRIC,
@@ -578,27 +711,16 @@ def main(argv=None):
             "workload": "3D stick+plate tensor voting 128^3 (tubes+sheets, seed 1002), sigma=2, w=13, p=1, "
                         "N=20 plate samples, then eig3 of the voted field",
             "shape": list(shape), "sigma": sigma, "w": w, "power": power, "samples": samples,
-            "plate_mode": {0: "auto", 1: "sum", 2: "closed"}[mode],
+            "plate_mode": {0: "auto", 1: "sum", 2: "closed", 4: "spectral"}[mode],
+            "vote_method": "spectral (FFT convolution of quartic voter features, exact plate)" if spectral
+                           else vote_name,
             "pairs_per_step": pairs,
         },
-        "roofline": {
-            "kernel": vote_name,
-            "bound": "valu",
-            "achieved": round(achieved, 3),
-            "peak": PEAK_FP32_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-            "traffic": profiled_traffic("k_" + vote_name),
-            "traffic_source": "profiles/traffic.json (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 per launch)",
-            "flop_per_pair": pair_flop,
-            "avg_launch_ms": round(vote_ms, 4),
-            "launches": vote_cnt,
-            "frac_of_unpacked_peak": round(achieved / PEAK_FP32_UNPACKED_TFLOPS, 4),
-            "reference_formulation_tflops": round(ref_equiv, 3),
-        },
-        "stages_ms": {"vote3_prep": round(prep_ms, 4), vote_name: round(vote_ms, 4), "eig3": round(eig_ms, 4)},
+        "roofline": roof,
+        "stages_ms": stages,
         "wall_s_timed": round(wall, 3),
     }
+    out["accuracy_and_gather"] = bench_headline_check(d_in, d_vote, field, shape, sigma, w, samples, st)
     out["calibration"] = calibrate()
     if not a.no_batch:
         out["eig3_batch"] = bench_batch(max(3, a.steps // 2), max(1, a.warmup))

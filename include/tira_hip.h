@@ -48,10 +48,15 @@ typedef enum tira_status {
 
 /* Plate-vote evaluation for the 3D vote (samples > 0 is the reference's
  * numerically integrated plate, tensorvote.h:1651-1695 / 2457-2485). */
-#define TIRA_PLATE_AUTO    0   /* closed form when samples >= power+2 (the Riemann sum is then exact), else literal sum */
+#define TIRA_PLATE_AUTO    0   /* the spectral vote when it applies (power 1, samples 0 or >= 3, w >= 5, at
+                                  least 32^3 voxels, memory within 80% of the device); else the closed
+                                  form when samples >= power+2 (the Riemann sum is then exact), else the
+                                  literal sum */
 #define TIRA_PLATE_SUM     1   /* literal N-sample Riemann sum, the reference's algorithm, any N */
 #define TIRA_PLATE_CLOSED  2   /* closed form of the same integral (exact for N >= p+2 and for samples == 0) */
 #define TIRA_PLATE_MLP     3   /* learned surrogate with per-voxel integrator fallback: served by tira_tensorvote3_mlp */
+#define TIRA_VOTE_SPECTRAL 4   /* stick + exact (closed-form) plate vote as an FFT convolution: power 1 and
+                                  samples == 0 or >= 3; other requests take the direct gather (AUTO) */
 
 /* eig3 flags */
 #define TIRA_EIG_REFINE_FP64   1u  /* FP64 re-solve of voxels with relative eigengap < 1e-4 */
@@ -191,6 +196,25 @@ int tira_tensorvote3_atv(float* output_field, const float* lambdas, const float*
  * library stream (layouts TIRA_LAYOUT_*). */
 int tira_tensorvote3_atv_async(const float* input_field, int in_layout, float* output_field, int out_layout,
                                unsigned s0, unsigned s1, unsigned s2, float sigma, unsigned w);
+
+/* Spectral vote internals, exported for tests (no reference counterpart).
+ * Offset kernels K[ch][dw][dv][du] (ch = c * nf + f; c = xx, xy, xz, yy, yz,
+ * zz; f < nf the voter features: the 15 quartic monomials q_x^a q_y^b q_z^c,
+ * a = 4..0, b = 4-a..0, of the stick vector e2 (when stick) then of the plate
+ * normal e0 (when plate)) such that the pair vote of tensorvote.h:1183-1213 /
+ * 1651-1695 (normalised by sticknorm3 and pi, plate exact) at offset d equals
+ * sum_f K[c,f](d) * scale * monomial_f. Host computation, FP64, no GPU; out
+ * holds 6 * nf * (2(w/2)+1)^3 doubles; *nf_out receives nf. power must be 1. */
+int tira_spectral_offset_kernels(unsigned w, float sigma1, float sigma2, unsigned power, int stick, int plate,
+                                 double* out, unsigned* nf_out);
+/* In-place 3D FFT of nchan complex volumes (interleaved float pairs, each
+ * L0 x L1 x L2 with L2 fastest, volumes back to back); forward e^{-2 pi i k x / L},
+ * inverse unnormalised. Every L must be <= 1024 with prime factors 2, 3, 5, 7.
+ * Device pointer, non-blocking. */
+int tira_fft3_async(float* data, unsigned nchan, unsigned L0, unsigned L1, unsigned L2, int inverse);
+/* FFT length the spectral vote uses for an axis of n voxels and window w
+ * (smallest 7-smooth L >= n + w/2), 0 when above 1024. */
+unsigned tira_spectral_length(unsigned n, unsigned w);
 
 /* ------------------------------------------------------------------ Gaussian filter */
 /* tira::cuda::gaussian_filter3d<float>(src, width, height, depth, sigma_w, sigma_h,

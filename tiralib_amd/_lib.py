@@ -28,6 +28,7 @@ PLATE_AUTO = 0
 PLATE_SUM = 1
 PLATE_CLOSED = 2
 PLATE_MLP = 3
+VOTE_SPECTRAL = 4
 
 EIG_REFINE_FP64 = 1
 EIG_SCALE_INVARIANT = 2
@@ -97,6 +98,9 @@ SIGNATURES = {
     "tira_tensorvote3_async": [_vp, _i, _vp, _i, _u, _u, _u, _f, _f, _u, _u, _i, _i, _u, _i],
     "tira_tensorvote2_atv": [_vp, _vp, _vp, _f, _u, _u, _u],
     "tira_stick_saliency2_async": [_vp, _sz, _vp],
+    "tira_spectral_offset_kernels": [_u, _f, _f, _u, _i, _i, _vp, _vp],
+    "tira_fft3_async": [_vp, _u, _u, _u, _u, _i],
+    "tira_spectral_length": [_u, _u],
     "tira_otsu_threshold_async": [_vp, _sz, _i, _vp],
     "tira_threshold_async": [_vp, _sz, _i, _vp, _vp],
     "tira_lee_thin3": [_vp, _vp, _u, _u, _u, _vp],
@@ -134,6 +138,9 @@ SIGNATURES = {
 }
 
 _lib = None
+# Entry points that return a value rather than a status.
+RESTYPES = {"tira_spectral_length": _u}
+
 _lock = threading.Lock()
 
 
@@ -151,7 +158,7 @@ def load():
             for name, args in SIGNATURES.items():
                 fn = getattr(L, name)
                 fn.argtypes = args
-                fn.restype = _i
+                fn.restype = RESTYPES.get(name, _i)
             _lib = L
     return _lib
 

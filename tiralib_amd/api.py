@@ -140,6 +140,32 @@ def tensorvote3_async(field, out, shape, sigma1, sigma2, w, power, stick=True, p
     return out
 
 
+def spectral_offset_kernels(w, sigma1, sigma2=0.0, stick=True, plate=True):
+    """Offset kernels of the spectral vote, (6, nf, W, W, W) float64 indexed
+    [c, f, dw, dv, du] with W = 2 (w // 2) + 1 (host computation, no GPU)."""
+    W = 2 * (int(w) // 2) + 1
+    nf = 15 * (int(bool(stick)) + int(bool(plate)))
+    out = np.empty((6, nf, W, W, W), np.float64)
+    got = C.c_uint()
+    call("tira_spectral_offset_kernels", int(w), float(sigma1), float(sigma2), 1, int(stick), int(plate),
+         out.ctypes.data, C.byref(got))
+    assert got.value == nf
+    return out
+
+
+def spectral_length(n, w):
+    """FFT length of an axis of n voxels with window w (0 when above 1024)."""
+    return int(_lib.load().tira_spectral_length(int(n), int(w)))
+
+
+def fft3(data, inverse=False):
+    """In-place 3D FFT of a DeviceArray of complex volumes (nchan, L0, L1, L2, 2) float32 (blocking)."""
+    nchan, L0, L1, L2 = data.shape[:4]
+    call("tira_fft3_async", data.ptr, nchan, L0, L1, L2, int(bool(inverse)))
+    call("tira_hip_synchronize")
+    return data
+
+
 def tensorvote3_atv(lambdas, evec1, evec2, shape, sigma, w):
     """tira::cuda::_tensorvote3_atv (tensorvote.h:2695-2739): analytical tensor voting
     from eigenpairs (ascending eigenvalues, middle and largest eigenvectors)."""

@@ -74,7 +74,7 @@ int tira_tensorvote3_pipeline_async(const float* intensity, unsigned s0, unsigne
     TH_ARG(s0 >= 2 && s1 >= 2 && s2 >= 2, "every axis needs at least 2 voxels (got %u x %u x %u)", s0, s1, s2);
     TH_ARG(is_device(intensity) && is_device(saliency), "the pipeline takes device pointers");
     TH_ARG(smooth_sigma >= 0.0f, "smoothing sigma must be >= 0");
-    TH_ARG(plate_mode >= TIRA_PLATE_AUTO && plate_mode <= TIRA_PLATE_MLP, "bad plate mode %d", plate_mode);
+    TH_ARG(plate_mode >= TIRA_PLATE_AUTO && plate_mode <= TIRA_VOTE_SPECTRAL, "bad plate mode %d", plate_mode);
     TH_ARG(plate_mode != TIRA_PLATE_MLP || mlp, "plate mode MLP needs a network handle (its sigmas and power apply)");
     const size_t n = (size_t)s0 * s1 * s2;
     float *a = nullptr, *b = nullptr;

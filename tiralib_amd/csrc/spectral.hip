@@ -1,0 +1,1048 @@
+// spectral.hip -- the 3D stick + plate vote as an FFT convolution on gfx950.
+//
+// For power p = 1 the vote a voter casts at offset d is a homogeneous quartic
+// in its eigenvector:
+//   * stick (stickvote3_accumulate_kernel, tensorvote.h:1183-1213):
+//     s * eta(q.d) r r^T with r = q - 2 (q.d) d and eta = c1 (1 - t^2) + c2 t^2,
+//     where 1 = |q|^2 makes every term degree 4 in q;
+//   * plate (tensorvote.h:1651-1695): the exact circle integral (the
+//     reference's N-sample sum equals it for N >= p + 2 = 3), a quartic in the
+//     normal n by the same argument.
+// On the unit sphere the even polynomials of degree <= 4 span exactly the 15
+// quartic monomials q_x^a q_y^b q_z^c (a + b + c = 4), so
+//     V_c(x) = sum_f sum_d K_{c,f}(d) F_f(x + d),      c = xx, xy, xz, yy, yz, zz,
+// with the voter features F_f = scale * monomial_f (15 of e2 for the stick,
+// 15 of e0 for the plate) and offset kernels K_{c,f}(d) fitted once on the
+// host in FP64. The whole field's vote is a 30-in / 6-out correlation with a
+// w^3 window: O(N log N) through FFTs instead of O(N w^3) pair evaluations.
+//
+// Pipeline of one call (device buffers, zero padding to L >= n + w/2 per axis
+// so the circular convolution equals the reference's bounded window):
+//  1. payload: eigen-decomposition of every voxel (shared with the gather path);
+//  2. forward x pass fused with the features: rows of the field only, two
+//     real features packed as one complex line;
+//  3. forward y pass (planes z < n0), forward z pass (all columns);
+//  4. pointwise product: one thread per octant group {(+-kx, +-ky, +-kz)}.
+//     The kernels are real, even and of definite parity along each axis, so
+//     their spectra are real and one octant (FP32, FP64-built) serves all
+//     eight frequencies; the packed complex features are untangled with
+//     Z(-k), and the six real outputs are packed back into three complex
+//     spectra, computed at four of the eight frequencies and conjugated for
+//     the other four;
+//  5. inverse z pass (outputs z < n0), inverse y pass (outputs y < n1), and
+//     the inverse x pass fused with the output write (either layout).
+// FFTs are Stockham autosort, mixed radix 2/3/4/5/7/8, one block per batch of
+// lines staged in LDS; the twiddle table is FP64-built and rounded once.
+// The kernel spectra depend on (L0, L1, L2, w, sigma1, sigma2, stick, plate)
+// only: built on the device (separable FP64 cosine/sine transforms of the
+// offset kernels) and cached per host thread and device.
+#include "common.hpp"
+#include "vote3.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace th {
+
+constexpr int kFT = 256;       // threads per FFT block
+constexpr int kFftBL = 4096;   // complex values staged per block (32 KB of LDS)
+constexpr int kMaxL = 1024;    // longest transform
+constexpr int kMaxStages = 16;
+constexpr int kMono = 15;      // quartic monomials
+
+// Exponents (a, b, c) of monomial m: a = 4..0, b = 4-a..0, c = 4-a-b.
+__host__ __device__ constexpr int mono_a(int m) {
+    return m < 1 ? 4 : m < 3 ? 3 : m < 6 ? 2 : m < 10 ? 1 : 0;
+}
+__host__ __device__ constexpr int mono_b(int m) {
+    const int a = mono_a(m);
+    const int first = (4 - a) * (5 - a) / 2;  // index of (a, 4-a, 0)
+    return (4 - a) - (m - first);
+}
+__host__ __device__ constexpr int mono_c(int m) { return 4 - mono_a(m) - mono_b(m); }
+// Number of x / y / z indices in output component c (xx, xy, xz, yy, yz, zz).
+__host__ __device__ constexpr int comp_nx(int c) { return c == 0 ? 2 : (c == 1 || c == 2) ? 1 : 0; }
+__host__ __device__ constexpr int comp_ny(int c) { return c == 3 ? 2 : (c == 1 || c == 4) ? 1 : 0; }
+__host__ __device__ constexpr int comp_nz(int c) { return c == 5 ? 2 : (c == 2 || c == 4) ? 1 : 0; }
+
+struct FftPlan {
+    int L;
+    int nst;
+    int rad[kMaxStages];
+};
+
+// Radices of L (7s, 5s, 3s, then 8s and 4s, a final 2); false when L has a
+// prime factor above 7.
+static bool make_plan(int L, FftPlan& p) {
+    p.L = L;
+    p.nst = 0;
+    int n = L;
+    for (int f : {7, 5, 3})
+        while (n % f == 0) { p.rad[p.nst++] = f; n /= f; }
+    int t = 0;
+    while (n % 2 == 0) { t++; n /= 2; }
+    if (n != 1) return false;
+    while (t >= 3 && t != 4) { p.rad[p.nst++] = 8; t -= 3; }
+    while (t >= 2) { p.rad[p.nst++] = 4; t -= 2; }
+    if (t) p.rad[p.nst++] = 2;
+    return p.nst <= kMaxStages;
+}
+
+static unsigned spectral_length(unsigned n, unsigned w) {
+    const unsigned need = n + w / 2;
+    for (unsigned L = std::max(need, 1u); L <= (unsigned)kMaxL; L++) {
+        FftPlan p;
+        if (make_plan((int)L, p)) return L;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------- device FFT
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+// cos / sin of 2 pi n / R for the generic small DFTs.
+template <int R> __device__ __forceinline__ float dcos(int n);
+template <int R> __device__ __forceinline__ float dsin(int n);
+template <> __device__ __forceinline__ float dcos<5>(int n) {
+    constexpr float t[5] = {1.0f, 0.309016994374947424f, -0.809016994374947424f, -0.809016994374947424f,
+                            0.309016994374947424f};
+    return t[n];
+}
+template <> __device__ __forceinline__ float dsin<5>(int n) {
+    constexpr float t[5] = {0.0f, 0.951056516295153572f, 0.587785252292473129f, -0.587785252292473129f,
+                            -0.951056516295153572f};
+    return t[n];
+}
+template <> __device__ __forceinline__ float dcos<7>(int n) {
+    constexpr float t[7] = {1.0f, 0.623489801858733531f, -0.222520933956314404f, -0.900968867902419126f,
+                            -0.900968867902419126f, -0.222520933956314404f, 0.623489801858733531f};
+    return t[n];
+}
+template <> __device__ __forceinline__ float dsin<7>(int n) {
+    constexpr float t[7] = {0.0f, 0.781831482468029809f, 0.974927912181823607f, 0.433883739117558120f,
+                            -0.433883739117558120f, -0.974927912181823607f, -0.781831482468029809f};
+    return t[n];
+}
+template <> __device__ __forceinline__ float dcos<8>(int n) {
+    constexpr float t[8] = {1.0f, 0.707106781186547524f, 0.0f, -0.707106781186547524f,
+                            -1.0f, -0.707106781186547524f, 0.0f, 0.707106781186547524f};
+    return t[n];
+}
+template <> __device__ __forceinline__ float dsin<8>(int n) {
+    constexpr float t[8] = {0.0f, 0.707106781186547524f, 1.0f, 0.707106781186547524f,
+                            0.0f, -0.707106781186547524f, -1.0f, -0.707106781186547524f};
+    return t[n];
+}
+
+// In-register DFT of length R: y_k = sum_j u_j W^(jk), W = e^(-+2 pi i / R).
+template <int R, bool INV>
+__device__ __forceinline__ void dft_small(float2 (&u)[R]) {
+    if constexpr (R == 2) {
+        const float2 a = u[0], b = u[1];
+        u[0] = cadd(a, b);
+        u[1] = csub(a, b);
+    } else if constexpr (R == 3) {
+        constexpr float kS = 0.866025403784438647f;  // sin(2 pi / 3)
+        const float2 s = cadd(u[1], u[2]), d = csub(u[1], u[2]);
+        const float2 t = make_float2(fmaf(-0.5f, s.x, u[0].x), fmaf(-0.5f, s.y, u[0].y));
+        const float sg = INV ? kS : -kS;  // i * sg * d
+        u[0] = cadd(u[0], s);
+        u[1] = make_float2(t.x - sg * d.y, t.y + sg * d.x);
+        u[2] = make_float2(t.x + sg * d.y, t.y - sg * d.x);
+    } else if constexpr (R == 4) {
+        const float2 a = cadd(u[0], u[2]), b = csub(u[0], u[2]);
+        const float2 c = cadd(u[1], u[3]), d = csub(u[1], u[3]);
+        // forward: y1 = b - i d, y3 = b + i d
+        const float2 mid = INV ? make_float2(b.x - d.y, b.y + d.x) : make_float2(b.x + d.y, b.y - d.x);
+        const float2 oth = INV ? make_float2(b.x + d.y, b.y - d.x) : make_float2(b.x - d.y, b.y + d.x);
+        u[0] = cadd(a, c);
+        u[2] = csub(a, c);
+        u[1] = mid;
+        u[3] = oth;
+    } else {
+        float2 y[R];
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            float2 acc = u[0];
+#pragma unroll
+            for (int j = 1; j < R; j++) {
+                const int n = (j * k) % R;
+                const float c = dcos<R>(n), s = INV ? dsin<R>(n) : -dsin<R>(n);
+                acc.x = fmaf(u[j].x, c, fmaf(-u[j].y, s, acc.x));
+                acc.y = fmaf(u[j].x, s, fmaf(u[j].y, c, acc.y));
+            }
+            y[k] = acc;
+        }
+#pragma unroll
+        for (int k = 0; k < R; k++) u[k] = y[k];
+    }
+}
+
+// LDS pointers keep their address space across the (non-inlined) stage calls,
+// so the stages use ds_read / ds_write.
+typedef float __attribute__((ext_vector_type(2))) f2v;
+typedef __attribute__((address_space(3))) f2v lds_f2;
+__device__ __forceinline__ lds_f2* to_lds(float2* p) { return (lds_f2*)p; }
+__device__ __forceinline__ float2 ld2(const lds_f2* p) {
+    const f2v v = *p;
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void st2(lds_f2* p, float2 a) { *p = f2v{a.x, a.y}; }
+
+// One Stockham stage of radix R over `nlines` lines of length L in LDS
+// (line stride L), p = product of the radices already applied: butterfly i
+// of a line reads u_r = x[i + r L/R], twiddles u_r by W_L^(r k L/(pR)),
+// k = i mod p, and writes y[(i - k) R + k + r p]. In place: every thread
+// reads its butterflies into registers before any write.
+// Each radix is its own function: inlined together, the six stage bodies
+// shared one register allocation of more than twice any single one.
+template <int R, bool INV>
+__device__ __noinline__ void fft_stage(lds_f2* __restrict__ s, int L, int nlines, int p,
+                                       const lds_f2* __restrict__ tw) {
+    constexpr int MAXQ = (kFftBL / R + kFT - 1) / kFT;
+    const int m = L / R;
+    const int nb = nlines * m;
+    const int step = L / (p * R);
+    float2 u[MAXQ][R];
+#pragma unroll
+    for (int q = 0; q < MAXQ; q++) {
+        const int t = (int)threadIdx.x + q * kFT;
+        if (t < nb) {
+            const int line = t / m, i = t - line * m;
+            const lds_f2* src = s + line * L + i;
+#pragma unroll
+            for (int r = 0; r < R; r++) u[q][r] = ld2(src + r * m);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < MAXQ; q++) {
+        const int t = (int)threadIdx.x + q * kFT;
+        if (t < nb) {
+            const int line = t / m, i = t - line * m;
+            const int k = i % p;
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                float2 w = ld2(tw + r * k * step);  // r k step < L
+                if (INV) w.y = -w.y;
+                u[q][r] = cmul(u[q][r], w);
+            }
+            dft_small<R, INV>(u[q]);
+            lds_f2* dst = s + line * L + (i - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) st2(dst + r * p, u[q][r]);
+        }
+    }
+    __syncthreads();
+}
+
+// Transform `nlines` lines staged in LDS (nlines * L <= kFftBL).
+template <bool INV>
+__device__ __forceinline__ void fft_lines(float2* __restrict__ s_, const FftPlan& pl, int nlines,
+                                          const float2* __restrict__ tw_) {
+    lds_f2* s = to_lds(s_);
+    const lds_f2* tw = to_lds(const_cast<float2*>(tw_));
+    int p = 1;
+    for (int st = 0; st < pl.nst; st++) {
+        const int R = pl.rad[st];
+        switch (R) {
+            case 2: fft_stage<2, INV>(s, pl.L, nlines, p, tw); break;
+            case 3: fft_stage<3, INV>(s, pl.L, nlines, p, tw); break;
+            case 4: fft_stage<4, INV>(s, pl.L, nlines, p, tw); break;
+            case 5: fft_stage<5, INV>(s, pl.L, nlines, p, tw); break;
+            case 7: fft_stage<7, INV>(s, pl.L, nlines, p, tw); break;
+            default: fft_stage<8, INV>(s, pl.L, nlines, p, tw); break;
+        }
+        p *= R;
+    }
+}
+
+__device__ __forceinline__ void load_twiddles(float2* __restrict__ tw, const float2* __restrict__ g, int L) {
+    for (int i = (int)threadIdx.x; i < L; i += kFT) tw[i] = g[i];
+}
+
+// Lines of one axis of a stack of complex volumes. Line (a, b) of channel c
+// starts at c * cstride + b * bstride + a * astride; its elements are
+// estride apart. Elements at or beyond n_in read as zero; only the first
+// n_out are written back. Blocks take `B` consecutive a (a batch), then b,
+// then the channel.
+struct LineGeom {
+    size_t cstride, bstride, astride, estride;
+    int na, nb, nchan, B, nab;
+    int n_in, n_out;
+};
+
+// ROWS: elements contiguous (estride 1), loads walk a line first; otherwise
+// the B lines of a batch are adjacent in memory (astride 1) and loads walk
+// across them first, so both cases read contiguous runs.
+template <bool INV, bool ROWS>
+__global__ __launch_bounds__(kFT) void k_fft_lines(float2* __restrict__ data, LineGeom g, FftPlan pl,
+                                                   const float2* __restrict__ twg) {
+    __shared__ float2 s[kFftBL];
+    __shared__ float2 tw[kMaxL];
+    const int L = pl.L;
+    unsigned bid = blockIdx.x;
+    const int ab = (int)(bid % (unsigned)g.nab);
+    bid /= (unsigned)g.nab;
+    const int b = (int)(bid % (unsigned)g.nb);
+    const int c = (int)(bid / (unsigned)g.nb);
+    if (c >= g.nchan) return;
+    const int a0 = ab * g.B;
+    const int Bv = min(g.B, g.na - a0);
+    load_twiddles(tw, twg, L);
+    float2* base = data + (size_t)c * g.cstride + (size_t)b * g.bstride + (size_t)a0 * g.astride;
+    const int nin = Bv * g.n_in;
+    for (int idx = (int)threadIdx.x; idx < Bv * L; idx += kFT) {
+        int e, l;
+        if (ROWS) { l = idx / L; e = idx - l * L; }
+        else { e = idx / Bv; l = idx - e * Bv; }
+        float2 v = make_float2(0.0f, 0.0f);
+        if (e < g.n_in) v = base[(size_t)l * g.astride + (size_t)e * g.estride];
+        s[l * L + e] = v;
+    }
+    (void)nin;
+    __syncthreads();
+    fft_lines<INV>(s, pl, Bv, tw);
+    for (int idx = (int)threadIdx.x; idx < Bv * g.n_out; idx += kFT) {
+        int e, l;
+        if (ROWS) { l = idx / g.n_out; e = idx - l * g.n_out; }
+        else { e = idx / Bv; l = idx - e * Bv; }
+        base[(size_t)l * g.astride + (size_t)e * g.estride] = s[l * L + e];
+    }
+}
+
+// ---------------------------------------------------------------- spectral vote kernels
+struct SpecGeom {
+    int n0, n1, n2;       // field (n2 fastest)
+    int L0, L1, L2;       // FFT lengths
+    int h0, h1, h2;       // octant extents L/2 + 1
+    int nf, nz, nst;      // real features, complex channels, stick features (15 or 0)
+    size_t cs;            // complex values per channel volume
+    size_t ocs;           // kernel-spectrum values per channel (octant)
+};
+
+__device__ __forceinline__ float ipow4(float x, int a) {
+    const float x2 = x * x;
+    return a == 0 ? 1.0f : a == 1 ? x : a == 2 ? x2 : a == 3 ? x2 * x : x2 * x2;
+}
+
+// Feature f of a voxel: scale * monomial of e2 (stick, f < nst) or of e0.
+__device__ __forceinline__ float feature(int f, int nst, const float4* __restrict__ ps,
+                                         const float4* __restrict__ pp, size_t v) {
+    const float4 P = f < nst ? ps[v] : pp[v];
+    const int m = f < nst ? f : f - nst;
+    return P.w * ipow4(P.x, mono_a(m)) * ipow4(P.y, mono_b(m)) * ipow4(P.z, mono_c(m));
+}
+
+// Forward x pass fused with the features: rows (z, y) of the field, complex
+// channel j = features (2j, 2j+1); block = (row batch, channel), channel fastest
+// so the blocks reading the same payload rows run together.
+__global__ __launch_bounds__(kFT) void k_spec_xfwd(const float4* __restrict__ ps, const float4* __restrict__ pp,
+                                                   SpecGeom g, int B, FftPlan pl, const float2* __restrict__ twg,
+                                                   float2* __restrict__ Z) {
+    __shared__ float2 s[kFftBL];
+    __shared__ float2 tw[kMaxL];
+    const int L = pl.L;
+    const int j = (int)(blockIdx.x % (unsigned)g.nz);
+    const int row0 = (int)(blockIdx.x / (unsigned)g.nz) * B;
+    const int nrows = g.n0 * g.n1;
+    if (row0 >= nrows) return;
+    const int Bv = min(B, nrows - row0);
+    load_twiddles(tw, twg, L);
+    const int f0 = 2 * j, f1 = 2 * j + 1;
+    for (int idx = (int)threadIdx.x; idx < Bv * L; idx += kFT) {
+        const int l = idx / L, e = idx - l * L;
+        float2 v = make_float2(0.0f, 0.0f);
+        if (e < g.n2) {
+            const size_t vox = (size_t)(row0 + l) * g.n2 + e;
+            v.x = feature(f0, g.nst, ps, pp, vox);
+            if (f1 < g.nf) v.y = feature(f1, g.nst, ps, pp, vox);
+        }
+        s[l * L + e] = v;
+    }
+    __syncthreads();
+    fft_lines<false>(s, pl, Bv, tw);
+    float2* zc = Z + (size_t)j * g.cs;
+    for (int idx = (int)threadIdx.x; idx < Bv * L; idx += kFT) {
+        const int l = idx / L, e = idx - l * L;
+        const int row = row0 + l;
+        const int z = row / g.n1, y = row - z * g.n1;
+        zc[((size_t)z * g.L1 + y) * g.L2 + e] = s[l * L + e];
+    }
+}
+
+// Pointwise product, one thread per octant group (kx, ky, kz) in
+// [0, h2) x [0, h1) x [0, h0). Slot bits (x, y, z flip) pick the frequency
+// (+-kx, +-ky, +-kz). For slots 0..3 (kz unflipped):
+//   F_f(k) = (Z_j(k) + conj Z_j(-k)) / 2        (f = 2j, real part's spectrum)
+//          = -i (Z_j(k) - conj Z_j(-k)) / 2     (f = 2j + 1)
+//   Y_c(k) = sum_f Khat_{c,f}(k) F_f(k),  Khat(flipped axes) = parity sign x octant value
+// (the 1/2 and the inverse-FFT 1/(L0 L1 L2) are folded into the octant values);
+// then W_i(k) = Y_{2i}(k) + i Y_{2i+1}(k) and W_i(-k) = conj Y_{2i}(k) + i conj Y_{2i+1}(k)
+// overwrite Z_0..Z_2 (each group's frequencies belong to this thread only).
+__global__ __launch_bounds__(256) void k_spec_mul(float2* __restrict__ Z, const float* __restrict__ Kh,
+                                                  SpecGeom g) {
+    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= g.ocs) return;
+    const int kx = (int)(gid % (size_t)g.h2);
+    const int ky = (int)((gid / (size_t)g.h2) % (size_t)g.h1);
+    const int kz = (int)(gid / ((size_t)g.h2 * g.h1));
+    const int fx[2] = {kx, kx ? g.L2 - kx : 0};
+    const int fy[2] = {ky, ky ? g.L1 - ky : 0};
+    const int fz[2] = {kz, kz ? g.L0 - kz : 0};
+    size_t off[8];
+#pragma unroll
+    for (int sl = 0; sl < 8; sl++)
+        off[sl] = ((size_t)fz[(sl >> 2) & 1] * g.L1 + fy[(sl >> 1) & 1]) * g.L2 + fx[sl & 1];
+    float2 Y[4][6];
+#pragma unroll
+    for (int sl = 0; sl < 4; sl++)
+#pragma unroll
+        for (int c = 0; c < 6; c++) Y[sl][c] = make_float2(0.0f, 0.0f);
+
+    for (int j = 0; j < g.nz; j++) {
+        const float2* zc = Z + (size_t)j * g.cs;
+        float2 z8[8];
+#pragma unroll
+        for (int sl = 0; sl < 8; sl++) z8[sl] = zc[off[sl]];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int f = 2 * j + h;
+            if (f >= g.nf) break;
+            const int m = f < g.nst ? f : f - g.nst;
+            const int am = mono_a(m), bm = mono_b(m);
+            float2 F[4];
+#pragma unroll
+            for (int sl = 0; sl < 4; sl++) {
+                const float2 a = z8[sl], bc = make_float2(z8[sl ^ 7].x, -z8[sl ^ 7].y);
+                F[sl] = h == 0 ? cadd(a, bc) : make_float2(a.y - bc.y, bc.x - a.x);
+            }
+#pragma unroll
+            for (int c = 0; c < 6; c++) {
+                const float k = Kh[(size_t)(c * g.nf + f) * g.ocs + gid];
+                const bool ox = ((comp_nx(c) + am) & 1) != 0, oy = ((comp_ny(c) + bm) & 1) != 0;
+                const float k1 = ox ? -k : k, k2 = oy ? -k : k, k3 = (ox != oy) ? -k : k;
+                Y[0][c].x = fmaf(k, F[0].x, Y[0][c].x);  Y[0][c].y = fmaf(k, F[0].y, Y[0][c].y);
+                Y[1][c].x = fmaf(k1, F[1].x, Y[1][c].x); Y[1][c].y = fmaf(k1, F[1].y, Y[1][c].y);
+                Y[2][c].x = fmaf(k2, F[2].x, Y[2][c].x); Y[2][c].y = fmaf(k2, F[2].y, Y[2][c].y);
+                Y[3][c].x = fmaf(k3, F[3].x, Y[3][c].x); Y[3][c].y = fmaf(k3, F[3].y, Y[3][c].y);
+            }
+        }
+    }
+#pragma unroll
+    for (int sl = 0; sl < 4; sl++) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const float2 ya = Y[sl][2 * i], yb = Y[sl][2 * i + 1];
+            float2* zc = Z + (size_t)i * g.cs;
+            zc[off[sl]] = make_float2(ya.x - yb.y, ya.y + yb.x);
+            zc[off[sl ^ 7]] = make_float2(ya.x + yb.y, yb.x - ya.y);
+        }
+    }
+}
+
+// Inverse x pass fused with the output: rows (z, y) of the field, the three
+// output channels of B rows together; W_0 = (xx, xy), W_1 = (xz, yy),
+// W_2 = (yz, zz) as (real, imaginary) parts.
+__global__ __launch_bounds__(kFT) void k_spec_xinv(const float2* __restrict__ Z, SpecGeom g, int B, FftPlan pl,
+                                                   const float2* __restrict__ twg, float* __restrict__ out,
+                                                   int layout) {
+    __shared__ float2 s[kFftBL];
+    __shared__ float2 tw[kMaxL];
+    const int L = pl.L;
+    const int row0 = (int)blockIdx.x * B;
+    const int nrows = g.n0 * g.n1;
+    if (row0 >= nrows) return;
+    const int Bv = min(B, nrows - row0);
+    load_twiddles(tw, twg, L);
+    for (int idx = (int)threadIdx.x; idx < 3 * Bv * L; idx += kFT) {
+        const int line = idx / L, e = idx - line * L;  // line = 3 l + i
+        const int l = line / 3, i = line - 3 * l;
+        const int row = row0 + l;
+        const int z = row / g.n1, y = row - z * g.n1;
+        s[line * L + e] = Z[(size_t)i * g.cs + ((size_t)z * g.L1 + y) * g.L2 + e];
+    }
+    __syncthreads();
+    fft_lines<true>(s, pl, 3 * Bv, tw);
+    for (int idx = (int)threadIdx.x; idx < Bv * g.n2; idx += kFT) {
+        const int l = idx / g.n2, x = idx - l * g.n2;
+        const float2 w0 = s[(3 * l + 0) * L + x], w1 = s[(3 * l + 1) * L + x], w2 = s[(3 * l + 2) * L + x];
+        const size_t v = (size_t)(row0 + l) * g.n2 + x;
+        if (layout == TIRA_LAYOUT_MAT9) {
+            float* o = out + v * 9;
+            o[0] = w0.x; o[1] = w0.y; o[2] = w1.x;
+            o[3] = w0.y; o[4] = w1.y; o[5] = w2.x;
+            o[6] = w1.x; o[7] = w2.x; o[8] = w2.y;
+        } else {
+            float* o = out + v * 6;
+            o[0] = w0.x; o[1] = w0.y; o[2] = w1.x; o[3] = w1.y; o[4] = w2.x; o[5] = w2.y;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- kernel spectra
+// Separable FP64 transforms of the offset kernels K[ch][dw][dv][du] (parity
+// of channel ch along x / y / z in bits 0 / 1 / 2 of par[ch]): cosine along
+// even axes, sine along odd ones, trig tables trig[(odd * h + k) * w + d + r]
+// = cos / sin(2 pi k d / L).
+__global__ __launch_bounds__(256) void k_khat_x(const double* __restrict__ K, const double* __restrict__ trig,
+                                                const unsigned char* __restrict__ par, double* __restrict__ T1,
+                                                int w, int h2, int nch) {
+    const size_t n = (size_t)nch * w * w * h2;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int kx = (int)(i % (size_t)h2);
+        const size_t line = i / (size_t)h2;  // (ch, dw, dv)
+        const int ch = (int)(line / ((size_t)w * w));
+        const double* kr = K + line * w;
+        const double* tr = trig + ((size_t)(par[ch] & 1) * h2 + kx) * w;
+        double acc = 0.0;
+        for (int d = 0; d < w; d++) acc = fma(kr[d], tr[d], acc);
+        T1[i] = acc;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_khat_y(const double* __restrict__ T1, const double* __restrict__ trig,
+                                                const unsigned char* __restrict__ par, double* __restrict__ T2,
+                                                int w, int h1, int h2, int nch) {
+    const size_t n = (size_t)nch * w * h1 * h2;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int kx = (int)(i % (size_t)h2);
+        const int ky = (int)((i / (size_t)h2) % (size_t)h1);
+        const size_t cw = i / ((size_t)h2 * h1);  // (ch, dw)
+        const int ch = (int)(cw / (size_t)w);
+        const double* tr = trig + ((size_t)((par[ch] >> 1) & 1) * h1 + ky) * w;
+        const double* src = T1 + cw * w * h2 + kx;  // T1[ch][dw][dv][kx]
+        double acc = 0.0;
+        for (int d = 0; d < w; d++) acc = fma(src[(size_t)d * h2], tr[d], acc);
+        T2[i] = acc;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_khat_z(const double* __restrict__ T2, const double* __restrict__ trig,
+                                                const unsigned char* __restrict__ par, float* __restrict__ Kh,
+                                                int w, int h0, int h1, int h2, int nch, double scale) {
+    const size_t plane = (size_t)h1 * h2;
+    const size_t n = (size_t)nch * h0 * plane;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t kyx = i % plane;
+        const int kz = (int)((i / plane) % (size_t)h0);
+        const int ch = (int)(i / (plane * h0));
+        const unsigned pb = par[ch];
+        const double* tr = trig + ((size_t)((pb >> 2) & 1) * h0 + kz) * w;
+        const double* src = T2 + (size_t)ch * w * plane + kyx;  // T2[ch][dw][ky][kx]
+        double acc = 0.0;
+        for (int d = 0; d < w; d++) acc = fma(src[(size_t)d * plane], tr[d], acc);
+        const int odd = (pb & 1) + ((pb >> 1) & 1) + ((pb >> 2) & 1);
+        Kh[i] = (float)((odd == 2 ? -scale : scale) * acc);  // (-i)^2 for two sine axes
+    }
+}
+
+// ---------------------------------------------------------------- host: offset kernels
+// Fibonacci points on the unit sphere; the monomial matrix at these points
+// has full rank 15 (even polynomials of degree <= 4 restricted to the sphere).
+static void fit_points(std::vector<double>& Q, int npts) {
+    Q.resize((size_t)npts * 3);
+    const double ga = M_PI * (3.0 - std::sqrt(5.0));
+    for (int i = 0; i < npts; i++) {
+        const double z = 1.0 - (2.0 * i + 1.0) / npts;
+        const double rr = std::sqrt(std::max(0.0, 1.0 - z * z));
+        Q[i * 3 + 0] = rr * std::cos(ga * i);
+        Q[i * 3 + 1] = rr * std::sin(ga * i);
+        Q[i * 3 + 2] = z;
+    }
+}
+
+static double mono_eval(const double* q, int m) {
+    return std::pow(q[0], mono_a(m)) * std::pow(q[1], mono_b(m)) * std::pow(q[2], mono_c(m));
+}
+
+// Least-squares projector P (15 x npts): coefficients = P * values.
+static bool fit_projector(const std::vector<double>& Q, int npts, std::vector<double>& P) {
+    std::vector<double> A((size_t)npts * kMono), N(kMono * 2 * kMono, 0.0);
+    for (int i = 0; i < npts; i++)
+        for (int m = 0; m < kMono; m++) A[(size_t)i * kMono + m] = mono_eval(&Q[i * 3], m);
+    // [A^T A | I] -> [I | (A^T A)^-1], Gauss-Jordan with partial pivoting.
+    for (int r = 0; r < kMono; r++) {
+        for (int c = 0; c < kMono; c++) {
+            double s = 0.0;
+            for (int i = 0; i < npts; i++) s += A[(size_t)i * kMono + r] * A[(size_t)i * kMono + c];
+            N[r * 2 * kMono + c] = s;
+        }
+        N[r * 2 * kMono + kMono + r] = 1.0;
+    }
+    for (int c = 0; c < kMono; c++) {
+        int piv = c;
+        for (int r = c + 1; r < kMono; r++)
+            if (std::fabs(N[r * 2 * kMono + c]) > std::fabs(N[piv * 2 * kMono + c])) piv = r;
+        if (std::fabs(N[piv * 2 * kMono + c]) < 1e-12) return false;
+        if (piv != c)
+            for (int k = 0; k < 2 * kMono; k++) std::swap(N[c * 2 * kMono + k], N[piv * 2 * kMono + k]);
+        const double inv = 1.0 / N[c * 2 * kMono + c];
+        for (int k = 0; k < 2 * kMono; k++) N[c * 2 * kMono + k] *= inv;
+        for (int r = 0; r < kMono; r++) {
+            if (r == c) continue;
+            const double f = N[r * 2 * kMono + c];
+            if (f != 0.0)
+                for (int k = 0; k < 2 * kMono; k++) N[r * 2 * kMono + k] -= f * N[c * 2 * kMono + k];
+        }
+    }
+    P.assign((size_t)kMono * npts, 0.0);
+    for (int m = 0; m < kMono; m++)
+        for (int i = 0; i < npts; i++) {
+            double s = 0.0;
+            for (int k = 0; k < kMono; k++) s += N[m * 2 * kMono + kMono + k] * A[(size_t)i * kMono + k];
+            P[(size_t)m * npts + i] = s;
+        }
+    return true;
+}
+
+static void sym_add(double V[6], double f, const double a[3], const double b[3]) {  // f (a b^T + b a^T) / 2
+    V[0] += f * a[0] * b[0];
+    V[1] += 0.5 * f * (a[0] * b[1] + a[1] * b[0]);
+    V[2] += 0.5 * f * (a[0] * b[2] + a[2] * b[0]);
+    V[3] += f * a[1] * b[1];
+    V[4] += 0.5 * f * (a[1] * b[2] + a[2] * b[1]);
+    V[5] += f * a[2] * b[2];
+}
+
+// Pair votes (FP64) of a unit voter vector q at unit offset d (d = 0 at the
+// voter itself), attenuations c1, c2, power 1, normalised as the gather path
+// normalises them (1 / sticknorm3, 1 / pi).
+static void stick_vote64(const double q[3], const double d[3], double c1, double c2, double sn, double V[6]) {
+    const double t = q[0] * d[0] + q[1] * d[1] + q[2] * d[2];
+    const double eta = c1 * (1.0 - t * t) + c2 * t * t;
+    const double r[3] = {q[0] - 2 * t * d[0], q[1] - 2 * t * d[1], q[2] - 2 * t * d[2]};
+    for (int k = 0; k < 6; k++) V[k] = 0.0;
+    sym_add(V, eta * sn, r, r);
+}
+
+static void plate_vote64(const double n[3], const double d[3], bool self, const std::vector<double>& al,
+                         const std::vector<double>& be, const std::vector<double>& ga,
+                         const std::vector<double>& de, double V[6]) {
+    for (int k = 0; k < 6; k++) V[k] = 0.0;
+    const double s = n[0] * d[0] + n[1] * d[1] + n[2] * d[2];
+    const double x = 1.0 - s * s;
+    auto poly = [x](const std::vector<double>& c) {
+        double r = 0.0;
+        for (size_t i = c.size(); i-- > 0;) r = r * x + c[i];
+        return r;
+    };
+    const double A = poly(al) / M_PI, B = poly(be) / M_PI;
+    V[0] += A; V[3] += A; V[5] += A;
+    sym_add(V, B, n, n);
+    if (!self) {
+        const double G = s * poly(ga) / M_PI, D = 2.0 * poly(de) / M_PI;  // de holds delta / 2
+        sym_add(V, 2.0 * G, n, d);
+        sym_add(V, D, d, d);
+    }
+}
+
+// K[ch][dw][dv][du], ch = c * nf + f (see tira_spectral_offset_kernels).
+static int offset_kernels(unsigned w, float s1, float s2, bool stick, bool plate, std::vector<double>& K, int& nf) {
+    const int r = (int)(w / 2), W = 2 * r + 1;
+    nf = (stick ? kMono : 0) + (plate ? kMono : 0);
+    const int npts = 32;
+    std::vector<double> Q, P;
+    fit_points(Q, npts);
+    if (!fit_projector(Q, npts, P)) {
+        set_error("spectral vote: degenerate fitting points (internal error)");
+        return TIRA_ERR_INVALID_ARG;
+    }
+    const double sn = 1.0 / (double)sticknorm3_host(s1, s2, 1);
+    const size_t noff = (size_t)W * W * W;
+    K.assign((size_t)6 * nf * noff, 0.0);
+    std::vector<double> al, be, ga, de, Vs((size_t)npts * 6), Vp((size_t)npts * 6);
+    size_t o = 0;
+    for (int dw = -r; dw <= r; dw++)
+        for (int dv = -r; dv <= r; dv++)
+            for (int du = -r; du <= r; du++, o++) {
+                const double l2 = (double)du * du + (double)dv * dv + (double)dw * dw;
+                double d[3] = {0.0, 0.0, 0.0};
+                if (l2 > 0) { const double L = std::sqrt(l2); d[0] = du / L; d[1] = dv / L; d[2] = dw / L; }
+                const double c1 = s1 > 0 ? std::exp(-l2 / ((double)s1 * s1)) : (l2 > 0 ? 0.0 : 1.0);
+                const double c2 = s2 > 0 ? std::exp(-l2 / ((double)s2 * s2)) : (l2 > 0 ? 0.0 : 1.0);
+                if (plate) {
+                    if (l2 == 0) {
+                        al.assign(2, 0.0); be.assign(2, 0.0); ga.assign(2, 0.0); de.assign(3, 0.0);
+                        al[0] = 0.5 * M_PI * c1;
+                        be[0] = -0.5 * M_PI * c1;
+                    } else {
+                        plate_polys(1, c1, c2, al, be, ga, de);
+                    }
+                }
+                for (int i = 0; i < npts; i++) {
+                    if (stick) stick_vote64(&Q[i * 3], d, c1, c2, sn, &Vs[(size_t)i * 6]);
+                    if (plate) plate_vote64(&Q[i * 3], d, l2 == 0, al, be, ga, de, &Vp[(size_t)i * 6]);
+                }
+                for (int kind = 0; kind < 2; kind++) {
+                    if ((kind == 0 && !stick) || (kind == 1 && !plate)) continue;
+                    const std::vector<double>& V = kind == 0 ? Vs : Vp;
+                    const int f0 = (kind == 1 && stick) ? kMono : 0;
+                    for (int m = 0; m < kMono; m++)
+                        for (int c = 0; c < 6; c++) {
+                            double acc = 0.0;
+                            for (int i = 0; i < npts; i++) acc += P[(size_t)m * npts + i] * V[(size_t)i * 6 + c];
+                            K[((size_t)c * nf + f0 + m) * noff + o] = acc;
+                        }
+                }
+            }
+    return TIRA_OK;
+}
+
+// ---------------------------------------------------------------- host: spectra cache
+struct SpecKey {
+    int L0, L1, L2, w;
+    float s1, s2;
+    int stick, plate;
+    bool operator==(const SpecKey& o) const {
+        return L0 == o.L0 && L1 == o.L1 && L2 == o.L2 && w == o.w && s1 == o.s1 && s2 == o.s2 &&
+               stick == o.stick && plate == o.plate;
+    }
+};
+
+struct SpecCache {
+    SpecKey key{0, 0, 0, 0, 0, 0, 0, 0};
+    int device = -1;
+    void* buf = nullptr;  // twiddles (3 axes) then the octant spectra
+    size_t bytes = 0;
+    ~SpecCache() {
+        if (buf) (void)hipFree(buf);
+    }
+};
+static thread_local SpecCache g_spec;
+
+static void twiddles(int L, float2* t) {
+    for (int j = 0; j < L; j++) {
+        const double a = -2.0 * M_PI * (double)j / (double)L;
+        t[j] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+}
+
+static size_t tw_bytes(int L) { return ((size_t)L * 8 + 255) & ~size_t(255); }
+
+// Kernel spectra for `key` (built and uploaded when the key changes).
+static int spectra_for(const SpecKey& key, const SpecGeom& g, hipStream_t st, const float2** tw0,
+                       const float2** tw1, const float2** tw2, const float** kh) {
+    int dev = 0;
+    TH_HIP(hipGetDevice(&dev));
+    const size_t tb = tw_bytes(key.L0) + tw_bytes(key.L1) + tw_bytes(key.L2);
+    const size_t kbytes = (size_t)6 * g.nf * g.ocs * 4;
+    auto set_ptrs = [&]() {
+        char* b = (char*)g_spec.buf;
+        *tw0 = (const float2*)b;
+        *tw1 = (const float2*)(b + tw_bytes(key.L0));
+        *tw2 = (const float2*)(b + tw_bytes(key.L0) + tw_bytes(key.L1));
+        *kh = (const float*)(b + tb);
+    };
+    if (g_spec.buf && g_spec.key == key && g_spec.device == dev) {
+        set_ptrs();
+        return TIRA_OK;
+    }
+    TH_HIP(hipStreamSynchronize(st));
+    if (g_spec.buf) {
+        int cur = dev;
+        if (g_spec.device != dev) TH_HIP(hipSetDevice(g_spec.device));
+        (void)hipFree(g_spec.buf);
+        if (g_spec.device != dev) TH_HIP(hipSetDevice(cur));
+        g_spec.buf = nullptr;
+    }
+    if (hipMalloc(&g_spec.buf, tb + kbytes) != hipSuccess) {
+        g_spec.buf = nullptr;
+        (void)hipGetLastError();
+        set_error("spectral vote: out of device memory for the kernel spectra (%zu bytes)", tb + kbytes);
+        return TIRA_ERR_OUT_OF_MEMORY;
+    }
+    g_spec.bytes = tb + kbytes;
+    g_spec.device = dev;
+    g_spec.key = SpecKey{0, 0, 0, 0, 0, 0, 0, 0};
+    set_ptrs();
+
+    // twiddles
+    {
+        std::vector<float2> t(tb / 8);
+        twiddles(key.L0, t.data());
+        twiddles(key.L1, t.data() + tw_bytes(key.L0) / 8);
+        twiddles(key.L2, t.data() + (tw_bytes(key.L0) + tw_bytes(key.L1)) / 8);
+        TH_HIP(hipMemcpy(g_spec.buf, t.data(), tb, hipMemcpyHostToDevice));
+    }
+    // offset kernels, parities and trig tables
+    std::vector<double> K;
+    int nf = 0;
+    TH_TRY(offset_kernels((unsigned)key.w, key.s1, key.s2, key.stick != 0, key.plate != 0, K, nf));
+    const int w = 2 * (key.w / 2) + 1, r = key.w / 2;
+    const int nch = 6 * nf;
+    std::vector<unsigned char> par(nch);
+    for (int c = 0; c < 6; c++)
+        for (int f = 0; f < nf; f++) {
+            const int m = f % kMono;
+            par[c * nf + f] = (unsigned char)(((comp_nx(c) + mono_a(m)) & 1) | (((comp_ny(c) + mono_b(m)) & 1) << 1) |
+                                              (((comp_nz(c) + mono_c(m)) & 1) << 2));
+        }
+    const int hs[3] = {g.h0, g.h1, g.h2}, Ls[3] = {g.L0, g.L1, g.L2};
+    std::vector<double> trig[3];
+    for (int a = 0; a < 3; a++) {
+        trig[a].resize((size_t)2 * hs[a] * w);
+        for (int k = 0; k < hs[a]; k++)
+            for (int d = -r; d <= r; d++) {
+                const double ang = 2.0 * M_PI * (double)k * (double)d / (double)Ls[a];
+                trig[a][(size_t)k * w + d + r] = std::cos(ang);
+                trig[a][((size_t)hs[a] + k) * w + d + r] = std::sin(ang);
+            }
+    }
+    const size_t t1n = (size_t)nch * w * w * g.h2, t2n = (size_t)nch * w * g.h1 * g.h2;
+    const size_t trig_n = trig[0].size() + trig[1].size() + trig[2].size();
+    const size_t tmp_bytes = (K.size() + t1n + t2n + trig_n) * 8 + nch + 256;
+    char* tmp = nullptr;
+    if (hipMalloc((void**)&tmp, tmp_bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("spectral vote: out of device memory building the kernel spectra (%zu bytes)", tmp_bytes);
+        return TIRA_ERR_OUT_OF_MEMORY;
+    }
+    double* dK = (double*)tmp;
+    double* dT1 = dK + K.size();
+    double* dT2 = dT1 + t1n;
+    double* dtr[3];
+    dtr[0] = dT2 + t2n;
+    dtr[1] = dtr[0] + trig[0].size();
+    dtr[2] = dtr[1] + trig[1].size();
+    unsigned char* dpar = (unsigned char*)(dtr[2] + trig[2].size());
+    int rc = TIRA_OK;
+    do {
+        if (hipMemcpy(dK, K.data(), K.size() * 8, hipMemcpyHostToDevice) != hipSuccess) { rc = -2; break; }
+        bool ok = true;
+        for (int a = 0; a < 3; a++)
+            ok = ok && hipMemcpy(dtr[a], trig[a].data(), trig[a].size() * 8, hipMemcpyHostToDevice) == hipSuccess;
+        ok = ok && hipMemcpy(dpar, par.data(), nch, hipMemcpyHostToDevice) == hipSuccess;
+        if (!ok) { rc = -2; break; }
+        auto grid = [](size_t n) { return (unsigned)std::min<size_t>(ceil_div(n, 256), 65536); };
+        k_khat_x<<<grid(t1n), 256, 0, st>>>(dK, dtr[2], dpar, dT1, w, g.h2, nch);
+        k_khat_y<<<grid(t2n), 256, 0, st>>>(dT1, dtr[1], dpar, dT2, w, g.h1, g.h2, nch);
+        const double scale = 0.5 / ((double)g.L0 * g.L1 * g.L2);
+        k_khat_z<<<grid((size_t)nch * g.ocs), 256, 0, st>>>(dT2, dtr[0], dpar, (float*)*kh, w, g.h0, g.h1, g.h2,
+                                                           nch, scale);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) rc = -2;
+    } while (false);
+    (void)hipFree(tmp);
+    if (rc != TIRA_OK) {
+        set_error("spectral vote: building the kernel spectra failed");
+        return TIRA_ERR_HIP;
+    }
+    g_spec.key = key;
+    return TIRA_OK;
+}
+
+static int launch_lines(float2* data, const LineGeom& lg, const FftPlan& pl, const float2* tw, bool inv, bool rows,
+                        hipStream_t st) {
+    const size_t nblk = (size_t)lg.nab * lg.nb * lg.nchan;
+    if (nblk == 0) return TIRA_OK;
+    TH_ARG(nblk < (1ull << 31), "spectral vote: volume too large");
+    const unsigned grid = (unsigned)nblk;
+    if (inv) {
+        if (rows) k_fft_lines<true, true><<<grid, kFT, 0, st>>>(data, lg, pl, tw);
+        else k_fft_lines<true, false><<<grid, kFT, 0, st>>>(data, lg, pl, tw);
+    } else {
+        if (rows) k_fft_lines<false, true><<<grid, kFT, 0, st>>>(data, lg, pl, tw);
+        else k_fft_lines<false, false><<<grid, kFT, 0, st>>>(data, lg, pl, tw);
+    }
+    TH_HIP(hipGetLastError());
+    return TIRA_OK;
+}
+
+// Lines of the axis with stride `estride` over a (n0 x n1 x n2)-indexed
+// stack: the batch dimension `a` is x (stride 1) for the y and z axes.
+static LineGeom strided_geom(size_t cs, int nchan, int L, size_t estride, int na, int nb, size_t bstride, int n_in,
+                             int n_out) {
+    LineGeom g{};
+    g.cstride = cs;
+    g.bstride = bstride;
+    g.astride = 1;
+    g.estride = estride;
+    g.na = na;
+    g.nb = nb;
+    g.nchan = nchan;
+    g.B = std::max(1, std::min(na, kFftBL / L));
+    g.nab = (int)ceil_div((size_t)na, (size_t)g.B);
+    g.n_in = n_in;
+    g.n_out = n_out;
+    return g;
+}
+
+// Whether TIRA_PLATE_AUTO takes the spectral vote: power 1, an exact plate
+// (samples 0 or >= 3), FFT lengths within range, a window of at least 5 and
+// a volume of at least 32^3 voxels (below that the gather costs microseconds
+// and the first call's spectra build would dominate), and device memory for
+// the spectra, their build and the feature volumes within 80% of the device.
+bool spectral_preferred(unsigned s0, unsigned s1, unsigned s2, unsigned w, unsigned power, bool stick, bool plate,
+                        unsigned samples) {
+    const unsigned p = power < 1 ? 1 : power;
+    if (p != 1 || (plate && samples > 0 && samples < 3) || w < 5) return false;
+    if ((size_t)s0 * s1 * s2 < 32768) return false;
+    const unsigned L0 = spectral_length(s0, w), L1 = spectral_length(s1, w), L2 = spectral_length(s2, w);
+    if (!L0 || !L1 || !L2 || 3 * L2 > (unsigned)kFftBL) return false;
+    const size_t nf = (stick ? kMono : 0) + (plate ? kMono : 0), nz = (nf + 1) / 2;
+    const size_t W = 2 * (w / 2) + 1;
+    const size_t h0 = L0 / 2 + 1, h1 = L1 / 2 + 1, h2 = L2 / 2 + 1;
+    const size_t need = nz * (size_t)L0 * L1 * L2 * 8 + 6 * nf * h0 * h1 * h2 * 4 + 6 * nf * W * h1 * h2 * 8 +
+                        (size_t)s0 * s1 * s2 * 32;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return need <= total_b / 10 * 8;
+}
+
+int vote3_spectral_run(const float* tensors, int in_layout, const float* lam, const float* ev2, const float* ev0,
+                       float* out, int out_layout, unsigned s0, unsigned s1, unsigned s2, float sigma1,
+                       float sigma2, unsigned w, unsigned power, bool stick, bool plate, unsigned samples,
+                       hipStream_t st) {
+    const unsigned p = power < 1 ? 1 : power;
+    if (p != 1 || (plate && samples > 0 && samples < 3)) return kSpectralNotApplicable;
+    const unsigned L0 = spectral_length(s0, w), L1 = spectral_length(s1, w), L2 = spectral_length(s2, w);
+    if (!L0 || !L1 || !L2 || 3 * L2 > (unsigned)kFftBL) return kSpectralNotApplicable;
+    FftPlan p0, p1, p2;
+    make_plan((int)L0, p0);
+    make_plan((int)L1, p1);
+    make_plan((int)L2, p2);
+
+    SpecGeom g{};
+    g.n0 = (int)s0; g.n1 = (int)s1; g.n2 = (int)s2;
+    g.L0 = (int)L0; g.L1 = (int)L1; g.L2 = (int)L2;
+    g.h0 = g.L0 / 2 + 1; g.h1 = g.L1 / 2 + 1; g.h2 = g.L2 / 2 + 1;
+    g.nf = (stick ? kMono : 0) + (plate ? kMono : 0);
+    g.nz = (g.nf + 1) / 2;
+    g.nst = stick ? kMono : 0;
+    g.cs = (size_t)L0 * L1 * L2;
+    g.ocs = (size_t)g.h0 * g.h1 * g.h2;
+
+    const size_t nvox = (size_t)s0 * s1 * s2;
+    float4 *ps = nullptr, *pp = nullptr;
+    float2* Z = nullptr;
+    if (stick) TH_TRY(ws_get(WS_PAY0, nvox * 16, (void**)&ps));
+    if (plate) TH_TRY(ws_get(WS_PAY1, nvox * 16, (void**)&pp));
+    TH_TRY(ws_get(WS_SPEC0, (size_t)g.nz * g.cs * 8, (void**)&Z));
+    const SpecKey key{g.L0, g.L1, g.L2, (int)w, sigma1, sigma2, stick ? 1 : 0, plate ? 1 : 0};
+    const float2 *tw0, *tw1, *tw2;
+    const float* kh;
+    TH_TRY(spectra_for(key, g, st, &tw0, &tw1, &tw2, &kh));
+
+    {
+        ProfScope prof("vote3_prep", st);
+        TH_TRY(vote3_payload_flat(tensors, in_layout, lam, ev2, ev0, s0, s1, s2, ps, pp, st));
+    }
+    const size_t plane = (size_t)L1 * L2;
+    {
+        ProfScope prof("spec_fwd_x", st);
+        const int B = std::max(1, kFftBL / g.L2);
+        const size_t nrows = (size_t)s0 * s1;
+        const size_t nblk = ceil_div(nrows, (size_t)B) * (size_t)g.nz;
+        TH_ARG(nblk < (1ull << 31), "spectral vote: volume too large");
+        k_spec_xfwd<<<(unsigned)nblk, kFT, 0, st>>>(ps, pp, g, B, p2, tw2, Z);
+        TH_HIP(hipGetLastError());
+    }
+    {
+        ProfScope prof("spec_fwd_y", st);
+        TH_TRY(launch_lines(Z, strided_geom(g.cs, g.nz, g.L1, L2, g.L2, g.n0, plane, g.n1, g.L1), p1, tw1, false,
+                            false, st));
+    }
+    {
+        ProfScope prof("spec_fwd_z", st);
+        TH_TRY(launch_lines(Z, strided_geom(g.cs, g.nz, g.L0, plane, g.L2, g.L1, L2, g.n0, g.L0), p0, tw0, false,
+                            false, st));
+    }
+    {
+        ProfScope prof("spec_mul", st);
+        k_spec_mul<<<(unsigned)ceil_div(g.ocs, 256), 256, 0, st>>>(Z, kh, g);
+        TH_HIP(hipGetLastError());
+    }
+    {
+        ProfScope prof("spec_inv_z", st);
+        TH_TRY(launch_lines(Z, strided_geom(g.cs, 3, g.L0, plane, g.L2, g.L1, L2, g.L0, g.n0), p0, tw0, true, false,
+                            st));
+    }
+    {
+        ProfScope prof("spec_inv_y", st);
+        TH_TRY(launch_lines(Z, strided_geom(g.cs, 3, g.L1, L2, g.L2, g.n0, plane, g.L1, g.n1), p1, tw1, true, false,
+                            st));
+    }
+    {
+        ProfScope prof("spec_inv_x", st);
+        const int B = std::max(1, kFftBL / (3 * g.L2));
+        const size_t nrows = (size_t)s0 * s1;
+        k_spec_xinv<<<ceil_div(nrows, (size_t)B), kFT, 0, st>>>(Z, g, B, p2, tw2, out, out_layout);
+        TH_HIP(hipGetLastError());
+    }
+    return TIRA_OK;
+}
+
+}  // namespace th
+
+using namespace th;
+
+extern "C" {
+
+int tira_spectral_offset_kernels(unsigned w, float sigma1, float sigma2, unsigned power, int stick, int plate,
+                                 double* out, unsigned* nf_out) {
+    TH_ARG(out, "NULL pointer");
+    TH_ARG(power == 1, "the spectral vote serves power 1 only");
+    TH_ARG(stick || plate, "neither stick nor plate");
+    TH_ARG(sigma1 >= 0.0f && sigma2 >= 0.0f, "sigmas must be >= 0");
+    std::vector<double> K;
+    int nf = 0;
+    TH_TRY(offset_kernels(w, sigma1, sigma2, stick != 0, plate != 0, K, nf));
+    std::copy(K.begin(), K.end(), out);
+    if (nf_out) *nf_out = (unsigned)nf;
+    return TIRA_OK;
+}
+
+unsigned tira_spectral_length(unsigned n, unsigned w) { return spectral_length(n, w); }
+
+int tira_fft3_async(float* data, unsigned nchan, unsigned L0, unsigned L1, unsigned L2, int inverse) {
+    TH_ARG(data, "NULL pointer");
+    FftPlan p0, p1, p2;
+    TH_ARG(L0 >= 1 && L1 >= 1 && L2 >= 1 && L0 <= (unsigned)kMaxL && L1 <= (unsigned)kMaxL &&
+               L2 <= (unsigned)kMaxL,
+           "FFT lengths must be in [1, %d]", kMaxL);
+    TH_ARG(make_plan((int)L0, p0) && make_plan((int)L1, p1) && make_plan((int)L2, p2),
+           "FFT lengths must have prime factors 2, 3, 5, 7 only");
+    TH_ARG(is_device(data), "tira_fft3_async takes a device pointer");
+    if (!nchan) return TIRA_OK;
+    hipStream_t st = stream();
+    float2* d = (float2*)data;
+    const size_t cs = (size_t)L0 * L1 * L2, plane = (size_t)L1 * L2;
+    const bool inv = inverse != 0;
+    // Twiddle tables of the three lengths in a workspace slot of this entry.
+    const size_t tb = tw_bytes((int)L0) + tw_bytes((int)L1) + tw_bytes((int)L2);
+    float2* tw = nullptr;
+    TH_TRY(ws_get(WS_SPEC1, tb, (void**)&tw));
+    std::vector<float2> t(tb / 8);
+    twiddles((int)L0, t.data());
+    twiddles((int)L1, t.data() + tw_bytes((int)L0) / 8);
+    twiddles((int)L2, t.data() + (tw_bytes((int)L0) + tw_bytes((int)L1)) / 8);
+    TH_HIP(hipStreamSynchronize(st));  // earlier calls may still read the previous tables
+    TH_HIP(hipMemcpy(tw, t.data(), tb, hipMemcpyHostToDevice));
+    const float2* t0 = tw;
+    const float2* t1 = tw + tw_bytes((int)L0) / 8;
+    const float2* t2 = tw + (tw_bytes((int)L0) + tw_bytes((int)L1)) / 8;
+    // x: rows of L2 contiguous elements; the batch dimension is the row (stride L2)
+    LineGeom gx{};
+    gx.cstride = cs; gx.bstride = plane; gx.astride = L2; gx.estride = 1;
+    gx.na = (int)L1; gx.nb = (int)L0; gx.nchan = (int)nchan;
+    gx.B = std::max(1, std::min((int)L1, kFftBL / (int)L2));
+    gx.nab = (int)ceil_div(L1, (size_t)gx.B);
+    gx.n_in = (int)L2; gx.n_out = (int)L2;
+    TH_TRY(launch_lines(d, gx, p2, t2, inv, true, st));
+    // y: columns x of every plane z; z: columns x of every row y
+    TH_TRY(launch_lines(d, strided_geom(cs, (int)nchan, (int)L1, L2, (int)L2, (int)L0, plane, (int)L1, (int)L1), p1,
+                        t1, inv, false, st));
+    TH_TRY(launch_lines(d, strided_geom(cs, (int)nchan, (int)L0, plane, (int)L2, (int)L1, L2, (int)L0, (int)L0), p0,
+                        t0, inv, false, st));
+    return TIRA_OK;
+}
+
+}  // extern "C"

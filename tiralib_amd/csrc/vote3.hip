@@ -86,7 +86,7 @@ static void offset_geometry(int du, int dv, int dw, float s1, float s2, float d[
 // The N-point Riemann sum of the reference (pi/N sum over beta_i = i pi/N,
 // tensorvote.h:1676-1689) integrates trigonometric polynomials of degree
 // 2p+2 exactly once N >= p+2, so it equals this closed form there.
-static void plate_polys(int P, double c1, double c2, std::vector<double>& al, std::vector<double>& be,
+void plate_polys(int P, double c1, double c2, std::vector<double>& al, std::vector<double>& be,
                         std::vector<double>& ga, std::vector<double>& de) {
     std::vector<double> e(P + 1), A(P + 3, 0.0), B(P + 3, 0.0), C(P + 3, 0.0), D(P + 3, 0.0);
     for (int j = 0; j <= P; j++) e[j] = c1 * binom(P, j) * ((j & 1) ? -1.0 : 1.0) + (j == P ? c2 : 0.0);
@@ -961,6 +961,22 @@ VoteGeom make_geom(unsigned s0, unsigned s1, unsigned s2, unsigned w, int bz) {
     return g;
 }
 
+// Unpadded voter payloads (stick (e2, scale), plate (e0, scale)) of every
+// voxel, for the spectral vote; either pointer may be NULL.
+int vote3_payload_flat(const float* tensors, int in_layout, const float* lam, const float* ev2, const float* ev0,
+                       unsigned s0, unsigned s1, unsigned s2, float4* ps, float4* pp, hipStream_t st) {
+    VoteGeom g{};
+    g.s0 = (int)s0; g.s1 = (int)s1; g.s2 = (int)s2;
+    g.r = 0;
+    g.P0 = (int)s0; g.P1 = (int)s1; g.P2 = (int)s2;
+    const size_t n = (size_t)s0 * s1 * s2;
+    const unsigned grid = (unsigned)std::min<size_t>(ceil_div(n, 256), 65536);
+    if (tensors) k_vote3_prep<<<grid, 256, 0, st>>>(tensors, in_layout, g, ps, pp, nullptr, 1.0f);
+    else k_vote3_prep_tk<<<grid, 256, 0, st>>>(lam, ev2, ev0, g, ps, pp, nullptr, 1.0f);
+    TH_HIP(hipGetLastError());
+    return TIRA_OK;
+}
+
 int resolve_plate_mode(int plate_mode, unsigned power, unsigned samples) {
     const unsigned p = power < 1 ? 1 : power;
     if (plate_mode == TIRA_PLATE_AUTO) {
@@ -1056,6 +1072,14 @@ int vote3_run(const float* tensors, int in_layout, const float* lam, const float
         k_fill_zero<<<std::min<size_t>(4096, ceil_div(nf, 256)), 256, 0, st>>>(out, nf);
         TH_HIP(hipGetLastError());
         return TIRA_OK;
+    }
+    if (plate_mode == TIRA_PLATE_AUTO && spectral_preferred(s0, s1, s2, w, power, stick, plate, samples))
+        plate_mode = TIRA_VOTE_SPECTRAL;
+    if (plate_mode == TIRA_VOTE_SPECTRAL) {
+        const int rc = vote3_spectral_run(tensors, in_layout, lam, ev2, ev0, out, out_layout, s0, s1, s2, sigma1,
+                                          sigma2, w, power, stick, plate, samples, st);
+        if (rc != kSpectralNotApplicable) return rc;
+        plate_mode = TIRA_PLATE_AUTO;  // power, samples or size outside the spectral method: direct gather
     }
     const unsigned p = power < 1 ? 1 : power;
     const int mode = plate ? resolve_plate_mode(plate_mode, power, samples) : TIRA_PLATE_CLOSED;
@@ -1341,7 +1365,7 @@ int tira_tensorvote3_async(const float* input_field, int in_layout, float* outpu
     TH_ARG(input_field && output_field, "NULL pointer");
     TH_ARG(in_layout == TIRA_LAYOUT_MAT9 || in_layout == TIRA_LAYOUT_PACKED6, "bad input layout");
     TH_ARG(out_layout == TIRA_LAYOUT_MAT9 || out_layout == TIRA_LAYOUT_PACKED6, "bad output layout");
-    TH_ARG(plate_mode >= TIRA_PLATE_AUTO && plate_mode <= TIRA_PLATE_CLOSED, "bad plate mode %d", plate_mode);
+    TH_ARG((plate_mode >= TIRA_PLATE_AUTO && plate_mode <= TIRA_PLATE_CLOSED) || plate_mode == TIRA_VOTE_SPECTRAL, "bad plate mode %d", plate_mode);
     TH_TRY(check_vote_args(s0, s1, s2, w, sigma1, sigma2));
     return vote3_run(input_field, in_layout, nullptr, nullptr, nullptr, output_field, out_layout, s0, s1, s2, sigma1,
                      sigma2, w, power, stick != 0, plate != 0, samples, plate_mode, stream());
