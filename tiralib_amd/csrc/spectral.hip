@@ -46,7 +46,10 @@
 namespace th {
 
 constexpr int kFT = 256;       // threads per FFT block
-constexpr int kFftBL = 4096;   // complex values staged per block (32 KB of LDS)
+#ifndef TH_FFT_BL
+#define TH_FFT_BL 4096
+#endif
+constexpr int kFftBL = TH_FFT_BL;  // complex values staged per block (32 KB of LDS)
 constexpr int kMaxL = 1024;    // longest transform
 constexpr int kMaxStages = 16;
 constexpr int kMono = 15;      // quartic monomials
@@ -194,6 +197,15 @@ __device__ __forceinline__ float2 ld2(const lds_f2* p) {
 }
 __device__ __forceinline__ void st2(lds_f2* p, float2 a) { *p = f2v{a.x, a.y}; }
 
+// n / d for 0 <= n < 2^22 and d >= 1, with rd = 1 / d in FP32: the float
+// quotient is within one of the exact one, which the remainder corrects.
+__device__ __forceinline__ int fdiv(int n, int d, float rd) {
+    int q = (int)((float)n * rd);
+    const int r = n - q * d;
+    q += (r >= d) - (r < 0);
+    return q;
+}
+
 // One Stockham stage of radix R over `nlines` lines of length L in LDS
 // (line stride L), p = product of the radices already applied: butterfly i
 // of a line reads u_r = x[i + r L/R], twiddles u_r by W_L^(r k L/(pR)),
@@ -208,12 +220,13 @@ __device__ __noinline__ void fft_stage(lds_f2* __restrict__ s, int L, int nlines
     const int m = L / R;
     const int nb = nlines * m;
     const int step = L / (p * R);
+    const float rm = 1.0f / (float)m, rp = 1.0f / (float)p;
     float2 u[MAXQ][R];
 #pragma unroll
     for (int q = 0; q < MAXQ; q++) {
         const int t = (int)threadIdx.x + q * kFT;
         if (t < nb) {
-            const int line = t / m, i = t - line * m;
+            const int line = fdiv(t, m, rm), i = t - line * m;
             const lds_f2* src = s + line * L + i;
 #pragma unroll
             for (int r = 0; r < R; r++) u[q][r] = ld2(src + r * m);
@@ -224,8 +237,8 @@ __device__ __noinline__ void fft_stage(lds_f2* __restrict__ s, int L, int nlines
     for (int q = 0; q < MAXQ; q++) {
         const int t = (int)threadIdx.x + q * kFT;
         if (t < nb) {
-            const int line = t / m, i = t - line * m;
-            const int k = i % p;
+            const int line = fdiv(t, m, rm), i = t - line * m;
+            const int k = i - fdiv(i, p, rp) * p;
 #pragma unroll
             for (int r = 1; r < R; r++) {
                 float2 w = ld2(tw + r * k * step);  // r k step < L
@@ -275,6 +288,7 @@ struct LineGeom {
     size_t cstride, bstride, astride, estride;
     int na, nb, nchan, B, nab;
     int n_in, n_out;
+    int lb;  // log2 B for k_fft_cols (B a power of two)
 };
 
 // ROWS: elements contiguous (estride 1), loads walk a line first; otherwise
@@ -316,6 +330,133 @@ __global__ __launch_bounds__(kFT) void k_fft_lines(float2* __restrict__ data, Li
     }
 }
 
+// Strided passes (y and z): the B lines of a block are B adjacent columns
+// (astride 1, B a power of two, lane-fastest in every global access). The
+// first radix stage reads its butterfly inputs straight from global memory
+// (p = 1: no twiddles), the last one writes its outputs straight back
+// (p = L / R: output r of butterfly i is element i + r L/R), so the lines
+// touch LDS only between stages.
+template <int R, bool INV>
+__device__ __forceinline__ void cols_first(const float2* __restrict__ base, lds_f2* __restrict__ s, int L, int B,
+                                           int lb, int Bv, size_t estride, int n_in) {
+    // every load of the thread is issued before the first butterfly
+    constexpr int MAXQ = (kFftBL / R + kFT - 1) / kFT;
+    const int m = L / R;
+    const int nb = B * m;
+    float2 u[MAXQ][R];
+#pragma unroll
+    for (int q = 0; q < MAXQ; q++) {
+        const int t = (int)threadIdx.x + q * kFT;
+        const int l = t & (B - 1), i = t >> lb;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int e = i + r * m;
+            u[q][r] = (t < nb && l < Bv && e < n_in) ? base[(size_t)e * estride + l] : make_float2(0.0f, 0.0f);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < MAXQ; q++) {
+        const int t = (int)threadIdx.x + q * kFT;
+        if (t < nb) {
+            const int l = t & (B - 1), i = t >> lb;
+            dft_small<R, INV>(u[q]);
+            lds_f2* d = s + l * L + i * R;
+#pragma unroll
+            for (int r = 0; r < R; r++) st2(d + r, u[q][r]);
+        }
+    }
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void cols_last(const lds_f2* __restrict__ s, float2* __restrict__ base,
+                                          const lds_f2* __restrict__ tw, int L, int B, int lb, int Bv,
+                                          size_t estride, int n_out) {
+    const int m = L / R;
+    for (int t = (int)threadIdx.x; t < B * m; t += kFT) {
+        const int l = t & (B - 1), i = t >> lb;
+        float2 u[R];
+        const lds_f2* src = s + l * L + i;
+#pragma unroll
+        for (int r = 0; r < R; r++) u[r] = ld2(src + r * m);
+#pragma unroll
+        for (int r = 1; r < R; r++) {
+            float2 w = ld2(tw + r * i);  // W_L^(r i), r i < L
+            if (INV) w.y = -w.y;
+            u[r] = cmul(u[r], w);
+        }
+        dft_small<R, INV>(u);
+        if (l < Bv) {
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int e = i + r * m;
+                if (e < n_out) base[(size_t)e * estride + l] = u[r];
+            }
+        }
+    }
+}
+
+// Whole transform of a length-R line (R <= 8) in registers.
+template <int R, bool INV>
+__device__ __forceinline__ void cols_single(float2* __restrict__ base, int B, int Bv, size_t estride, int n_in,
+                                            int n_out) {
+    for (int l = (int)threadIdx.x; l < B; l += kFT) {
+        float2 u[R];
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            u[r] = (l < Bv && r < n_in) ? base[(size_t)r * estride + l] : make_float2(0.0f, 0.0f);
+        dft_small<R, INV>(u);
+        if (l < Bv) {
+#pragma unroll
+            for (int r = 0; r < R; r++)
+                if (r < n_out) base[(size_t)r * estride + l] = u[r];
+        }
+    }
+}
+
+#define TH_RADIX_SWITCH(R, CALL)                          \
+    switch (R) {                                           \
+        case 2: { constexpr int RR = 2; CALL; } break;     \
+        case 3: { constexpr int RR = 3; CALL; } break;     \
+        case 4: { constexpr int RR = 4; CALL; } break;     \
+        case 5: { constexpr int RR = 5; CALL; } break;     \
+        case 7: { constexpr int RR = 7; CALL; } break;     \
+        default: { constexpr int RR = 8; CALL; } break;    \
+    }
+
+template <bool INV>
+__global__ __launch_bounds__(kFT) void k_fft_cols(float2* __restrict__ data, LineGeom g, FftPlan pl,
+                                                  const float2* __restrict__ twg) {
+    __shared__ float2 s_[kFftBL];
+    __shared__ float2 tw_[kMaxL];
+    const int L = pl.L;
+    unsigned bid = blockIdx.x;
+    const int ab = (int)(bid % (unsigned)g.nab);
+    bid /= (unsigned)g.nab;
+    const int b = (int)(bid % (unsigned)g.nb);
+    const int c = (int)(bid / (unsigned)g.nb);
+    if (c >= g.nchan) return;
+    const int B = g.B, lb = g.lb;
+    const int a0 = ab * B;
+    const int Bv = min(B, g.na - a0);
+    float2* base = data + (size_t)c * g.cstride + (size_t)b * g.bstride + (size_t)a0;
+    if (pl.nst == 1) {
+        TH_RADIX_SWITCH(pl.rad[0], (cols_single<RR, INV>(base, B, Bv, g.estride, g.n_in, g.n_out)));
+        return;
+    }
+    load_twiddles(tw_, twg, L);
+    lds_f2* s = to_lds(s_);
+    const lds_f2* tw = to_lds(tw_);
+    TH_RADIX_SWITCH(pl.rad[0], (cols_first<RR, INV>(base, s, L, B, lb, Bv, g.estride, g.n_in)));
+    __syncthreads();
+    int p = pl.rad[0];
+    for (int st = 1; st < pl.nst - 1; st++) {
+        const int R = pl.rad[st];
+        TH_RADIX_SWITCH(R, (fft_stage<RR, INV>(s, L, B, p, tw)));
+        p *= R;
+    }
+    TH_RADIX_SWITCH(pl.rad[pl.nst - 1], (cols_last<RR, INV>(s, base, tw, L, B, lb, Bv, g.estride, g.n_out)));
+}
+
 // ---------------------------------------------------------------- spectral vote kernels
 struct SpecGeom {
     int n0, n1, n2;       // field (n2 fastest)
@@ -326,53 +467,62 @@ struct SpecGeom {
     size_t ocs;           // kernel-spectrum values per channel (octant)
 };
 
-__device__ __forceinline__ float ipow4(float x, int a) {
-    const float x2 = x * x;
-    return a == 0 ? 1.0f : a == 1 ? x : a == 2 ? x2 : a == 3 ? x2 * x : x2 * x2;
+// The 15 quartic monomials of (x, y, z) in the order of mono_a / mono_b,
+// times the scale w.
+__device__ __forceinline__ void mono15(const float4 P, float* __restrict__ A) {
+    const float x = P.x, y = P.y, z = P.z;
+    const float sx = P.w * x, sy = P.w * y, sz = P.w * z;
+    const float x2 = x * x, y2 = y * y, z2 = z * z, xy = x * y, xz = x * z, yz = y * z;
+    A[0] = sx * x * x2;   A[1] = sx * x * xy;   A[2] = sx * x * xz;
+    A[3] = sx * x * y2;   A[4] = sx * x * yz;   A[5] = sx * x * z2;
+    A[6] = sx * y * y2;   A[7] = sx * y * yz;   A[8] = sx * y * z2;   A[9] = sx * z * z2;
+    A[10] = sy * y * y2;  A[11] = sy * y * yz;  A[12] = sy * y * z2;  A[13] = sy * z * z2;
+    A[14] = sz * z * z2;
 }
 
-// Feature f of a voxel: scale * monomial of e2 (stick, f < nst) or of e0.
-__device__ __forceinline__ float feature(int f, int nst, const float4* __restrict__ ps,
-                                         const float4* __restrict__ pp, size_t v) {
-    const float4 P = f < nst ? ps[v] : pp[v];
-    const int m = f < nst ? f : f - nst;
-    return P.w * ipow4(P.x, mono_a(m)) * ipow4(P.y, mono_b(m)) * ipow4(P.z, mono_c(m));
-}
-
-// Forward x pass fused with the features: rows (z, y) of the field, complex
-// channel j = features (2j, 2j+1); block = (row batch, channel), channel fastest
-// so the blocks reading the same payload rows run together.
+// Forward x pass fused with the features. A block takes B rows (z, y) of the
+// field and the complex channels in groups of C (C B L2 <= kFftBL): every
+// element computes all features of its voxel from one payload read, channel
+// j = features (2j, 2j+1) as (real, imaginary), lines (channel, row) are
+// transformed together. Groups after the first re-read the payload from cache.
 __global__ __launch_bounds__(kFT) void k_spec_xfwd(const float4* __restrict__ ps, const float4* __restrict__ pp,
-                                                   SpecGeom g, int B, FftPlan pl, const float2* __restrict__ twg,
-                                                   float2* __restrict__ Z) {
+                                                   SpecGeom g, int B, int C, FftPlan pl,
+                                                   const float2* __restrict__ twg, float2* __restrict__ Z) {
     __shared__ float2 s[kFftBL];
     __shared__ float2 tw[kMaxL];
     const int L = pl.L;
-    const int j = (int)(blockIdx.x % (unsigned)g.nz);
-    const int row0 = (int)(blockIdx.x / (unsigned)g.nz) * B;
+    const int row0 = (int)blockIdx.x * B;
     const int nrows = g.n0 * g.n1;
     if (row0 >= nrows) return;
     const int Bv = min(B, nrows - row0);
     load_twiddles(tw, twg, L);
-    const int f0 = 2 * j, f1 = 2 * j + 1;
-    for (int idx = (int)threadIdx.x; idx < Bv * L; idx += kFT) {
-        const int l = idx / L, e = idx - l * L;
-        float2 v = make_float2(0.0f, 0.0f);
-        if (e < g.n2) {
-            const size_t vox = (size_t)(row0 + l) * g.n2 + e;
-            v.x = feature(f0, g.nst, ps, pp, vox);
-            if (f1 < g.nf) v.y = feature(f1, g.nst, ps, pp, vox);
+    const bool two = g.nf == 2 * kMono;  // stick and plate
+    for (int j0 = 0; j0 < g.nz; j0 += C) {
+        const int Cv = min(C, g.nz - j0);
+        if (j0) __syncthreads();  // the previous group's stores have read s
+        for (int idx = (int)threadIdx.x; idx < Bv * L; idx += kFT) {
+            const int b = idx / L, e = idx - b * L;
+            float A[2 * kMono];
+#pragma unroll
+            for (int f = 0; f < 2 * kMono; f++) A[f] = 0.0f;
+            if (e < g.n2) {
+                const size_t vox = (size_t)(row0 + b) * g.n2 + e;
+                mono15(g.nst ? ps[vox] : pp[vox], A);
+                if (two) mono15(pp[vox], A + kMono);
+            }
+#pragma unroll
+            for (int j = 0; j < kMono; j++)
+                if (j >= j0 && j < j0 + Cv) s[((j - j0) * Bv + b) * L + e] = make_float2(A[2 * j], A[2 * j + 1]);
         }
-        s[l * L + e] = v;
-    }
-    __syncthreads();
-    fft_lines<false>(s, pl, Bv, tw);
-    float2* zc = Z + (size_t)j * g.cs;
-    for (int idx = (int)threadIdx.x; idx < Bv * L; idx += kFT) {
-        const int l = idx / L, e = idx - l * L;
-        const int row = row0 + l;
-        const int z = row / g.n1, y = row - z * g.n1;
-        zc[((size_t)z * g.L1 + y) * g.L2 + e] = s[l * L + e];
+        __syncthreads();
+        fft_lines<false>(s, pl, Cv * Bv, tw);
+        for (int idx = (int)threadIdx.x; idx < Cv * Bv * L; idx += kFT) {
+            const int line = idx / L, e = idx - line * L;
+            const int c = line / Bv, b = line - c * Bv;
+            const int row = row0 + b;
+            const int z = row / g.n1, y = row - z * g.n1;
+            Z[(size_t)(j0 + c) * g.cs + ((size_t)z * g.L1 + y) * g.L2 + e] = s[line * L + e];
+        }
     }
 }
 
@@ -844,10 +994,10 @@ static int launch_lines(float2* data, const LineGeom& lg, const FftPlan& pl, con
     const unsigned grid = (unsigned)nblk;
     if (inv) {
         if (rows) k_fft_lines<true, true><<<grid, kFT, 0, st>>>(data, lg, pl, tw);
-        else k_fft_lines<true, false><<<grid, kFT, 0, st>>>(data, lg, pl, tw);
+        else k_fft_cols<true><<<grid, kFT, 0, st>>>(data, lg, pl, tw);
     } else {
         if (rows) k_fft_lines<false, true><<<grid, kFT, 0, st>>>(data, lg, pl, tw);
-        else k_fft_lines<false, false><<<grid, kFT, 0, st>>>(data, lg, pl, tw);
+        else k_fft_cols<false><<<grid, kFT, 0, st>>>(data, lg, pl, tw);
     }
     TH_HIP(hipGetLastError());
     return TIRA_OK;
@@ -865,7 +1015,10 @@ static LineGeom strided_geom(size_t cs, int nchan, int L, size_t estride, int na
     g.na = na;
     g.nb = nb;
     g.nchan = nchan;
-    g.B = std::max(1, std::min(na, kFftBL / L));
+    int B = 1, lb = 0;
+    while (2 * B * L <= kFftBL && 2 * B <= 32) { B *= 2; lb++; }
+    g.B = B;
+    g.lb = lb;
     g.nab = (int)ceil_div((size_t)na, (size_t)g.B);
     g.n_in = n_in;
     g.n_out = n_out;
@@ -938,11 +1091,12 @@ int vote3_spectral_run(const float* tensors, int in_layout, const float* lam, co
     const size_t plane = (size_t)L1 * L2;
     {
         ProfScope prof("spec_fwd_x", st);
-        const int B = std::max(1, kFftBL / g.L2);
+        const int C = std::min(g.nz, std::max(1, kFftBL / g.L2));
+        const int B = std::max(1, kFftBL / (C * g.L2));
         const size_t nrows = (size_t)s0 * s1;
-        const size_t nblk = ceil_div(nrows, (size_t)B) * (size_t)g.nz;
+        const size_t nblk = ceil_div(nrows, (size_t)B);
         TH_ARG(nblk < (1ull << 31), "spectral vote: volume too large");
-        k_spec_xfwd<<<(unsigned)nblk, kFT, 0, st>>>(ps, pp, g, B, p2, tw2, Z);
+        k_spec_xfwd<<<(unsigned)nblk, kFT, 0, st>>>(ps, pp, g, B, C, p2, tw2, Z);
         TH_HIP(hipGetLastError());
     }
     {
