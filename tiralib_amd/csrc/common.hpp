@@ -77,4 +77,16 @@ int launch_fma_loop(float* out, unsigned blocks, int iters, hipStream_t st);
 
 inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
+// XCD-major block numbering: hardware hands consecutive workgroup ids to the
+// 8 XCDs round-robin; remap so each XCD walks a contiguous run of tiles.
+__device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nblocks) {
+    constexpr unsigned kXcd = 8;
+    const unsigned per = (nblocks + kXcd - 1) / kXcd;
+    const unsigned xcd = b % kXcd, idx = b / kXcd;
+    const unsigned full = nblocks % kXcd == 0 ? kXcd : nblocks % kXcd;  // XCDs that own `per` blocks
+    const unsigned start = xcd < full ? xcd * per : full * per + (xcd - full) * (per - 1);
+    return start + idx;
+}
+
+
 }  // namespace th

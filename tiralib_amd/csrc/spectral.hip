@@ -50,6 +50,7 @@ constexpr int kFT = 256;       // threads per FFT block
 #define TH_FFT_BL 4096
 #endif
 constexpr int kFftBL = TH_FFT_BL;  // complex values staged per block (32 KB of LDS)
+
 constexpr int kMaxL = 1024;    // longest transform
 constexpr int kMaxStages = 16;
 constexpr int kMono = 15;      // quartic monomials
@@ -213,10 +214,10 @@ __device__ __forceinline__ int fdiv(int n, int d, float rd) {
 // reads its butterflies into registers before any write.
 // Each radix is its own function: inlined together, the six stage bodies
 // shared one register allocation of more than twice any single one.
-template <int R, bool INV>
+template <int R, bool INV, int BL = kFftBL>
 __device__ __noinline__ void fft_stage(lds_f2* __restrict__ s, int L, int nlines, int p,
                                        const lds_f2* __restrict__ tw) {
-    constexpr int MAXQ = (kFftBL / R + kFT - 1) / kFT;
+    constexpr int MAXQ = (BL / R + kFT - 1) / kFT;
     const int m = L / R;
     const int nb = nlines * m;
     const int step = L / (p * R);
@@ -336,11 +337,11 @@ __global__ __launch_bounds__(kFT) void k_fft_lines(float2* __restrict__ data, Li
 // (p = 1: no twiddles), the last one writes its outputs straight back
 // (p = L / R: output r of butterfly i is element i + r L/R), so the lines
 // touch LDS only between stages.
-template <int R, bool INV>
+template <int R, bool INV, int BL>
 __device__ __forceinline__ void cols_first(const float2* __restrict__ base, lds_f2* __restrict__ s, int L, int B,
                                            int lb, int Bv, size_t estride, int n_in) {
     // every load of the thread is issued before the first butterfly
-    constexpr int MAXQ = (kFftBL / R + kFT - 1) / kFT;
+    constexpr int MAXQ = (BL / R + kFT - 1) / kFT;
     const int m = L / R;
     const int nb = B * m;
     float2 u[MAXQ][R];
@@ -423,13 +424,15 @@ __device__ __forceinline__ void cols_single(float2* __restrict__ base, int B, in
         default: { constexpr int RR = 8; CALL; } break;    \
     }
 
-template <bool INV>
+template <bool INV, int BL>
 __global__ __launch_bounds__(kFT) void k_fft_cols(float2* __restrict__ data, LineGeom g, FftPlan pl,
                                                   const float2* __restrict__ twg) {
-    __shared__ float2 s_[kFftBL];
+    __shared__ float2 s_[BL];
     __shared__ float2 tw_[kMaxL];
     const int L = pl.L;
-    unsigned bid = blockIdx.x;
+    // XCD-major order: neighbouring column batches (which share 128-byte
+    // lines when B < 16) run on one XCD and meet in its L2
+    unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
     const int ab = (int)(bid % (unsigned)g.nab);
     bid /= (unsigned)g.nab;
     const int b = (int)(bid % (unsigned)g.nb);
@@ -446,12 +449,12 @@ __global__ __launch_bounds__(kFT) void k_fft_cols(float2* __restrict__ data, Lin
     load_twiddles(tw_, twg, L);
     lds_f2* s = to_lds(s_);
     const lds_f2* tw = to_lds(tw_);
-    TH_RADIX_SWITCH(pl.rad[0], (cols_first<RR, INV>(base, s, L, B, lb, Bv, g.estride, g.n_in)));
+    TH_RADIX_SWITCH(pl.rad[0], (cols_first<RR, INV, BL>(base, s, L, B, lb, Bv, g.estride, g.n_in)));
     __syncthreads();
     int p = pl.rad[0];
     for (int st = 1; st < pl.nst - 1; st++) {
         const int R = pl.rad[st];
-        TH_RADIX_SWITCH(R, (fft_stage<RR, INV>(s, L, B, p, tw)));
+        TH_RADIX_SWITCH(R, (fft_stage<RR, INV, BL>(s, L, B, p, tw)));
         p *= R;
     }
     TH_RADIX_SWITCH(pl.rad[pl.nst - 1], (cols_last<RR, INV>(s, base, tw, L, B, lb, Bv, g.estride, g.n_out)));
@@ -463,6 +466,7 @@ struct SpecGeom {
     int L0, L1, L2;       // FFT lengths
     int h0, h1, h2;       // octant extents L/2 + 1
     int nf, nz, nst;      // real features, complex channels, stick features (15 or 0)
+    int P2;               // row stride in complex values (L2 rounded up to 16)
     size_t cs;            // complex values per channel volume
     size_t ocs;           // kernel-spectrum values per channel (octant)
 };
@@ -521,7 +525,7 @@ __global__ __launch_bounds__(kFT) void k_spec_xfwd(const float4* __restrict__ ps
             const int c = line / Bv, b = line - c * Bv;
             const int row = row0 + b;
             const int z = row / g.n1, y = row - z * g.n1;
-            Z[(size_t)(j0 + c) * g.cs + ((size_t)z * g.L1 + y) * g.L2 + e] = s[line * L + e];
+            Z[(size_t)(j0 + c) * g.cs + ((size_t)z * g.L1 + y) * g.P2 + e] = s[line * L + e];
         }
     }
 }
@@ -548,7 +552,7 @@ __global__ __launch_bounds__(256) void k_spec_mul(float2* __restrict__ Z, const 
     size_t off[8];
 #pragma unroll
     for (int sl = 0; sl < 8; sl++)
-        off[sl] = ((size_t)fz[(sl >> 2) & 1] * g.L1 + fy[(sl >> 1) & 1]) * g.L2 + fx[sl & 1];
+        off[sl] = ((size_t)fz[(sl >> 2) & 1] * g.L1 + fy[(sl >> 1) & 1]) * g.P2 + fx[sl & 1];
     float2 Y[4][6];
 #pragma unroll
     for (int sl = 0; sl < 4; sl++)
@@ -615,7 +619,7 @@ __global__ __launch_bounds__(kFT) void k_spec_xinv(const float2* __restrict__ Z,
         const int l = line / 3, i = line - 3 * l;
         const int row = row0 + l;
         const int z = row / g.n1, y = row - z * g.n1;
-        s[line * L + e] = Z[(size_t)i * g.cs + ((size_t)z * g.L1 + y) * g.L2 + e];
+        s[line * L + e] = Z[(size_t)i * g.cs + ((size_t)z * g.L1 + y) * g.P2 + e];
     }
     __syncthreads();
     fft_lines<true>(s, pl, 3 * Bv, tw);
@@ -994,10 +998,10 @@ static int launch_lines(float2* data, const LineGeom& lg, const FftPlan& pl, con
     const unsigned grid = (unsigned)nblk;
     if (inv) {
         if (rows) k_fft_lines<true, true><<<grid, kFT, 0, st>>>(data, lg, pl, tw);
-        else k_fft_cols<true><<<grid, kFT, 0, st>>>(data, lg, pl, tw);
+        else k_fft_cols<true, kFftBL><<<grid, kFT, 0, st>>>(data, lg, pl, tw);
     } else {
         if (rows) k_fft_lines<false, true><<<grid, kFT, 0, st>>>(data, lg, pl, tw);
-        else k_fft_cols<false><<<grid, kFT, 0, st>>>(data, lg, pl, tw);
+        else k_fft_cols<false, kFftBL><<<grid, kFT, 0, st>>>(data, lg, pl, tw);
     }
     TH_HIP(hipGetLastError());
     return TIRA_OK;
@@ -1070,7 +1074,8 @@ int vote3_spectral_run(const float* tensors, int in_layout, const float* lam, co
     g.nf = (stick ? kMono : 0) + (plate ? kMono : 0);
     g.nz = (g.nf + 1) / 2;
     g.nst = stick ? kMono : 0;
-    g.cs = (size_t)L0 * L1 * L2;
+    g.P2 = ((int)L2 + 15) & ~15;  // rows start on 128-byte lines
+    g.cs = (size_t)L0 * L1 * g.P2;
     g.ocs = (size_t)g.h0 * g.h1 * g.h2;
 
     const size_t nvox = (size_t)s0 * s1 * s2;
@@ -1088,7 +1093,7 @@ int vote3_spectral_run(const float* tensors, int in_layout, const float* lam, co
         ProfScope prof("vote3_prep", st);
         TH_TRY(vote3_payload_flat(tensors, in_layout, lam, ev2, ev0, s0, s1, s2, ps, pp, st));
     }
-    const size_t plane = (size_t)L1 * L2;
+    const size_t row = (size_t)g.P2, plane = (size_t)L1 * row;
     {
         ProfScope prof("spec_fwd_x", st);
         const int C = std::min(g.nz, std::max(1, kFftBL / g.L2));
@@ -1101,12 +1106,12 @@ int vote3_spectral_run(const float* tensors, int in_layout, const float* lam, co
     }
     {
         ProfScope prof("spec_fwd_y", st);
-        TH_TRY(launch_lines(Z, strided_geom(g.cs, g.nz, g.L1, L2, g.L2, g.n0, plane, g.n1, g.L1), p1, tw1, false,
+        TH_TRY(launch_lines(Z, strided_geom(g.cs, g.nz, g.L1, row, g.L2, g.n0, plane, g.n1, g.L1), p1, tw1, false,
                             false, st));
     }
     {
         ProfScope prof("spec_fwd_z", st);
-        TH_TRY(launch_lines(Z, strided_geom(g.cs, g.nz, g.L0, plane, g.L2, g.L1, L2, g.n0, g.L0), p0, tw0, false,
+        TH_TRY(launch_lines(Z, strided_geom(g.cs, g.nz, g.L0, plane, g.L2, g.L1, row, g.n0, g.L0), p0, tw0, false,
                             false, st));
     }
     {
@@ -1116,12 +1121,12 @@ int vote3_spectral_run(const float* tensors, int in_layout, const float* lam, co
     }
     {
         ProfScope prof("spec_inv_z", st);
-        TH_TRY(launch_lines(Z, strided_geom(g.cs, 3, g.L0, plane, g.L2, g.L1, L2, g.L0, g.n0), p0, tw0, true, false,
+        TH_TRY(launch_lines(Z, strided_geom(g.cs, 3, g.L0, plane, g.L2, g.L1, row, g.L0, g.n0), p0, tw0, true, false,
                             st));
     }
     {
         ProfScope prof("spec_inv_y", st);
-        TH_TRY(launch_lines(Z, strided_geom(g.cs, 3, g.L1, L2, g.L2, g.n0, plane, g.L1, g.n1), p1, tw1, true, false,
+        TH_TRY(launch_lines(Z, strided_geom(g.cs, 3, g.L1, row, g.L2, g.n0, plane, g.L1, g.n1), p1, tw1, true, false,
                             st));
     }
     {

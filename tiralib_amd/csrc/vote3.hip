@@ -319,17 +319,6 @@ __device__ __forceinline__ void write_out(float* __restrict__ out, int layout, s
     }
 }
 
-// XCD-major block numbering: hardware hands consecutive workgroup ids to the
-// 8 XCDs round-robin; remap so each XCD walks a contiguous run of tiles.
-__device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nblocks) {
-    constexpr unsigned kXcd = 8;
-    const unsigned per = (nblocks + kXcd - 1) / kXcd;
-    const unsigned xcd = b % kXcd, idx = b / kXcd;
-    const unsigned full = nblocks % kXcd == 0 ? kXcd : nblocks % kXcd;  // XCDs that own `per` blocks
-    const unsigned start = xcd < full ? xcd * per : full * per + (xcd - full) * (per - 1);
-    return start + idx;
-}
-
 __device__ __forceinline__ void tile_coords(const VoteGeom& g, int bz_tile, unsigned& bx, unsigned& by,
                                             unsigned& bz) {
     const unsigned nb = g.nbx * g.nby * g.nbz;

@@ -5,14 +5,16 @@
 
 Runs a 1 GiB plain copy (the counter calibration reference: 1 GiB read,
 1 GiB written) three times, then `steps` iterations of: the 3D vote workload
-of bench.py (128^3 tubes + sheets, sigma 2, w 13, N 20) and the analytical
-vote (ATV) of the same volume; the batched eig3 +
+of bench.py (128^3 tubes + sheets, sigma 2, w 13, N 20) with the spectral
+vote and with the direct gather, and the analytical vote (ATV) of the same
+volume; the spectral vote of the 256^3 curved-sheet volume (sigma 4, w 25); the batched eig3 +
 Gaussian launch and the one-launch eig3 of its 48 volumes; the learned plate
 vote on a 96^3 curved-sheet volume (sigma 4, w 25; untrained weights with
 the error check off: the kernel's work does not depend on the weights, and
 training's ~10^5 small dispatches are left out of counter runs);
 and the full pipeline (structure tensor, Gaussian sigma 2, vote sigma 4,
-eig3 saliency) on a 192 x 256 x 256 vascular-tree volume. Prints one line per
+eig3 saliency) on a 192 x 256 x 256 vascular-tree volume, spectral then
+gather. Prints one line per
 stage.
 """
 from __future__ import annotations
@@ -48,12 +50,25 @@ def main(argv=None):
     field = DeviceArray.from_host(synth.tubes_sheets_volume(shape, seed=synth.SEED_CFG2))
     n = int(np.prod(shape))
     vote, ev = DeviceArray((n, 6)), DeviceArray((n, 3))
-    for _ in range(a.steps):
-        call("tira_tensorvote3_async", field.ptr, _lib.LAYOUT_MAT9, vote.ptr, _lib.LAYOUT_PACKED6, *shape, 2.0, 0.0,
-             13, 1, 1, 1, 20, _lib.PLATE_AUTO)
-        call("tira_eig3_async", vote.ptr, n, _lib.LAYOUT_PACKED6, ev.ptr, None, None, _lib.EIG_SCALE_INVARIANT, None)
-    call("tira_hip_synchronize")
-    print(f"vote3: {a.steps} steps of 128^3 sigma 2 w 13 N 20", flush=True)
+    for mode in (_lib.PLATE_AUTO, _lib.PLATE_CLOSED):  # the spectral vote, then the direct gather
+        for _ in range(a.steps):
+            call("tira_tensorvote3_async", field.ptr, _lib.LAYOUT_MAT9, vote.ptr, _lib.LAYOUT_PACKED6, *shape, 2.0,
+                 0.0, 13, 1, 1, 1, 20, mode)
+            call("tira_eig3_async", vote.ptr, n, _lib.LAYOUT_PACKED6, ev.ptr, None, None, _lib.EIG_SCALE_INVARIANT,
+                 None)
+        call("tira_hip_synchronize")
+    print(f"vote3: {a.steps} steps of 128^3 sigma 2 w 13 N 20, spectral then gather", flush=True)
+    if not a.skip_mlp:
+        sshape = (256, 256, 256)
+        svol = synth.curved_sheets_volume_device(sshape, 12, 0.02, seed=synth.SEED_CFG4)
+        sout = DeviceArray((int(np.prod(sshape)), 6))
+        for _ in range(a.steps):
+            call("tira_tensorvote3_async", svol.ptr, _lib.LAYOUT_PACKED6, sout.ptr, _lib.LAYOUT_PACKED6, *sshape,
+                 4.0, 0.0, 25, 1, 1, 1, 20, _lib.VOTE_SPECTRAL)
+        call("tira_hip_synchronize")
+        svol.free()
+        sout.free()
+        print(f"spectral: {a.steps} votes of the 256^3 curved-sheet volume, sigma 4, w 25", flush=True)
     for _ in range(a.steps):
         call("tira_tensorvote3_atv_async", field.ptr, _lib.LAYOUT_MAT9, vote.ptr, _lib.LAYOUT_PACKED6, *shape, 2.0,
              13)
@@ -94,9 +109,10 @@ def main(argv=None):
         pshape = (192, 256, 256)
         pvol, _ = synth.vessel_volume_device(pshape, seed=synth.SEED_CFG5, noise_sigma=0.1)
         sal = DeviceArray(pshape + (3,))
-        for _ in range(a.steps):
-            api.pipeline3(pvol, smooth_sigma=2.0, vote_sigma=4.0, w=25, plate_mode=_lib.PLATE_CLOSED, saliency=sal)
-        call("tira_hip_synchronize")
+        for mode in (_lib.VOTE_SPECTRAL, _lib.PLATE_CLOSED):
+            for _ in range(a.steps):
+                api.pipeline3(pvol, smooth_sigma=2.0, vote_sigma=4.0, w=25, plate_mode=mode, saliency=sal)
+            call("tira_hip_synchronize")
         print(f"pipeline: {a.steps} runs on 192 x 256 x 256", flush=True)
 
 
