@@ -46,7 +46,7 @@ int use_device(int device);
 enum Slot {
     WS_IN0 = 0, WS_IN1, WS_IN2, WS_OUT0, WS_OUT1,
     WS_PAY0, WS_PAY1, WS_TABLE, WS_TABLE2, WS_SCRATCH0, WS_SCRATCH1, WS_SCRATCH2, WS_COUNTER,
-    WS_TABLE3, WS_PAY2, WS_PIPE0, WS_PIPE1, WS_SPEC0, WS_SPEC1,
+    WS_TABLE3, WS_PAY2, WS_PIPE0, WS_PIPE1, WS_SPEC0, WS_SPEC1, WS_THIN0, WS_THIN1,
     WS_NSLOTS
 };
 int ws_get(int slot, size_t bytes, void** p);

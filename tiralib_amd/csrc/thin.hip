@@ -178,6 +178,62 @@ __global__ __launch_bounds__(256) void k_thin_candidates(const unsigned char* __
     }
 }
 
+// Object voxels (value 1) of the volume, or of a previous list, appended to
+// `list`; every lane of a wave runs the same number of iterations.
+__global__ __launch_bounds__(256) void k_thin_objects(const unsigned char* __restrict__ v, size_t n,
+                                                      const unsigned* __restrict__ from, unsigned nfrom,
+                                                      unsigned* __restrict__ list, unsigned* __restrict__ count) {
+    const size_t total = from ? (size_t)nfrom : n;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t iters = (total + stride - 1) / stride;
+    for (size_t it = 0; it < iters; it++) {
+        const size_t k = it * stride + (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+        unsigned i = 0;
+        bool take = false;
+        if (k < total) {
+            i = from ? from[k] : (unsigned)k;
+            take = v[i] == 1;
+        }
+        if (list) wave_append(take, i, list, count);
+        else wave_count(take, count);
+    }
+}
+
+// Candidates of border direction `dir` among the listed object voxels (the
+// same tests as k_thin_candidates, which scans the whole volume).
+__global__ __launch_bounds__(256) void k_thin_candidates_list(const unsigned char* __restrict__ v, int s0, int s1,
+                                                              int s2, int dir, const unsigned* __restrict__ obj,
+                                                              unsigned nobj, unsigned char* __restrict__ mark,
+                                                              unsigned* __restrict__ list,
+                                                              unsigned* __restrict__ count) {
+    const int ddx = dir == 2 ? 1 : dir == 3 ? -1 : 0;
+    const int ddy = dir == 0 ? -1 : dir == 1 ? 1 : 0;
+    const int ddz = dir == 4 ? 1 : dir == 5 ? -1 : 0;
+    const unsigned stride = gridDim.x * blockDim.x;
+    const unsigned iters = (nobj + stride - 1) / stride;
+    for (unsigned it = 0; it < iters; it++) {
+        const unsigned k = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
+        bool take = false;
+        unsigned i = 0;
+        if (k < nobj) {
+            i = obj[k];
+            if (v[i] == 1) {
+                const int x = (int)(i % (unsigned)s2), y = (int)((i / (unsigned)s2) % (unsigned)s1);
+                const int z = (int)(i / ((unsigned)s1 * (unsigned)s2));
+                const int X = x + ddx, Y = y + ddy, Z = z + ddz;
+                const bool border = (unsigned)X >= (unsigned)s2 || (unsigned)Y >= (unsigned)s1 ||
+                                    (unsigned)Z >= (unsigned)s0 || v[((size_t)Z * s1 + Y) * s2 + X] == 0;
+                if (border) {
+                    const unsigned code = thin_code(v, s0, s1, s2, x, y, z);
+                    take = !thin_endpoint(code) && thin_euler_invariant(code) && thin_simple(code);
+                }
+            }
+        }
+        if (take) mark[i] = 1;
+        wave_append(take, i, list, count);
+    }
+}
+
 // One Jacobi sweep of the sequential re-check (lee_thinning.h:651-661):
 // candidate p is deleted (mark 1) when it is simple in the volume where the
 // candidates before it in raster order (neighbour bits 0..12) that are
@@ -378,27 +434,48 @@ int otsu_run(const float* v, size_t n, int stride, float* thr, hipStream_t st) {
     return TIRA_OK;
 }
 
-// Lee thinning of the 0/1 byte volume `v` (device) in place; blocking.
+// Lee thinning of the 0/1 byte volume `v` (device) in place; blocking. The
+// object voxels are compacted into a list once and the list is refiltered
+// after every outer pass, so each border direction examines the remaining
+// object voxels only instead of the whole volume.
 int lee_run(unsigned char* v, unsigned s0, unsigned s1, unsigned s2, unsigned* iterations, hipStream_t st) {
     const size_t n = (size_t)s0 * s1 * s2;
     unsigned char *ma = nullptr, *mb = nullptr;
-    unsigned *list = nullptr, *cnt = nullptr;
+    unsigned *list = nullptr, *cnt = nullptr, *objA = nullptr, *objB = nullptr;
     TH_TRY(ws_get(WS_SCRATCH0, n, (void**)&ma));
     TH_TRY(ws_get(WS_SCRATCH1, n, (void**)&mb));
-    TH_TRY(ws_get(WS_SCRATCH2, n * 4, (void**)&list));
     TH_TRY(ws_get(WS_COUNTER, 64, (void**)&cnt));
     k_fill_u8<<<grid_for(n), 256, 0, st>>>(ma, n);
     k_fill_u8<<<grid_for(n), 256, 0, st>>>(mb, n);
     TH_HIP(hipGetLastError());
     unsigned host[4];
-    int unchanged = 0, iters = 0;
     ProfScope prof("lee_thin", st);
+    // object list: count, size the buffers, gather
+    k_thin_zero<<<1, 64, 0, st>>>(cnt, 4);
+    k_thin_objects<<<grid_for(n), 256, 0, st>>>(v, n, nullptr, 0, nullptr, cnt + 3);
+    TH_HIP(hipGetLastError());
+    TH_HIP(hipMemcpyAsync(host + 3, cnt + 3, 4, hipMemcpyDeviceToHost, st));
+    TH_HIP(hipStreamSynchronize(st));
+    unsigned nobj = host[3];
+    if (nobj == 0) {
+        if (iterations) *iterations = 1;  // one outer pass finds nothing to delete
+        return TIRA_OK;
+    }
+    TH_TRY(ws_get(WS_SCRATCH2, (size_t)nobj * 4, (void**)&list));
+    TH_TRY(ws_get(WS_THIN0, (size_t)nobj * 4, (void**)&objA));
+    TH_TRY(ws_get(WS_THIN1, (size_t)nobj * 4, (void**)&objB));
+    k_thin_zero<<<1, 64, 0, st>>>(cnt + 3, 1);
+    k_thin_objects<<<grid_for(n), 256, 0, st>>>(v, n, nullptr, 0, objA, cnt + 3);
+    TH_HIP(hipGetLastError());
+    int unchanged = 0, iters = 0;
     while (unchanged < 6) {
         unchanged = 0;
         iters++;
+        bool deleted_any = false;
         for (int dir = 0; dir < 6; dir++) {
             k_thin_zero<<<1, 64, 0, st>>>(cnt, 3);
-            k_thin_candidates<<<grid_for(n), 256, 0, st>>>(v, (int)s0, (int)s1, (int)s2, dir, ma, list, cnt);
+            k_thin_candidates_list<<<grid_for(nobj), 256, 0, st>>>(v, (int)s0, (int)s1, (int)s2, dir, objA, nobj, ma,
+                                                                   list, cnt);
             TH_HIP(hipGetLastError());
             TH_HIP(hipMemcpyAsync(host, cnt, 4, hipMemcpyDeviceToHost, st));
             TH_HIP(hipStreamSynchronize(st));
@@ -421,6 +498,20 @@ int lee_run(unsigned char* v, unsigned s0, unsigned s1, unsigned s2, unsigned* i
             TH_HIP(hipMemcpyAsync(host + 2, cnt + 2, 4, hipMemcpyDeviceToHost, st));
             TH_HIP(hipStreamSynchronize(st));
             if (host[2] == 0) unchanged++;
+            else deleted_any = true;
+        }
+        if (deleted_any && unchanged < 6) {  // drop deleted voxels from the object list
+            k_thin_zero<<<1, 64, 0, st>>>(cnt + 3, 1);
+            k_thin_objects<<<grid_for(nobj), 256, 0, st>>>(v, n, objA, nobj, objB, cnt + 3);
+            TH_HIP(hipGetLastError());
+            TH_HIP(hipMemcpyAsync(host + 3, cnt + 3, 4, hipMemcpyDeviceToHost, st));
+            TH_HIP(hipStreamSynchronize(st));
+            nobj = host[3];
+            std::swap(objA, objB);
+            if (nobj == 0) {  // the next outer pass would find no candidate in any direction
+                iters++;
+                break;
+            }
         }
     }
     if (iterations) *iterations = (unsigned)iters;
