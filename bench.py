@@ -500,6 +500,54 @@ def bench_skeleton(sal, shape, st):
             "outer_iterations": int(it.value), "mvoxel_s": round(n / (ms * 1e-3) / 1e6, 1)}
 
 
+def bench_vote2d(steps=20):
+    """Config 1: 256 x 256 field of 8 seeded cubic Bezier curves + 2% noise
+    sticks (seed 1001), stick voting sigma 3, w 19, then eigen2 saliency
+    l1 - l0. The device-resident step (tira_tensorvote2 + tira_evals2_symmetric
+    on device buffers; both calls block, as the reference's do), the
+    host-array step through the pybind-module mirror (PCIe copies included),
+    and the oracle's CPU restatement on one core; error against the FP64
+    oracle."""
+    from oracle import oracle as O
+    from tiralib_amd import _lib, synth, tensorvote_tk as tvk
+    from tiralib_amd._lib import DeviceArray, call
+    f = synth.bezier_field(seed=synth.SEED_CFG1)
+    H, W = f.shape[:2]
+    n = H * W
+    d_in, d_out, d_ev = DeviceArray.from_host(f), DeviceArray((H, W, 4)), DeviceArray((n, 2))
+    try:
+        def step():
+            call("tira_tensorvote2", d_out.ptr, d_in.ptr, 3.0, 0.0, 1, 19, H, W, 1, 0, 20, 0)
+            call("tira_evals2_symmetric", d_out.ptr, n, d_ev.ptr, 0)
+        for _ in range(3):
+            step()
+        t = time.perf_counter()
+        for _ in range(steps):
+            step()
+        dev_ms = (time.perf_counter() - t) / steps * 1e3
+        gpu = d_out.to_host()
+    finally:
+        d_in.free(); d_out.free(); d_ev.free()
+    t = time.perf_counter()
+    for _ in range(steps):
+        tvk.tensorvote_tk(f, 3.0, 0.0, 1, 19, True, False, 20)
+    host_ms = (time.perf_counter() - t) / steps * 1e3
+    t = time.perf_counter()
+    ref32 = O.tensorvote2(f, 3.0, 0.0, 1, 19, True, False, 20, nthreads=1)
+    O.evals2(ref32.reshape(-1, 4))
+    cpu_s = time.perf_counter() - t
+    ref64 = O.tensorvote2_f64(f, 3.0, 0.0, 1, 19, True, False, 20, nthreads=min(16, os.cpu_count() or 1))
+    err = float(np.linalg.norm(gpu.astype(np.float64) - ref64) / np.linalg.norm(ref64))
+    pairs = in_bounds_pairs((H, W), 19)
+    return {"workload": "2D stick voting, 256x256 Bezier field (8 curves, 2% noise sticks, seed 1001), sigma=3, "
+                        "w=19, then eigen2 saliency", "pixels": n, "pairs": pairs,
+            "device_ms_per_step": round(dev_ms, 4), "mpx_s": round(n / dev_ms / 1e3, 1),
+            "host_arrays_ms_per_step": round(host_ms, 4), "host_arrays_mpx_s": round(n / host_ms / 1e3, 1),
+            "cpu_restatement_1core_s": round(cpu_s, 3), "cpu_mpx_s": round(n / cpu_s / 1e6, 3),
+            "rel_frob_err_vs_fp64_oracle": err,
+            "note": "launch- and synchronisation-bound: 2.28e7 pairs are ~4 us of FP32 VALU at peak"}
+
+
 def bench_headline_check(d_in, d_vote, field, shape, sigma, w, samples, st, nrecv=600, steps=5):
     """The headline vote's error against the FP64 oracle on sampled receivers,
     and the direct closed-form gather (the per-pair FP32 VALU kernel) timed
@@ -782,6 +830,7 @@ def main(argv=None):
         "stages_ms": stages,
         "wall_s_timed": round(wall, 3),
     }
+    out["vote2d"] = bench_vote2d()
     out["accuracy_and_gather"] = bench_headline_check(d_in, d_vote, field, shape, sigma, w, samples, st)
     out["calibration"] = calibrate()
     if not a.no_batch:

@@ -163,3 +163,21 @@ def test_spectral_config2_against_gather(hip, oracle):
     e64d = rel_frob(b.reshape(-1, 9)[recv][:, SLOTS], ref)
     print(f"128^3: spectral vs gather {e:.2e}; sampled vs FP64: spectral {e64:.2e}, gather {e64d:.2e}")
     assert e <= 1e-5 and e64 <= 1e-5
+
+
+def test_auto_selects_spectral_by_size(hip, oracle):
+    """TIRA_PLATE_AUTO takes the spectral vote from 32^3 voxels up (and the
+    reference-style host-pointer entry with it), the gather below; both agree
+    with the oracle."""
+    for shape, expect in (((32, 33, 34), "spectral"), ((20, 21, 22), "gather")):
+        f = _random_field(shape, seed=17)
+        call("tira_hip_profile_begin")
+        gpu = api.tensorvote3(f, 2.0, 0.0, 9, 1, samples=20)  # tira_tensorvote3, host arrays
+        call("tira_hip_profile_end")
+        n_spec = _lib.profile_get("spec_mul")[1]
+        n_gather = _lib.profile_get("vote3_closed")[1]
+        assert (n_spec, n_gather) == ((1, 0) if expect == "spectral" else (0, 1)), (shape, n_spec, n_gather)
+        ref = oracle.tensorvote3_f64(f, 2.0, 0.0, 1, 9, samples=20, nthreads=NT)
+        e = rel_frob(gpu.reshape(-1, 9)[:, SLOTS], ref.reshape(-1, 9)[:, SLOTS])
+        print(shape, expect, f"{e:.2e}")
+        assert e <= 1e-5

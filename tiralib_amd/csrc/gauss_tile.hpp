@@ -148,14 +148,6 @@ __device__ __forceinline__ void gauss3_tile(const float* __restrict__ in, float*
                         *reinterpret_cast<float4*>(s_xf + row * kGT + seg * 4) = make_float4(o4[0], o4[1], o4[2], o4[3]);
                     }
                 }
-                if (EIG) {
-                    // eigenvalues of this tile's own voxels of input plane iz0 + i
-                    // (same mode only: input and output grids coincide)
-                    const int t = i - R;
-                    if (tid < 256 && live && t >= 0 && t < nz)
-                        tile_eig3(s_raw, (R + ty) * RS + R + tx, PW * RS,
-                                  evals + (((size_t)(oz0 + t) * g.o1 + oy) * g.o2 + ox) * 3);
-                }
                 __syncthreads();
                 // y-pass for this thread's column, then the z ring
 #pragma unroll
@@ -180,6 +172,21 @@ __device__ __forceinline__ void gauss3_tile(const float* __restrict__ in, float*
                 }
 #pragma unroll
                 for (int c = 0; c < CP; c++) acc[(m + 1) % K][c] = 0.0f;
+            }
+        }
+    }
+    if (EIG) {
+        // Eigenvalues of the tile's own voxels (same mode only: input and
+        // output grids coincide), after the z loop so that the solver's
+        // registers do not add to the ring's: the tile's input columns were
+        // read moments ago and come from L2.
+        const int ntile = kGT * kGT * nz;
+        for (int k = tid; k < ntile; k += NT) {
+            const int t = k / (kGT * kGT), rem = k % (kGT * kGT);
+            const int ex = ox0 + rem % kGT, ey = oy0 + rem / kGT;
+            if (ex < g.o2 && ey < g.o1) {
+                const size_t v = ((size_t)(oz0 + t) * g.o1 + ey) * g.o2 + ex;
+                tile_eig3(in + v * 6, 0, 1, evals + v * 3);
             }
         }
     }
