@@ -52,15 +52,19 @@ def spectral_bytes(shape, w, in_bytes=36, out_bytes=24, nf=30):
     every array a pass reads or writes, once (zero padding that the passes
     skip is not counted). Features: nf real channels packed two per complex
     volume of FFT lengths L0 x L1 x L2 (8 B per complex value); kernel spectra
-    6 nf octant values (4 B); payload 32 B per voxel; output packed-6."""
+    6 nf octant values (4 B); payload 32 B per voxel (or the input tensors, in_bytes
+    per voxel, when the x pass decomposes them itself); output packed-6."""
     from tiralib_amd import api
     n0, n1, n2 = shape
     L0, L1, L2 = (api.spectral_length(n, w) for n in shape)
     h0, h1, h2 = L0 // 2 + 1, L1 // 2 + 1, L2 // 2 + 1
     nz, N, c = (nf + 1) // 2, n0 * n1 * n2, 8
+    # every channel in one x-pass group (15 x L2 <= 4096 complex values): the x pass
+    # eigen-decomposes the input tensors itself and no payload is written
+    fused = nz <= 4096 // L2
     return {
-        "vote3_prep": N * (in_bytes + 32),
-        "spec_fwd_x": N * 32 + nz * n0 * n1 * L2 * c,
+        "vote3_prep": 0 if fused else N * (in_bytes + 32),
+        "spec_fwd_x": N * (in_bytes if fused else 32) + nz * n0 * n1 * L2 * c,
         "spec_fwd_y": nz * n0 * n1 * L2 * c + nz * n0 * L1 * L2 * c,
         "spec_fwd_z": nz * n0 * L1 * L2 * c + nz * L0 * L1 * L2 * c,
         "spec_mul": nz * L0 * L1 * L2 * c + 6 * nf * h0 * h1 * h2 * 4 + 3 * L0 * L1 * L2 * c,

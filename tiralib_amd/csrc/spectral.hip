@@ -38,6 +38,7 @@
 // offset kernels) and cached per host thread and device.
 #include "common.hpp"
 #include "vote3.hpp"
+#include "voxel_payload.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -489,9 +490,14 @@ __device__ __forceinline__ void mono15(const float4 P, float* __restrict__ A) {
 // element computes all features of its voxel from one payload read, channel
 // j = features (2j, 2j+1) as (real, imaginary), lines (channel, row) are
 // transformed together. Groups after the first re-read the payload from cache.
+// FROM_TENSORS: each element eigen-decomposes its voxel of the input field
+// (layout) instead of reading the payload (one channel group only, so every
+// voxel is decomposed once).
+template <bool FROM_TENSORS>
 __global__ __launch_bounds__(kFT) void k_spec_xfwd(const float4* __restrict__ ps, const float4* __restrict__ pp,
-                                                   SpecGeom g, int B, int C, FftPlan pl,
-                                                   const float2* __restrict__ twg, float2* __restrict__ Z) {
+                                                   const float* __restrict__ tensors, int layout, SpecGeom g, int B,
+                                                   int C, FftPlan pl, const float2* __restrict__ twg,
+                                                   float2* __restrict__ Z) {
     __shared__ float2 s[kFftBL];
     __shared__ float2 tw[kMaxL];
     const int L = pl.L;
@@ -511,8 +517,17 @@ __global__ __launch_bounds__(kFT) void k_spec_xfwd(const float4* __restrict__ ps
             for (int f = 0; f < 2 * kMono; f++) A[f] = 0.0f;
             if (e < g.n2) {
                 const size_t vox = (size_t)(row0 + b) * g.n2 + e;
-                mono15(g.nst ? ps[vox] : pp[vox], A);
-                if (two) mono15(pp[vox], A + kMono);
+                float4 S, N;
+                if (FROM_TENSORS) {
+                    float l0, l1, l2, e0[3], e2[3];
+                    voxel_eig(tensors, layout, vox, l0, l1, l2, e0, e2);
+                    stick_plate_payload(l0, l1, l2, e0, e2, S, N);
+                } else {
+                    if (g.nst) S = ps[vox];
+                    if (g.nf > g.nst) N = pp[vox];
+                }
+                mono15(g.nst ? S : N, A);
+                if (two) mono15(N, A + kMono);
             }
 #pragma unroll
             for (int j = 0; j < kMono; j++)
@@ -1079,29 +1094,32 @@ int vote3_spectral_run(const float* tensors, int in_layout, const float* lam, co
     g.ocs = (size_t)g.h0 * g.h1 * g.h2;
 
     const size_t nvox = (size_t)s0 * s1 * s2;
+    const int C = std::min(g.nz, std::max(1, kFftBL / g.L2));
+    const int B = std::max(1, kFftBL / (C * g.L2));
+    // one channel group: the x pass decomposes the voxels itself (no payload round trip)
+    const bool fused = tensors && C == g.nz;
     float4 *ps = nullptr, *pp = nullptr;
     float2* Z = nullptr;
-    if (stick) TH_TRY(ws_get(WS_PAY0, nvox * 16, (void**)&ps));
-    if (plate) TH_TRY(ws_get(WS_PAY1, nvox * 16, (void**)&pp));
+    if (stick && !fused) TH_TRY(ws_get(WS_PAY0, nvox * 16, (void**)&ps));
+    if (plate && !fused) TH_TRY(ws_get(WS_PAY1, nvox * 16, (void**)&pp));
     TH_TRY(ws_get(WS_SPEC0, (size_t)g.nz * g.cs * 8, (void**)&Z));
     const SpecKey key{g.L0, g.L1, g.L2, (int)w, sigma1, sigma2, stick ? 1 : 0, plate ? 1 : 0};
     const float2 *tw0, *tw1, *tw2;
     const float* kh;
     TH_TRY(spectra_for(key, g, st, &tw0, &tw1, &tw2, &kh));
 
-    {
+    if (!fused) {
         ProfScope prof("vote3_prep", st);
         TH_TRY(vote3_payload_flat(tensors, in_layout, lam, ev2, ev0, s0, s1, s2, ps, pp, st));
     }
     const size_t row = (size_t)g.P2, plane = (size_t)L1 * row;
     {
         ProfScope prof("spec_fwd_x", st);
-        const int C = std::min(g.nz, std::max(1, kFftBL / g.L2));
-        const int B = std::max(1, kFftBL / (C * g.L2));
         const size_t nrows = (size_t)s0 * s1;
         const size_t nblk = ceil_div(nrows, (size_t)B);
         TH_ARG(nblk < (1ull << 31), "spectral vote: volume too large");
-        k_spec_xfwd<<<(unsigned)nblk, kFT, 0, st>>>(ps, pp, g, B, C, p2, tw2, Z);
+        if (fused) k_spec_xfwd<true><<<(unsigned)nblk, kFT, 0, st>>>(ps, pp, tensors, in_layout, g, B, C, p2, tw2, Z);
+        else k_spec_xfwd<false><<<(unsigned)nblk, kFT, 0, st>>>(ps, pp, tensors, in_layout, g, B, C, p2, tw2, Z);
         TH_HIP(hipGetLastError());
     }
     {

@@ -26,6 +26,7 @@
 #include "eig_device.hpp"
 #include "mlp.hpp"
 #include "vote3.hpp"
+#include "voxel_payload.hpp"
 
 #include <cmath>
 #include <vector>
@@ -428,41 +429,6 @@ __global__ __launch_bounds__(kThreads) void k_vote3_sum(const float4* __restrict
 }
 
 // ---------------------------------------------------------------- payload prep
-__device__ __forceinline__ void load_tensor(const float* __restrict__ in, int layout, size_t v, float& a, float& b,
-                                            float& c, float& d, float& e, float& f) {
-    if (layout == TIRA_LAYOUT_MAT9) {
-        const float* m = in + v * 9;
-        a = m[0]; b = m[1]; d = m[2]; c = m[4]; e = m[5]; f = m[8];
-    } else {
-        const float* m = in + v * 6;
-        a = m[0]; b = m[1]; d = m[2]; c = m[3]; e = m[4]; f = m[5];
-    }
-}
-
-// Eigen-decomposition of voxel v for the payload: power-of-two prescale
-// (exact; makes every threshold relative to |A|), FP32 deflated solve, FP64
-// re-solve when the relative eigengap is below 1e-4.
-__device__ __forceinline__ void voxel_eig(const float* __restrict__ in, int layout, size_t v, float& l0, float& l1,
-                                          float& l2, float e0[3], float e2[3]) {
-    float a, b, c, d, e, f;
-    load_tensor(in, layout, v, a, b, c, d, e, f);
-    const float m = fmaxf(fmaxf(fmaxf(fabsf(a), fabsf(b)), fmaxf(fabsf(c), fabsf(d))), fmaxf(fabsf(e), fabsf(f)));
-    int ex = 0;
-    if (m > 0.0f && isfinite(m)) {
-        frexpf(m, &ex);
-        a = ldexpf(a, -ex); b = ldexpf(b, -ex); c = ldexpf(c, -ex);
-        d = ldexpf(d, -ex); e = ldexpf(e, -ex); f = ldexpf(f, -ex);
-    }
-    float lam[3];
-    eig3_solve<float>(a, b, c, d, e, f, 1e-30f, lam, e0, e2);
-    if (eig3_near_degenerate(lam[0], lam[1], lam[2])) {
-        double dl[3], de0[3], de2[3];
-        eig3_solve<double>(a, b, c, d, e, f, 1e-30, dl, de0, de2);
-        for (int k = 0; k < 3; k++) { lam[k] = (float)dl[k]; e0[k] = (float)de0[k]; e2[k] = (float)de2[k]; }
-    }
-    l0 = ldexpf(lam[0], ex); l1 = ldexpf(lam[1], ex); l2 = ldexpf(lam[2], ex);
-}
-
 // Plate basis of tensorvote.h:1667-1673 / 2468-2470 for a unit normal n.
 __device__ __forceinline__ void plate_basis(const float n[3], float u[3], float v[3]) {
     float a0 = 0.0f, a1 = 0.0f, a2 = 1.0f;
@@ -482,21 +448,17 @@ __device__ __forceinline__ void plate_basis(const float n[3], float u[3], float 
 __device__ __forceinline__ void write_payload(const VoteGeom& g, size_t pidx, float l0, float l1, float l2,
                                               const float* e0, const float* e2, float4* pay_s, float4* pay_p,
                                               float4* pay_v, float sum_inv_n) {
-    if (pay_s) pay_s[pidx] = make_float4(e2[0], e2[1], e2[2], copysignf(fabsf(l2) - fabsf(l1), l2));
+    float4 S, N;
+    stick_plate_payload(l0, l1, l2, e0, e2, S, N);
+    if (pay_s) pay_s[pidx] = S;
     if (pay_p) {
-        // The reference skips plate voters whose normal has |n|^2 <= 1e-12 and
-        // normalises the rest (tensorvote.h:2464-2467).
-        float n[3] = {e0[0], e0[1], e0[2]};
-        const float nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
-        float sp = copysignf(fabsf(l1) - fabsf(l0), l1);
-        if (!(nn > 1e-12f)) { sp = 0.0f; n[0] = n[1] = n[2] = 0.0f; }
-        else { const float inv = rsqrtf(nn); n[0] *= inv; n[1] *= inv; n[2] *= inv; }
         if (!pay_v) {
-            pay_p[pidx] = make_float4(n[0], n[1], n[2], sp);
+            pay_p[pidx] = N;
         } else {
+            const float n[3] = {N.x, N.y, N.z};
             float u[3], v[3];
             plate_basis(n, u, v);
-            pay_p[pidx] = make_float4(u[0], u[1], u[2], sp * sum_inv_n);
+            pay_p[pidx] = make_float4(u[0], u[1], u[2], N.w * sum_inv_n);
             pay_v[pidx] = make_float4(v[0], v[1], v[2], 0.0f);
         }
     }
