@@ -74,9 +74,9 @@ def spectral_bytes(shape, w, in_bytes=36, out_bytes=24, nf=30):
     }
 
 
-def spectral_roofline(shape, w, stage_ms, in_bytes=36):
+def spectral_roofline(shape, w, stage_ms, in_bytes=36, nf=30):
     """Per-kernel achieved HBM bandwidth of the spectral vote and the dominant kernel."""
-    b = spectral_bytes(shape, w, in_bytes)
+    b = spectral_bytes(shape, w, in_bytes, nf=nf)
     per = {k: {"ms": round(stage_ms[k], 4), "bytes": int(b[k]),
                "gbs": round(b[k] / (stage_ms[k] * 1e-3) / 1e9, 1),
                "frac": round(b[k] / (stage_ms[k] * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
@@ -609,22 +609,31 @@ def bench_atv(d_in, field, shape, sigma, w, steps, warmup, cpu_seconds=None):
     d_out = DeviceArray((n, 6))
     st = Stream().use()
 
-    def step():
-        call("tira_tensorvote3_atv_async", d_in.ptr, _lib.LAYOUT_MAT9, d_out.ptr, _lib.LAYOUT_PACKED6, *shape,
-             sigma, w)
+    def run(method):
+        def step():
+            call("tira_tensorvote3_atv_method_async", d_in.ptr, _lib.LAYOUT_MAT9, d_out.ptr, _lib.LAYOUT_PACKED6,
+                 *shape, sigma, w, method)
+        for _ in range(warmup):
+            step()
+        st.synchronize()
+        e0, e1 = Event(), Event()
+        call("tira_hip_profile_begin")
+        e0.record(st)
+        for _ in range(steps):
+            step()
+        e1.record(st)
+        st.synchronize()
+        call("tira_hip_profile_end")
+        return e0.elapsed_ms(e1) / steps
 
-    for _ in range(warmup):
-        step()
-    st.synchronize()
-    e0, e1 = Event(), Event()
-    call("tira_hip_profile_begin")
-    e0.record(st)
-    for _ in range(steps):
-        step()
-    e1.record(st)
-    st.synchronize()
-    call("tira_hip_profile_end")
-    ms = e0.elapsed_ms(e1) / steps
+    # the default (TIRA_PLATE_AUTO): the spectral form at this size
+    ms = run(_lib.PLATE_AUTO)
+    stage_ms = {k: _lib.profile_get(k)[0] for k in SPECTRAL_KERNELS}
+    per, dom, tot = spectral_roofline(shape, w, stage_ms, in_bytes=36, nf=31)
+    spec_out = d_out.to_host()
+    # the per-pair gather
+    ms_g = run(_lib.PLATE_CLOSED)
+    gath_out = d_out.to_host()
     k_ms, k_cnt = _lib.profile_get("vote3_atv")
     p_ms, _ = _lib.profile_get("vote3_atv_prep")
     pairs = in_bounds_pairs(shape, w)
@@ -633,12 +642,18 @@ def bench_atv(d_in, field, shape, sigma, w, steps, warmup, cpu_seconds=None):
         "workload": f"3D ATV {shape[0]}x{shape[1]}x{shape[2]} (tubes+sheets, seed 1002), sigma={sigma}, w={w}, "
                     f"device-resident, packed-6 output",
         "value": round(n / (ms * 1e-3) / 1e6, 3), "unit": "Mvoxel/s", "ms_per_step": round(ms, 4),
-        "stages_ms": {"vote3_atv_prep": round(p_ms, 4), "vote3_atv": round(k_ms, 4)},
-        "roofline": {"kernel": "vote3_atv", "bound": "valu", "achieved": round(achieved, 3),
-                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                     "frac_of_unpacked_peak": round(achieved / PEAK_FP32_UNPACKED_TFLOPS, 4),
-                     "traffic": profiled_traffic("k_vote3_atv"), "flop_per_pair": ATV3_PAIR_FLOP,
-                     "avg_launch_ms": round(k_ms, 4), "launches": k_cnt},
+        "method": "spectral (31 features: eigen-subspace weights x quartic monomials, and l0)",
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": per[dom]["gbs"], "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": per[dom]["frac"], "all_kernels": tot, "per_kernel": per},
+        "spectral_vs_gather_rel_frob": _frob6(spec_out - gath_out) / _frob6(gath_out),
+        "gather": {"ms_per_step": round(ms_g, 4), "mvoxel_s": round(n / (ms_g * 1e-3) / 1e6, 3),
+                   "stages_ms": {"vote3_atv_prep": round(p_ms, 4), "vote3_atv": round(k_ms, 4)},
+                   "roofline": {"kernel": "vote3_atv", "bound": "valu", "achieved": round(achieved, 3),
+                                "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                                "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                                "frac_of_unpacked_peak": round(achieved / PEAK_FP32_UNPACKED_TFLOPS, 4),
+                                "traffic": profiled_traffic("k_vote3_atv"), "flop_per_pair": ATV3_PAIR_FLOP,
+                                "avg_launch_ms": round(k_ms, 4), "launches": k_cnt}},
     }
     d_out.free()
     # 2D: 256^2 mixed field as the reference's test builds it, window 2 int(3 sigma) + 1

@@ -193,9 +193,15 @@ int tira_tensorvote3_async(const float* input_field, int in_layout, float* outpu
 int tira_tensorvote3_atv(float* output_field, const float* lambdas, const float* evec1, const float* evec2,
                          unsigned s0, unsigned s1, unsigned s2, float sigma, unsigned w);
 /* Native 3D analytical tensor voting on device buffers, non-blocking on the
- * library stream (layouts TIRA_LAYOUT_*). */
+ * library stream (layouts TIRA_LAYOUT_*). The spectral form is taken when it
+ * applies (w >= 5, at least 32^3 voxels, x-rows whose FFT length is at most 256). */
 int tira_tensorvote3_atv_async(const float* input_field, int in_layout, float* output_field, int out_layout,
                                unsigned s0, unsigned s1, unsigned s2, float sigma, unsigned w);
+/* The same with the method named: TIRA_PLATE_AUTO, TIRA_PLATE_CLOSED (the
+ * per-pair gather) or TIRA_VOTE_SPECTRAL (the FFT form whenever the input
+ * allows it). No reference counterpart (the reference has one algorithm). */
+int tira_tensorvote3_atv_method_async(const float* input_field, int in_layout, float* output_field, int out_layout,
+                                      unsigned s0, unsigned s1, unsigned s2, float sigma, unsigned w, int method);
 
 /* Spectral vote internals, exported for tests (no reference counterpart).
  * Offset kernels K[ch][dw][dv][du] (ch = c * nf + f; c = xx, xy, xz, yy, yz,
@@ -207,6 +213,10 @@ int tira_tensorvote3_atv_async(const float* input_field, int in_layout, float* o
  * holds 6 * nf * (2(w/2)+1)^3 doubles; *nf_out receives nf. power must be 1. */
 int tira_spectral_offset_kernels(unsigned w, float sigma1, float sigma2, unsigned power, int stick, int plate,
                                  double* out, unsigned* nf_out);
+/* The same for analytical tensor voting (tensorvote.h:688-760, attenuation
+ * exp(-|d|^2 / sigma^2), half-window w / 2): nf = 31 features, the 15
+ * monomials of e2 times l2 - l1, of e0 times l1 - l0, and l0 itself. */
+int tira_spectral_offset_kernels_atv(unsigned w, float sigma, double* out, unsigned* nf_out);
 /* In-place 3D FFT of nchan complex volumes (interleaved float pairs, each
  * L0 x L1 x L2 with L2 fastest, volumes back to back); forward e^{-2 pi i k x / L},
  * inverse unnormalised. Every L must be <= 1024 with prime factors 2, 3, 5, 7.

@@ -181,3 +181,46 @@ def test_auto_selects_spectral_by_size(hip, oracle):
         e = rel_frob(gpu.reshape(-1, 9)[:, SLOTS], ref.reshape(-1, 9)[:, SLOTS])
         print(shape, expect, f"{e:.2e}")
         assert e <= 1e-5
+
+
+def _atv(field, shape, sigma, w, method, in_layout=_lib.LAYOUT_MAT9, out_layout=_lib.LAYOUT_MAT9):
+    n = int(np.prod(shape))
+    src = np.ascontiguousarray(field, np.float32).reshape(n, -1)
+    if in_layout == _lib.LAYOUT_PACKED6:
+        src = np.ascontiguousarray(src[:, SLOTS])
+    d_in, d_out = DeviceArray.from_host(src), DeviceArray((n, 9 if out_layout == _lib.LAYOUT_MAT9 else 6))
+    try:
+        call("tira_tensorvote3_atv_method_async", d_in.ptr, in_layout, d_out.ptr, out_layout, *shape, float(sigma),
+             int(w), int(method))
+        call("tira_hip_synchronize")
+        return d_out.to_host()
+    finally:
+        d_in.free()
+        d_out.free()
+
+
+@pytest.mark.parametrize("shape,sigma,w", [((20, 22, 24), 2.0, 13), ((17, 9, 31), 1.5, 7), ((3, 2, 40), 3.0, 15),
+                                           ((12, 13, 14), 1.0, 6)])
+def test_spectral_atv_matches_oracle_f64(hip, oracle, shape, sigma, w):
+    """Analytical tensor voting in the spectral form (31 features) against the
+    FP64 oracle (tira::atv restated, tensorvote.h:688-760, 1443-1495) and the
+    gather; the even w = 6 takes the reference's w / 2 half-window."""
+    f = _random_field(shape, seed=w + 3)
+    spec = _atv(f, shape, sigma, w, _lib.VOTE_SPECTRAL)
+    gath = _atv(f, shape, sigma, w, _lib.PLATE_CLOSED)
+    ref = oracle.tensorvote3_atv_f64(f, sigma, w, nthreads=NT).reshape(-1, 9)
+    e, eg = rel_frob(spec, ref), rel_frob(gath, ref)
+    print(f"ATV {shape} w={w}: spectral {e:.2e}, gather {eg:.2e}, spectral vs gather {rel_frob(spec, gath):.2e}")
+    assert e <= 1e-5 and rel_frob(spec, gath) <= 1e-5
+    s6 = _atv(f, shape, sigma, w, _lib.VOTE_SPECTRAL, in_layout=_lib.LAYOUT_PACKED6, out_layout=_lib.LAYOUT_PACKED6)
+    np.testing.assert_array_equal(s6, spec[:, SLOTS])
+
+
+def test_spectral_atv_auto_by_size(hip):
+    """The ATV entry takes the spectral form from 32^3 voxels up, the gather below."""
+    for shape, expect in (((32, 33, 34), (1, 0)), ((20, 21, 22), (0, 1))):
+        f = _random_field(shape, seed=23)
+        call("tira_hip_profile_begin")
+        api.tensorvote3(f, 2.0, 0.0, 9, 1, is_atv=True)
+        call("tira_hip_profile_end")
+        assert (_lib.profile_get("spec_mul")[1], _lib.profile_get("vote3_atv")[1]) == expect, shape

@@ -33,7 +33,7 @@ def _atten(l2, s):
     return np.exp(-l2 / s ** 2) if s > 0 else (0.0 if l2 > 0 else 1.0)
 
 
-@pytest.mark.parametrize("w,s1,s2", [(7, 1.5, 0.0), (9, 2.0, 1.2), (6, 1.0, 0.0), (5, 0.0, 0.0)])
+@pytest.mark.parametrize("w,s1,s2", [(7, 1.5, 0.0), (9, 2.0, 1.2), (6, 1.0, 0.0), (5, 0.4, 0.0)])
 def test_offset_kernels_reproduce_pair_votes(oracle, w, s1, s2):
     K = api.spectral_offset_kernels(w, s1, s2)
     s1, s2 = float(np.float32(s1)), float(np.float32(s2))  # the C ABI takes float sigmas
@@ -100,3 +100,26 @@ def test_spectral_lengths(n, w, L):
             while m % p == 0:
                 m //= p
         assert m == 1
+
+
+@pytest.mark.parametrize("w,sigma", [(7, 1.5), (9, 2.5), (6, 1.0)])
+def test_atv_offset_kernels_reproduce_pair_votes(oracle, w, sigma):
+    """Analytical tensor voting: sum_f K[c,f](d) F_f with F = (l2 - l1) m(e2),
+    (l1 - l0) m(e0), l0 equals the oracle's literal FP64 atv() (tensorvote.h:688-760)
+    for random voters at every offset of the window, the voter itself included."""
+    K = api.spectral_offset_kernels_atv(w, sigma)
+    sigma = float(np.float32(sigma))
+    r = w // 2
+    rng = np.random.default_rng(w)
+    worst, scale = 0.0, 0.0
+    for _ in range(4):
+        Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        lam = np.sort(rng.uniform(-1.0, 3.0, 3))
+        e0, e1, e2 = Q[:, 0], Q[:, 1], Q[:, 2]
+        F = np.concatenate([(lam[2] - lam[1]) * monos(e2), (lam[1] - lam[0]) * monos(e0), [lam[0]]])
+        for dw, dv, du in itertools.product(range(-r, r + 1), repeat=3):
+            got = K[:, :, dw + r, dv + r, du + r] @ F
+            ref = oracle.atv3_pair(lam, e1, e2, (du, dv, dw), sigma, f64=True)[[0, 1, 3, 2, 4, 5]]
+            worst = max(worst, np.abs(got - ref).max())
+            scale = max(scale, np.abs(ref).max())
+    assert worst <= 1e-12 * max(scale, 1.0), (worst, scale)

@@ -99,6 +99,8 @@ SIGNATURES = {
     "tira_tensorvote2_atv": [_vp, _vp, _vp, _f, _u, _u, _u],
     "tira_stick_saliency2_async": [_vp, _sz, _vp],
     "tira_spectral_offset_kernels": [_u, _f, _f, _u, _i, _i, _vp, _vp],
+    "tira_spectral_offset_kernels_atv": [_u, _f, _vp, _vp],
+    "tira_tensorvote3_atv_method_async": [_vp, _i, _vp, _i, _u, _u, _u, _f, _u, _i],
     "tira_fft3_async": [_vp, _u, _u, _u, _u, _i],
     "tira_spectral_length": [_u, _u],
     "tira_otsu_threshold_async": [_vp, _sz, _i, _vp],

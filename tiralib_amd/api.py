@@ -153,6 +153,16 @@ def spectral_offset_kernels(w, sigma1, sigma2=0.0, stick=True, plate=True):
     return out
 
 
+def spectral_offset_kernels_atv(w, sigma):
+    """Offset kernels of the spectral analytical vote, (6, 31, W, W, W) float64 [c, f, dw, dv, du]."""
+    W = 2 * (int(w) // 2) + 1
+    out = np.empty((6, 31, W, W, W), np.float64)
+    got = C.c_uint()
+    call("tira_spectral_offset_kernels_atv", int(w), float(sigma), out.ctypes.data, C.byref(got))
+    assert got.value == 31
+    return out
+
+
 def spectral_length(n, w):
     """FFT length of an axis of n voxels with window w (0 when above 1024)."""
     return int(_lib.load().tira_spectral_length(int(n), int(w)))

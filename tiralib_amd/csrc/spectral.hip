@@ -467,6 +467,7 @@ struct SpecGeom {
     int L0, L1, L2;       // FFT lengths
     int h0, h1, h2;       // octant extents L/2 + 1
     int nf, nz, nst;      // real features, complex channels, stick features (15 or 0)
+    int atv;              // analytical tensor voting: features s1 m(e2), s2 m(e0), s3 (nf = 31)
     int P2;               // row stride in complex values (L2 rounded up to 16)
     size_t cs;            // complex values per channel volume
     size_t ocs;           // kernel-spectrum values per channel (octant)
@@ -506,22 +507,28 @@ __global__ __launch_bounds__(kFT) void k_spec_xfwd(const float4* __restrict__ ps
     if (row0 >= nrows) return;
     const int Bv = min(B, nrows - row0);
     load_twiddles(tw, twg, L);
-    const bool two = g.nf == 2 * kMono;  // stick and plate
+    const bool two = g.nf >= 2 * kMono;  // stick and plate (or both ATV subspaces)
     for (int j0 = 0; j0 < g.nz; j0 += C) {
         const int Cv = min(C, g.nz - j0);
         if (j0) __syncthreads();  // the previous group's stores have read s
         for (int idx = (int)threadIdx.x; idx < Bv * L; idx += kFT) {
             const int b = idx / L, e = idx - b * L;
-            float A[2 * kMono];
+            float A[2 * kMono + 2];
 #pragma unroll
-            for (int f = 0; f < 2 * kMono; f++) A[f] = 0.0f;
+            for (int f = 0; f < 2 * kMono + 2; f++) A[f] = 0.0f;
             if (e < g.n2) {
                 const size_t vox = (size_t)(row0 + b) * g.n2 + e;
                 float4 S, N;
                 if (FROM_TENSORS) {
                     float l0, l1, l2, e0[3], e2[3];
                     voxel_eig(tensors, layout, vox, l0, l1, l2, e0, e2);
-                    stick_plate_payload(l0, l1, l2, e0, e2, S, N);
+                    if (g.atv) {  // (e2, l2 - l1), (e0, l1 - l0), l0 (the ATV eigen-subspace weights)
+                        S = make_float4(e2[0], e2[1], e2[2], l2 - l1);
+                        N = make_float4(e0[0], e0[1], e0[2], l1 - l0);
+                        A[2 * kMono] = l0;
+                    } else {
+                        stick_plate_payload(l0, l1, l2, e0, e2, S, N);
+                    }
                 } else {
                     if (g.nst) S = ps[vox];
                     if (g.nf > g.nst) N = pp[vox];
@@ -530,7 +537,7 @@ __global__ __launch_bounds__(kFT) void k_spec_xfwd(const float4* __restrict__ ps
                 if (two) mono15(N, A + kMono);
             }
 #pragma unroll
-            for (int j = 0; j < kMono; j++)
+            for (int j = 0; j < kMono + 1; j++)
                 if (j >= j0 && j < j0 + Cv) s[((j - j0) * Bv + b) * L + e] = make_float2(A[2 * j], A[2 * j + 1]);
         }
         __syncthreads();
@@ -583,7 +590,7 @@ __global__ __launch_bounds__(256) void k_spec_mul(float2* __restrict__ Z, const 
         for (int h = 0; h < 2; h++) {
             const int f = 2 * j + h;
             if (f >= g.nf) break;
-            const int m = f < g.nst ? f : f - g.nst;
+            const int m = f % kMono;  // the constant feature (ATV) shares x^4's parities (all even)
             const int am = mono_a(m), bm = mono_b(m);
             float2 F[4];
 #pragma unroll
@@ -811,10 +818,42 @@ static void plate_vote64(const double n[3], const double d[3], bool self, const 
     }
 }
 
-// K[ch][dw][dv][du], ch = c * nf + f (see tira_spectral_offset_kernels).
-static int offset_kernels(unsigned w, float s1, float s2, bool stick, bool plate, std::vector<double>& K, int& nf) {
+// Analytical tensor voting pair vote (tira::atv, tensorvote.h:688-760) in
+// its collapsed form, FP64: voter eigen-subspace weights (s1, s2, s3) =
+// (l2 - l1, l1 - l0, l0), eigenvectors e0, e2, unit offset v (rho = 1) or the
+// voter itself (v = 0, rho = 0), attenuation c:
+//   KI I + K00 e0 e0^T + K22 e2 e2^T + Kvv v v^T + K0 (v e0^T + e0 v^T) + K2 (v e2^T + e2 v^T).
+static void atv_vote64(const double e0[3], const double e2[3], const double v[3], double rho, double c, double s1,
+                       double s2, double s3, double V[6]) {
+    const double a = e2[0] * v[0] + e2[1] * v[1] + e2[2] * v[2];
+    const double b = e0[0] * v[0] + e0[1] * v[1] + e0[2] * v[2];
+    const double S1 = c * s1, S2 = c * s2, S3 = c * s3;
+    const double K22 = S1 * (1.0 - a * a);
+    const double K2 = -2.0 * a * K22;
+    const double K00 = -S2 * (1.0 - rho / 4.0 + 0.75 * b * b);
+    const double K0 = S2 * b * (1.0 + 1.5 * b * b);
+    const double KI = S2 * (1.0 - rho / 4.0 + 0.25 * b * b) + S3 * (1.0 - rho / 5.0);
+    const double Kvv = -S2 * (0.5 + b * b + 3.0 * b * b * b * b) - 0.4 * S3 + 4.0 * a * a * K22;
+    for (int k = 0; k < 6; k++) V[k] = 0.0;
+    V[0] += KI; V[3] += KI; V[5] += KI;
+    sym_add(V, K00, e0, e0);
+    sym_add(V, K22, e2, e2);
+    sym_add(V, Kvv, v, v);
+    sym_add(V, 2.0 * K0, v, e0);
+    sym_add(V, 2.0 * K2, v, e2);
+}
+
+// K[ch][dw][dv][du], ch = c * nf + f (see tira_spectral_offset_kernels). With
+// `atv`, the features are s1 m(e2) (15), s2 m(e0) (15) and s3 (1), the window
+// half-width w / 2 and the attenuation exp(-|d|^2 / s1^2) of ATV.
+static int offset_kernels(unsigned w, float s1, float s2, bool stick, bool plate, bool atv, std::vector<double>& K,
+                          int& nf) {
     const int r = (int)(w / 2), W = 2 * r + 1;
-    nf = (stick ? kMono : 0) + (plate ? kMono : 0);
+    if (atv) {
+        stick = plate = true;
+        s2 = 0.0f;
+    }
+    nf = (stick ? kMono : 0) + (plate ? kMono : 0) + (atv ? 1 : 0);
     const int npts = 32;
     std::vector<double> Q, P;
     fit_points(Q, npts);
@@ -835,7 +874,7 @@ static int offset_kernels(unsigned w, float s1, float s2, bool stick, bool plate
                 if (l2 > 0) { const double L = std::sqrt(l2); d[0] = du / L; d[1] = dv / L; d[2] = dw / L; }
                 const double c1 = s1 > 0 ? std::exp(-l2 / ((double)s1 * s1)) : (l2 > 0 ? 0.0 : 1.0);
                 const double c2 = s2 > 0 ? std::exp(-l2 / ((double)s2 * s2)) : (l2 > 0 ? 0.0 : 1.0);
-                if (plate) {
+                if (plate && !atv) {
                     if (l2 == 0) {
                         al.assign(2, 0.0); be.assign(2, 0.0); ga.assign(2, 0.0); de.assign(3, 0.0);
                         al[0] = 0.5 * M_PI * c1;
@@ -844,9 +883,21 @@ static int offset_kernels(unsigned w, float s1, float s2, bool stick, bool plate
                         plate_polys(1, c1, c2, al, be, ga, de);
                     }
                 }
+                const double zero[3] = {0.0, 0.0, 0.0};
+                const double rho = l2 > 0 ? 1.0 : 0.0;
                 for (int i = 0; i < npts; i++) {
+                    if (atv) {
+                        atv_vote64(zero, &Q[i * 3], d, rho, c1, 1.0, 0.0, 0.0, &Vs[(size_t)i * 6]);
+                        atv_vote64(&Q[i * 3], zero, d, rho, c1, 0.0, 1.0, 0.0, &Vp[(size_t)i * 6]);
+                        continue;
+                    }
                     if (stick) stick_vote64(&Q[i * 3], d, c1, c2, sn, &Vs[(size_t)i * 6]);
                     if (plate) plate_vote64(&Q[i * 3], d, l2 == 0, al, be, ga, de, &Vp[(size_t)i * 6]);
+                }
+                if (atv) {  // the ball weight s3: a constant kernel
+                    double V3[6];
+                    atv_vote64(zero, zero, d, rho, c1, 0.0, 0.0, 1.0, V3);
+                    for (int c = 0; c < 6; c++) K[((size_t)c * nf + 2 * kMono) * noff + o] = V3[c];
                 }
                 for (int kind = 0; kind < 2; kind++) {
                     if ((kind == 0 && !stick) || (kind == 1 && !plate)) continue;
@@ -867,15 +918,15 @@ static int offset_kernels(unsigned w, float s1, float s2, bool stick, bool plate
 struct SpecKey {
     int L0, L1, L2, w;
     float s1, s2;
-    int stick, plate;
+    int stick, plate, atv;
     bool operator==(const SpecKey& o) const {
         return L0 == o.L0 && L1 == o.L1 && L2 == o.L2 && w == o.w && s1 == o.s1 && s2 == o.s2 &&
-               stick == o.stick && plate == o.plate;
+               stick == o.stick && plate == o.plate && atv == o.atv;
     }
 };
 
 struct SpecCache {
-    SpecKey key{0, 0, 0, 0, 0, 0, 0, 0};
+    SpecKey key{0, 0, 0, 0, 0, 0, 0, 0, 0};
     int device = -1;
     void* buf = nullptr;  // twiddles (3 axes) then the octant spectra
     size_t bytes = 0;
@@ -928,7 +979,7 @@ static int spectra_for(const SpecKey& key, const SpecGeom& g, hipStream_t st, co
     }
     g_spec.bytes = tb + kbytes;
     g_spec.device = dev;
-    g_spec.key = SpecKey{0, 0, 0, 0, 0, 0, 0, 0};
+    g_spec.key = SpecKey{0, 0, 0, 0, 0, 0, 0, 0, 0};
     set_ptrs();
 
     // twiddles
@@ -942,7 +993,7 @@ static int spectra_for(const SpecKey& key, const SpecGeom& g, hipStream_t st, co
     // offset kernels, parities and trig tables
     std::vector<double> K;
     int nf = 0;
-    TH_TRY(offset_kernels((unsigned)key.w, key.s1, key.s2, key.stick != 0, key.plate != 0, K, nf));
+    TH_TRY(offset_kernels((unsigned)key.w, key.s1, key.s2, key.stick != 0, key.plate != 0, key.atv != 0, K, nf));
     const int w = 2 * (key.w / 2) + 1, r = key.w / 2;
     const int nch = 6 * nf;
     std::vector<unsigned char> par(nch);
@@ -1049,14 +1100,18 @@ static LineGeom strided_geom(size_t cs, int nchan, int L, size_t estride, int na
 // a volume of at least 32^3 voxels (below that the gather costs microseconds
 // and the first call's spectra build would dominate), and device memory for
 // the spectra, their build and the feature volumes within 80% of the device.
+// With `atv`: analytical tensor voting, whose 31 features (16 complex
+// channels) must fit one x-pass group (L2 <= 256), as its features come from
+// the input tensors in the x pass only.
 bool spectral_preferred(unsigned s0, unsigned s1, unsigned s2, unsigned w, unsigned power, bool stick, bool plate,
-                        unsigned samples) {
+                        unsigned samples, bool atv) {
     const unsigned p = power < 1 ? 1 : power;
-    if (p != 1 || (plate && samples > 0 && samples < 3) || w < 5) return false;
-    if ((size_t)s0 * s1 * s2 < 32768) return false;
+    if (!atv && (p != 1 || (plate && samples > 0 && samples < 3))) return false;
+    if (w < 5 || (size_t)s0 * s1 * s2 < 32768) return false;
     const unsigned L0 = spectral_length(s0, w), L1 = spectral_length(s1, w), L2 = spectral_length(s2, w);
     if (!L0 || !L1 || !L2 || 3 * L2 > (unsigned)kFftBL) return false;
-    const size_t nf = (stick ? kMono : 0) + (plate ? kMono : 0), nz = (nf + 1) / 2;
+    const size_t nf = atv ? 2 * kMono + 1 : (stick ? kMono : 0) + (plate ? kMono : 0), nz = (nf + 1) / 2;
+    if (atv && nz * L2 > (size_t)kFftBL) return false;
     const size_t W = 2 * (w / 2) + 1;
     const size_t h0 = L0 / 2 + 1, h1 = L1 / 2 + 1, h2 = L2 / 2 + 1;
     const size_t need = nz * (size_t)L0 * L1 * L2 * 8 + 6 * nf * h0 * h1 * h2 * 4 + 6 * nf * W * h1 * h2 * 8 +
@@ -1072,9 +1127,15 @@ bool spectral_preferred(unsigned s0, unsigned s1, unsigned s2, unsigned w, unsig
 int vote3_spectral_run(const float* tensors, int in_layout, const float* lam, const float* ev2, const float* ev0,
                        float* out, int out_layout, unsigned s0, unsigned s1, unsigned s2, float sigma1,
                        float sigma2, unsigned w, unsigned power, bool stick, bool plate, unsigned samples,
-                       hipStream_t st) {
+                       bool atv, hipStream_t st) {
     const unsigned p = power < 1 ? 1 : power;
-    if (p != 1 || (plate && samples > 0 && samples < 3)) return kSpectralNotApplicable;
+    if (atv) {  // sigma2, power, stick, plate and samples do not apply to ATV
+        stick = plate = true;
+        sigma2 = 0.0f;
+        if (!tensors) return kSpectralNotApplicable;
+    } else if (p != 1 || (plate && samples > 0 && samples < 3)) {
+        return kSpectralNotApplicable;
+    }
     const unsigned L0 = spectral_length(s0, w), L1 = spectral_length(s1, w), L2 = spectral_length(s2, w);
     if (!L0 || !L1 || !L2 || 3 * L2 > (unsigned)kFftBL) return kSpectralNotApplicable;
     FftPlan p0, p1, p2;
@@ -1086,9 +1147,10 @@ int vote3_spectral_run(const float* tensors, int in_layout, const float* lam, co
     g.n0 = (int)s0; g.n1 = (int)s1; g.n2 = (int)s2;
     g.L0 = (int)L0; g.L1 = (int)L1; g.L2 = (int)L2;
     g.h0 = g.L0 / 2 + 1; g.h1 = g.L1 / 2 + 1; g.h2 = g.L2 / 2 + 1;
-    g.nf = (stick ? kMono : 0) + (plate ? kMono : 0);
+    g.nf = (stick ? kMono : 0) + (plate ? kMono : 0) + (atv ? 1 : 0);
     g.nz = (g.nf + 1) / 2;
     g.nst = stick ? kMono : 0;
+    g.atv = atv ? 1 : 0;
     g.P2 = ((int)L2 + 15) & ~15;  // rows start on 128-byte lines
     g.cs = (size_t)L0 * L1 * g.P2;
     g.ocs = (size_t)g.h0 * g.h1 * g.h2;
@@ -1098,12 +1160,13 @@ int vote3_spectral_run(const float* tensors, int in_layout, const float* lam, co
     const int B = std::max(1, kFftBL / (C * g.L2));
     // one channel group: the x pass decomposes the voxels itself (no payload round trip)
     const bool fused = tensors && C == g.nz;
+    if (atv && !fused) return kSpectralNotApplicable;
     float4 *ps = nullptr, *pp = nullptr;
     float2* Z = nullptr;
     if (stick && !fused) TH_TRY(ws_get(WS_PAY0, nvox * 16, (void**)&ps));
     if (plate && !fused) TH_TRY(ws_get(WS_PAY1, nvox * 16, (void**)&pp));
     TH_TRY(ws_get(WS_SPEC0, (size_t)g.nz * g.cs * 8, (void**)&Z));
-    const SpecKey key{g.L0, g.L1, g.L2, (int)w, sigma1, sigma2, stick ? 1 : 0, plate ? 1 : 0};
+    const SpecKey key{g.L0, g.L1, g.L2, (int)w, sigma1, sigma2, stick ? 1 : 0, plate ? 1 : 0, atv ? 1 : 0};
     const float2 *tw0, *tw1, *tw2;
     const float* kh;
     TH_TRY(spectra_for(key, g, st, &tw0, &tw1, &tw2, &kh));
@@ -1171,7 +1234,18 @@ int tira_spectral_offset_kernels(unsigned w, float sigma1, float sigma2, unsigne
     TH_ARG(sigma1 >= 0.0f && sigma2 >= 0.0f, "sigmas must be >= 0");
     std::vector<double> K;
     int nf = 0;
-    TH_TRY(offset_kernels(w, sigma1, sigma2, stick != 0, plate != 0, K, nf));
+    TH_TRY(offset_kernels(w, sigma1, sigma2, stick != 0, plate != 0, false, K, nf));
+    std::copy(K.begin(), K.end(), out);
+    if (nf_out) *nf_out = (unsigned)nf;
+    return TIRA_OK;
+}
+
+int tira_spectral_offset_kernels_atv(unsigned w, float sigma, double* out, unsigned* nf_out) {
+    TH_ARG(out, "NULL pointer");
+    TH_ARG(sigma >= 0.0f, "sigma must be >= 0");
+    std::vector<double> K;
+    int nf = 0;
+    TH_TRY(offset_kernels(w, sigma, 0.0f, true, true, true, K, nf));
     std::copy(K.begin(), K.end(), out);
     if (nf_out) *nf_out = (unsigned)nf;
     return TIRA_OK;

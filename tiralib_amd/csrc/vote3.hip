@@ -1028,7 +1028,7 @@ int vote3_run(const float* tensors, int in_layout, const float* lam, const float
         plate_mode = TIRA_VOTE_SPECTRAL;
     if (plate_mode == TIRA_VOTE_SPECTRAL) {
         const int rc = vote3_spectral_run(tensors, in_layout, lam, ev2, ev0, out, out_layout, s0, s1, s2, sigma1,
-                                          sigma2, w, power, stick, plate, samples, st);
+                                          sigma2, w, power, stick, plate, samples, false, st);
         if (rc != kSpectralNotApplicable) return rc;
         plate_mode = TIRA_PLATE_AUTO;  // power, samples or size outside the spectral method: direct gather
     }
@@ -1086,11 +1086,21 @@ int vote3_run(const float* tensors, int in_layout, const float* lam, const float
 // ATV on device buffers (tensorvote.h:2635-2739 semantics): `tensors` in
 // `in_layout`, or caller eigenpairs lam / ev1 (middle) / ev2 (largest).
 // The window half-width is w / 2, as the reference's loops take it.
+// method: TIRA_PLATE_AUTO (the spectral vote when it applies: tensor input,
+// w >= 5, at least 32^3 voxels, x-rows of at most 256), TIRA_PLATE_CLOSED
+// (always the gather) or TIRA_VOTE_SPECTRAL (spectral whenever the input
+// allows it).
 int vote3_atv_run(const float* tensors, int in_layout, const float* lam, const float* ev1, const float* ev2,
                   float* out, int out_layout, unsigned s0, unsigned s1, unsigned s2, float sigma, unsigned w,
-                  hipStream_t st) {
+                  hipStream_t st, int method) {
     const size_t nvox = (size_t)s0 * s1 * s2;
     if (nvox == 0) return TIRA_OK;
+    if (tensors && (method == TIRA_VOTE_SPECTRAL ||
+                    (method == TIRA_PLATE_AUTO && spectral_preferred(s0, s1, s2, w, 1, true, true, 0, true)))) {
+        const int rc = vote3_spectral_run(tensors, in_layout, nullptr, nullptr, nullptr, out, out_layout, s0, s1, s2,
+                                          sigma, 0.0f, w, 1, true, true, 0, true, st);
+        if (rc != kSpectralNotApplicable) return rc;
+    }
     const unsigned wo = 2 * (w / 2) + 1;
     const VoteGeom g = make_geom(s0, s1, s2, wo, kBZ);
     const size_t ptotal = (size_t)g.P0 * g.P1 * g.P2;
@@ -1263,7 +1273,7 @@ int tira_tensorvote3(const float* input_field, float* output_field, unsigned s0,
         TH_TRY(stage_in(input_field, n * 36, WS_IN0, (const void**)&din));
         TH_TRY(stage_out(output_field, n * 36, WS_OUT0, &dout));
         TH_TRY(vote3_atv_run(din, TIRA_LAYOUT_MAT9, nullptr, nullptr, nullptr, (float*)dout, TIRA_LAYOUT_MAT9, s0,
-                             s1, s2, sigma, w, stream()));
+                             s1, s2, sigma, w, stream(), TIRA_PLATE_AUTO));
         return finish_out(output_field, dout, n * 36);
     }
     TH_TRY(check_vote_args(s0, s1, s2, w, sigma, sigma2));
@@ -1333,7 +1343,8 @@ int tira_tensorvote3_atv(float* output_field, const float* lambdas, const float*
     TH_TRY(stage_in(evec1, n * 12, WS_IN1, (const void**)&d1));
     TH_TRY(stage_in(evec2, n * 12, WS_IN2, (const void**)&d2));
     TH_TRY(stage_out(output_field, n * 36, WS_OUT0, &dout));
-    TH_TRY(vote3_atv_run(nullptr, 0, dl, d1, d2, (float*)dout, TIRA_LAYOUT_MAT9, s0, s1, s2, sigma, w, stream()));
+    TH_TRY(vote3_atv_run(nullptr, 0, dl, d1, d2, (float*)dout, TIRA_LAYOUT_MAT9, s0, s1, s2, sigma, w, stream(),
+                         TIRA_PLATE_AUTO));
     return finish_out(output_field, dout, n * 36);
 }
 
@@ -1344,7 +1355,19 @@ int tira_tensorvote3_atv_async(const float* input_field, int in_layout, float* o
     TH_ARG(out_layout == TIRA_LAYOUT_MAT9 || out_layout == TIRA_LAYOUT_PACKED6, "bad output layout");
     TH_TRY(check_atv_args(s0, s1, s2, w, sigma));
     return vote3_atv_run(input_field, in_layout, nullptr, nullptr, nullptr, output_field, out_layout, s0, s1, s2,
-                         sigma, w, stream());
+                         sigma, w, stream(), TIRA_PLATE_AUTO);
+}
+
+int tira_tensorvote3_atv_method_async(const float* input_field, int in_layout, float* output_field, int out_layout,
+                                      unsigned s0, unsigned s1, unsigned s2, float sigma, unsigned w, int method) {
+    TH_ARG(input_field && output_field, "NULL pointer");
+    TH_ARG(in_layout == TIRA_LAYOUT_MAT9 || in_layout == TIRA_LAYOUT_PACKED6, "bad input layout");
+    TH_ARG(out_layout == TIRA_LAYOUT_MAT9 || out_layout == TIRA_LAYOUT_PACKED6, "bad output layout");
+    TH_ARG(method == TIRA_PLATE_AUTO || method == TIRA_PLATE_CLOSED || method == TIRA_VOTE_SPECTRAL,
+           "method must be TIRA_PLATE_AUTO, TIRA_PLATE_CLOSED (gather) or TIRA_VOTE_SPECTRAL");
+    TH_TRY(check_atv_args(s0, s1, s2, w, sigma));
+    return vote3_atv_run(input_field, in_layout, nullptr, nullptr, nullptr, output_field, out_layout, s0, s1, s2,
+                         sigma, w, stream(), method);
 }
 
 }  // extern "C"

@@ -23,13 +23,13 @@ void plate_polys(int P, double c1, double c2, std::vector<double>& al, std::vect
 int vote3_payload_flat(const float* tensors, int in_layout, const float* lam, const float* ev2, const float* ev0,
                        unsigned s0, unsigned s1, unsigned s2, float4* ps, float4* pp, hipStream_t st);
 bool spectral_preferred(unsigned s0, unsigned s1, unsigned s2, unsigned w, unsigned power, bool stick, bool plate,
-                        unsigned samples);
+                        unsigned samples, bool atv = false);
 // Status of vote3_spectral_run when the request is outside the spectral method.
 constexpr int kSpectralNotApplicable = -1;
 int vote3_spectral_run(const float* tensors, int in_layout, const float* lam, const float* ev2, const float* ev0,
                        float* out, int out_layout, unsigned s0, unsigned s1, unsigned s2, float sigma1,
                        float sigma2, unsigned w, unsigned power, bool stick, bool plate, unsigned samples,
-                       hipStream_t st);
+                       bool atv, hipStream_t st);
 int resolve_plate_mode(int plate_mode, unsigned power, unsigned samples);
 int vote3_run(const float* tensors, int in_layout, const float* lam, const float* ev2, const float* ev0, float* out,
               int out_layout, unsigned s0, unsigned s1, unsigned s2, float sigma1, float sigma2, unsigned w,
