@@ -215,7 +215,13 @@ __device__ __forceinline__ int fdiv(int n, int d, float rd) {
 // reads its butterflies into registers before any write.
 // Each radix is its own function: inlined together, the six stage bodies
 // shared one register allocation of more than twice any single one.
-template <int R, bool INV, int BL = kFftBL>
+// Two thread mappings: when m = L/R <= kFT and it takes no more rounds, a
+// thread keeps one butterfly index i for all its rounds (G = kFT/m lines per
+// round), so its R - 1 twiddles are read once per stage and a round costs
+// the butterfly alone; otherwise the butterflies of the block are dealt out
+// in order, kFT per round. FIXED = false compiles the second mapping only
+// (the column passes, whose B lines rarely fill whole rounds the first way).
+template <int R, bool INV, int BL = kFftBL, bool FIXED = true>
 __device__ __noinline__ void fft_stage(lds_f2* __restrict__ s, int L, int nlines, int p,
                                        const lds_f2* __restrict__ tw) {
     constexpr int MAXQ = (BL / R + kFT - 1) / kFT;
@@ -224,6 +230,42 @@ __device__ __noinline__ void fft_stage(lds_f2* __restrict__ s, int L, int nlines
     const int step = L / (p * R);
     const float rm = 1.0f / (float)m, rp = 1.0f / (float)p;
     float2 u[MAXQ][R];
+    const int G = kFT / m;
+    if (FIXED && G > 0 && (nlines + G - 1) / G <= (nb + kFT - 1) / kFT) {
+        const int l0 = fdiv((int)threadIdx.x, m, rm), i = (int)threadIdx.x - l0 * m;
+        const bool act = l0 < G;  // threadIdx.x < G m
+        const int k = i - fdiv(i, p, rp) * p;
+        float2 w[R];
+#pragma unroll
+        for (int r = 1; r < R; r++) {
+            w[r] = act ? ld2(tw + r * k * step) : make_float2(1.0f, 0.0f);  // r k step < L
+            if (INV) w[r].y = -w[r].y;
+        }
+#pragma unroll
+        for (int q = 0; q < MAXQ; q++) {  // rounds: ceil(nlines / G) <= MAXQ
+            const int line = l0 + q * G;
+            if (act && line < nlines) {
+                const lds_f2* src = s + line * L + i;
+#pragma unroll
+                for (int r = 0; r < R; r++) u[q][r] = ld2(src + r * m);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < MAXQ; q++) {
+            const int line = l0 + q * G;
+            if (act && line < nlines) {
+#pragma unroll
+                for (int r = 1; r < R; r++) u[q][r] = cmul(u[q][r], w[r]);
+                dft_small<R, INV>(u[q]);
+                lds_f2* dst = s + line * L + (i - k) * R + k;
+#pragma unroll
+                for (int r = 0; r < R; r++) st2(dst + r * p, u[q][r]);
+            }
+        }
+        __syncthreads();
+        return;
+    }
 #pragma unroll
     for (int q = 0; q < MAXQ; q++) {
         const int t = (int)threadIdx.x + q * kFT;
@@ -455,7 +497,7 @@ __global__ __launch_bounds__(kFT) void k_fft_cols(float2* __restrict__ data, Lin
     int p = pl.rad[0];
     for (int st = 1; st < pl.nst - 1; st++) {
         const int R = pl.rad[st];
-        TH_RADIX_SWITCH(R, (fft_stage<RR, INV, BL>(s, L, B, p, tw)));
+        TH_RADIX_SWITCH(R, (fft_stage<RR, INV, BL, false>(s, L, B, p, tw)));
         p *= R;
     }
     TH_RADIX_SWITCH(pl.rad[pl.nst - 1], (cols_last<RR, INV>(s, base, tw, L, B, lb, Bv, g.estride, g.n_out)));
@@ -508,20 +550,30 @@ __global__ __launch_bounds__(kFT) void k_spec_xfwd(const float4* __restrict__ ps
     const int Bv = min(B, nrows - row0);
     load_twiddles(tw, twg, L);
     const bool two = g.nf >= 2 * kMono;  // stick and plate (or both ATV subspaces)
+    const int npad = L - g.n2;
+    // index splits below use fdiv (a float reciprocal), not integer division
+    const float rB = 1.0f / (float)Bv, rn1 = 1.0f / (float)g.n1, rn2 = 1.0f / (float)g.n2,
+                rpad = 1.0f / (float)max(npad, 1);
     for (int j0 = 0; j0 < g.nz; j0 += C) {
         const int Cv = min(C, g.nz - j0);
         if (j0) __syncthreads();  // the previous group's stores have read s
-        for (int idx = (int)threadIdx.x; idx < Bv * L; idx += kFT) {
-            const int b = idx / L, e = idx - b * L;
+        // the zero padding of every line, then one element per voxel (the
+        // eigen-decompositions fill whole waves: 2 rows of 128 are one pass)
+        for (int idx = (int)threadIdx.x; idx < Cv * Bv * npad; idx += kFT) {
+            const int line = fdiv(idx, npad, rpad);
+            s[line * L + g.n2 + (idx - line * npad)] = make_float2(0.0f, 0.0f);
+        }
+        for (int idx = (int)threadIdx.x; idx < Bv * g.n2; idx += kFT) {
+            const int b = fdiv(idx, g.n2, rn2), e = idx - b * g.n2;
             float A[2 * kMono + 2];
 #pragma unroll
             for (int f = 0; f < 2 * kMono + 2; f++) A[f] = 0.0f;
-            if (e < g.n2) {
+            {
                 const size_t vox = (size_t)(row0 + b) * g.n2 + e;
                 float4 S, N;
                 if (FROM_TENSORS) {
                     float l0, l1, l2, e0[3], e2[3];
-                    voxel_eig(tensors, layout, vox, l0, l1, l2, e0, e2);
+                    voxel_eig<false>(tensors, layout, vox, l0, l1, l2, e0, e2);
                     if (g.atv) {  // (e2, l2 - l1), (e0, l1 - l0), l0 (the ATV eigen-subspace weights)
                         S = make_float4(e2[0], e2[1], e2[2], l2 - l1);
                         N = make_float4(e0[0], e0[1], e0[2], l1 - l0);
@@ -542,12 +594,14 @@ __global__ __launch_bounds__(kFT) void k_spec_xfwd(const float4* __restrict__ ps
         }
         __syncthreads();
         fft_lines<false>(s, pl, Cv * Bv, tw);
-        for (int idx = (int)threadIdx.x; idx < Cv * Bv * L; idx += kFT) {
-            const int line = idx / L, e = idx - line * L;
-            const int c = line / Bv, b = line - c * Bv;
+        // each wave stores whole lines (their bases are wave-uniform)
+        for (int line = (int)threadIdx.x >> 6; line < Cv * Bv; line += kFT / 64) {
+            const int ln = __builtin_amdgcn_readfirstlane(line);
+            const int c = fdiv(ln, Bv, rB), b = ln - c * Bv;
             const int row = row0 + b;
-            const int z = row / g.n1, y = row - z * g.n1;
-            Z[(size_t)(j0 + c) * g.cs + ((size_t)z * g.L1 + y) * g.P2 + e] = s[line * L + e];
+            const int z = fdiv(row, g.n1, rn1), y = row - z * g.n1;
+            float2* dst = Z + (size_t)(j0 + c) * g.cs + ((size_t)z * g.L1 + y) * g.P2;
+            for (int e = (int)threadIdx.x & 63; e < L; e += 64) dst[e] = s[ln * L + e];
         }
     }
 }
@@ -636,17 +690,19 @@ __global__ __launch_bounds__(kFT) void k_spec_xinv(const float2* __restrict__ Z,
     if (row0 >= nrows) return;
     const int Bv = min(B, nrows - row0);
     load_twiddles(tw, twg, L);
+    // index splits use fdiv (a float reciprocal), not integer division
+    const float rL = 1.0f / (float)L, rn1 = 1.0f / (float)g.n1, rn2 = 1.0f / (float)g.n2;
     for (int idx = (int)threadIdx.x; idx < 3 * Bv * L; idx += kFT) {
-        const int line = idx / L, e = idx - line * L;  // line = 3 l + i
-        const int l = line / 3, i = line - 3 * l;
+        const int line = fdiv(idx, L, rL), e = idx - line * L;  // line = 3 l + i
+        const int l = fdiv(line, 3, 1.0f / 3.0f), i = line - 3 * l;
         const int row = row0 + l;
-        const int z = row / g.n1, y = row - z * g.n1;
+        const int z = fdiv(row, g.n1, rn1), y = row - z * g.n1;
         s[line * L + e] = Z[(size_t)i * g.cs + ((size_t)z * g.L1 + y) * g.P2 + e];
     }
     __syncthreads();
     fft_lines<true>(s, pl, 3 * Bv, tw);
     for (int idx = (int)threadIdx.x; idx < Bv * g.n2; idx += kFT) {
-        const int l = idx / g.n2, x = idx - l * g.n2;
+        const int l = fdiv(idx, g.n2, rn2), x = idx - l * g.n2;
         const float2 w0 = s[(3 * l + 0) * L + x], w1 = s[(3 * l + 1) * L + x], w2 = s[(3 * l + 2) * L + x];
         const size_t v = (size_t)(row0 + l) * g.n2 + x;
         if (layout == TIRA_LAYOUT_MAT9) {

@@ -21,8 +21,13 @@ __device__ __forceinline__ void load_tensor(const float* __restrict__ in, int la
 }
 
 // Eigen-decomposition of voxel v for the payload: power-of-two prescale
-// (exact; makes every threshold relative to |A|), FP32 deflated solve, FP64
-// re-solve when the relative eigengap is below 1e-4.
+// (exact; makes every threshold relative to |A|), FP32 deflated solve, and
+// with REFINE an FP64 re-solve when the relative eigengap is below 1e-4.
+// Without REFINE the close pair's eigenvectors carry FP32 errors of order
+// eps |A| / gap; the vote weighs e2 by |l2| - |l1| and e0 by |l1| - |l0|,
+// so a vector of an ill-separated pair enters multiplied by that pair's gap
+// and the product stays of order eps |A|.
+template <bool REFINE = true>
 __device__ __forceinline__ void voxel_eig(const float* __restrict__ in, int layout, size_t v, float& l0, float& l1,
                                           float& l2, float e0[3], float e2[3]) {
     float a, b, c, d, e, f;
@@ -36,7 +41,7 @@ __device__ __forceinline__ void voxel_eig(const float* __restrict__ in, int layo
     }
     float lam[3];
     eig3_solve<float>(a, b, c, d, e, f, 1e-30f, lam, e0, e2);
-    if (eig3_near_degenerate(lam[0], lam[1], lam[2])) {
+    if (REFINE && eig3_near_degenerate(lam[0], lam[1], lam[2])) {
         double dl[3], de0[3], de2[3];
         eig3_solve<double>(a, b, c, d, e, f, 1e-30, dl, de0, de2);
         for (int k = 0; k < 3; k++) { lam[k] = (float)dl[k]; e0[k] = (float)de0[k]; e2[k] = (float)de2[k]; }
