@@ -31,7 +31,7 @@
 //     the other four;
 //  5. inverse z pass (outputs z < n0), inverse y pass (outputs y < n1), and
 //     the inverse x pass fused with the output write (either layout).
-// FFTs are Stockham autosort, mixed radix 2/3/4/5/7/8, one block per batch of
+// FFTs are Stockham autosort, mixed radix 2/3/4/5/7/8/9, one block per batch of
 // lines staged in LDS; the twiddle table is FP64-built and rounded once.
 // The kernel spectra depend on (L0, L1, L2, w, sigma1, sigma2, stick, plate)
 // only: built on the device (separable FP64 cosine/sine transforms of the
@@ -77,13 +77,13 @@ struct FftPlan {
     int rad[kMaxStages];
 };
 
-// Radices of L (7s, 5s, 3s, then 8s and 4s, a final 2); false when L has a
+// Radices of L (7s, 5s, 9s, a 3, then 8s and 4s, a final 2); false when L has a
 // prime factor above 7.
 static bool make_plan(int L, FftPlan& p) {
     p.L = L;
     p.nst = 0;
     int n = L;
-    for (int f : {7, 5, 3})
+    for (int f : {7, 5, 9, 3})  // 9 before 3: pairs of radix-3 stages become one radix-9 stage
         while (n % f == 0) { p.rad[p.nst++] = f; n /= f; }
     int t = 0;
     while (n % 2 == 0) { t++; n /= 2; }
@@ -169,6 +169,34 @@ __device__ __forceinline__ void dft_small(float2 (&u)[R]) {
         u[2] = csub(a, c);
         u[1] = mid;
         u[3] = oth;
+    } else if constexpr (R == 9) {
+        // 3 x 3 Cooley-Tukey: j = 3a + b, k = c + 3d, W9^(jk) = W3^(ac) W9^(bc) W3^(bd):
+        // length-3 DFTs over a, twiddles W9^(bc), length-3 DFTs over b
+        constexpr float c1 = 0.766044443118978035f, s1 = 0.642787609686539326f;   // 2 pi / 9
+        constexpr float c2 = 0.173648177666930349f, s2 = 0.984807753012208059f;   // 4 pi / 9
+        constexpr float c4 = -0.939692620785908384f, s4 = 0.342020143325668733f;  // 8 pi / 9
+        const float sg = INV ? 1.0f : -1.0f;
+        float2 v[3][3];
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            float2 t[3] = {u[b], u[3 + b], u[6 + b]};
+            dft_small<3, INV>(t);
+            v[b][0] = t[0];
+            v[b][1] = t[1];
+            v[b][2] = t[2];
+        }
+        v[1][1] = cmul(v[1][1], make_float2(c1, sg * s1));
+        v[1][2] = cmul(v[1][2], make_float2(c2, sg * s2));
+        v[2][1] = cmul(v[2][1], make_float2(c2, sg * s2));
+        v[2][2] = cmul(v[2][2], make_float2(c4, sg * s4));
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            float2 t[3] = {v[0][c], v[1][c], v[2][c]};
+            dft_small<3, INV>(t);
+            u[c] = t[0];
+            u[c + 3] = t[1];
+            u[c + 6] = t[2];
+        }
     } else {
         float2 y[R];
 #pragma unroll
@@ -313,6 +341,7 @@ __device__ __forceinline__ void fft_lines(float2* __restrict__ s_, const FftPlan
             case 4: fft_stage<4, INV>(s, pl.L, nlines, p, tw); break;
             case 5: fft_stage<5, INV>(s, pl.L, nlines, p, tw); break;
             case 7: fft_stage<7, INV>(s, pl.L, nlines, p, tw); break;
+            case 9: fft_stage<9, INV>(s, pl.L, nlines, p, tw); break;
             default: fft_stage<8, INV>(s, pl.L, nlines, p, tw); break;
         }
         p *= R;
@@ -439,7 +468,7 @@ __device__ __forceinline__ void cols_last(const lds_f2* __restrict__ s, float2* 
     }
 }
 
-// Whole transform of a length-R line (R <= 8) in registers.
+// Whole transform of a length-R line (R <= 9) in registers.
 template <int R, bool INV>
 __device__ __forceinline__ void cols_single(float2* __restrict__ base, int B, int Bv, size_t estride, int n_in,
                                             int n_out) {
@@ -464,6 +493,7 @@ __device__ __forceinline__ void cols_single(float2* __restrict__ base, int B, in
         case 4: { constexpr int RR = 4; CALL; } break;     \
         case 5: { constexpr int RR = 5; CALL; } break;     \
         case 7: { constexpr int RR = 7; CALL; } break;     \
+        case 9: { constexpr int RR = 9; CALL; } break;     \
         default: { constexpr int RR = 8; CALL; } break;    \
     }
 
