@@ -29,7 +29,8 @@ NT = 8
 
 
 @pytest.mark.parametrize("shape", [(1, 1, 1), (3, 5, 7), (8, 16, 4), (7, 25, 40), (16, 16, 16), (2, 3, 525),
-                                   (1024, 2, 3), (135, 6, 9), (5, 343, 4), (27, 81, 54), (972, 1, 270)])
+                                   (1024, 2, 3), (135, 6, 9), (5, 343, 4), (27, 81, 54), (972, 1, 270),
+                                   (15, 105, 8), (270, 75, 4)])
 def test_fft3_matches_numpy(hip, shape):
     rng = np.random.default_rng(sum(shape))
     nchan = 2

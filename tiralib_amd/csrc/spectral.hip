@@ -94,6 +94,27 @@ static bool make_plan(int L, FftPlan& p) {
     return p.nst <= kMaxStages;
 }
 
+// Plan of a column pass (k_fft_cols): as make_plan, but a 5 and a lone 3
+// become one radix-15 stage placed last. The last stage of a column pass
+// reads LDS and writes HBM one butterfly at a time, so radix 15 costs no
+// more registers there, while as an in-place LDS stage it would hold two
+// rounds of 15 values across the barrier.
+static bool make_cols_plan(int L, FftPlan& p) {
+    if (!make_plan(L, p)) return false;
+    int i5 = -1, i3 = -1;
+    for (int i = 0; i < p.nst; i++) {
+        if (p.rad[i] == 5 && i5 < 0) i5 = i;
+        if (p.rad[i] == 3) i3 = i;
+    }
+    if (i5 < 0 || i3 < 0) return true;
+    int k = 0;
+    for (int i = 0; i < p.nst; i++)
+        if (i != i5 && i != i3) p.rad[k++] = p.rad[i];
+    p.rad[k++] = 15;
+    p.nst = k;
+    return true;
+}
+
 static unsigned spectral_length(unsigned n, unsigned w) {
     const unsigned need = n + w / 2;
     for (unsigned L = std::max(need, 1u); L <= (unsigned)kMaxL; L++) {
@@ -196,6 +217,23 @@ __device__ __forceinline__ void dft_small(float2 (&u)[R]) {
             u[c] = t[0];
             u[c + 3] = t[1];
             u[c + 6] = t[2];
+        }
+    } else if constexpr (R == 15) {
+        // Good-Thomas (3 and 5 coprime, no twiddles): input n = (5 n1 + 3 n2) mod 15,
+        // output k = (10 k1 + 6 k2) mod 15, so W15^(nk) = W3^(n1 k1) W5^(n2 k2)
+        float2 v[3][5];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+#pragma unroll
+            for (int b = 0; b < 5; b++) v[a][b] = u[(5 * a + 3 * b) % 15];
+            dft_small<5, INV>(v[a]);
+        }
+#pragma unroll
+        for (int b = 0; b < 5; b++) {
+            float2 t[3] = {v[0][b], v[1][b], v[2][b]};
+            dft_small<3, INV>(t);
+#pragma unroll
+            for (int a = 0; a < 3; a++) u[(10 * a + 6 * b) % 15] = t[a];
         }
     } else {
         float2 y[R];
@@ -496,6 +534,12 @@ __device__ __forceinline__ void cols_single(float2* __restrict__ base, int B, in
         case 9: { constexpr int RR = 9; CALL; } break;     \
         default: { constexpr int RR = 8; CALL; } break;    \
     }
+// The last stage of a column pass also takes radix 15 (make_cols_plan).
+#define TH_RADIX_SWITCH_LAST(R, CALL)                     \
+    switch (R) {                                           \
+        case 15: { constexpr int RR = 15; CALL; } break;   \
+        default: TH_RADIX_SWITCH(R, CALL);                 \
+    }
 
 template <bool INV, int BL>
 __global__ __launch_bounds__(kFT) void k_fft_cols(float2* __restrict__ data, LineGeom g, FftPlan pl,
@@ -516,7 +560,7 @@ __global__ __launch_bounds__(kFT) void k_fft_cols(float2* __restrict__ data, Lin
     const int Bv = min(B, g.na - a0);
     float2* base = data + (size_t)c * g.cstride + (size_t)b * g.bstride + (size_t)a0;
     if (pl.nst == 1) {
-        TH_RADIX_SWITCH(pl.rad[0], (cols_single<RR, INV>(base, B, Bv, g.estride, g.n_in, g.n_out)));
+        TH_RADIX_SWITCH_LAST(pl.rad[0], (cols_single<RR, INV>(base, B, Bv, g.estride, g.n_in, g.n_out)));
         return;
     }
     load_twiddles(tw_, twg, L);
@@ -530,7 +574,8 @@ __global__ __launch_bounds__(kFT) void k_fft_cols(float2* __restrict__ data, Lin
         TH_RADIX_SWITCH(R, (fft_stage<RR, INV, BL, false>(s, L, B, p, tw)));
         p *= R;
     }
-    TH_RADIX_SWITCH(pl.rad[pl.nst - 1], (cols_last<RR, INV>(s, base, tw, L, B, lb, Bv, g.estride, g.n_out)));
+    TH_RADIX_SWITCH_LAST(pl.rad[pl.nst - 1],
+                         (cols_last<RR, INV>(s, base, tw, L, B, lb, Bv, g.estride, g.n_out)));
 }
 
 // ---------------------------------------------------------------- spectral vote kernels
@@ -1224,9 +1269,9 @@ int vote3_spectral_run(const float* tensors, int in_layout, const float* lam, co
     }
     const unsigned L0 = spectral_length(s0, w), L1 = spectral_length(s1, w), L2 = spectral_length(s2, w);
     if (!L0 || !L1 || !L2 || 3 * L2 > (unsigned)kFftBL) return kSpectralNotApplicable;
-    FftPlan p0, p1, p2;
-    make_plan((int)L0, p0);
-    make_plan((int)L1, p1);
+    FftPlan p0, p1, p2;  // z and y: column passes; x: lines in LDS
+    make_cols_plan((int)L0, p0);
+    make_cols_plan((int)L1, p1);
     make_plan((int)L2, p2);
 
     SpecGeom g{};
@@ -1345,7 +1390,7 @@ int tira_fft3_async(float* data, unsigned nchan, unsigned L0, unsigned L1, unsig
     TH_ARG(L0 >= 1 && L1 >= 1 && L2 >= 1 && L0 <= (unsigned)kMaxL && L1 <= (unsigned)kMaxL &&
                L2 <= (unsigned)kMaxL,
            "FFT lengths must be in [1, %d]", kMaxL);
-    TH_ARG(make_plan((int)L0, p0) && make_plan((int)L1, p1) && make_plan((int)L2, p2),
+    TH_ARG(make_cols_plan((int)L0, p0) && make_cols_plan((int)L1, p1) && make_plan((int)L2, p2),
            "FFT lengths must have prime factors 2, 3, 5, 7 only");
     TH_ARG(is_device(data), "tira_fft3_async takes a device pointer");
     if (!nchan) return TIRA_OK;
