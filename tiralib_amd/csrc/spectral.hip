@@ -94,6 +94,22 @@ static bool make_plan(int L, FftPlan& p) {
     return p.nst <= kMaxStages;
 }
 
+// Plan of a line transform in LDS (fft_lines): as make_plan, but a 9 and a
+// lone 3 become one radix-27 stage, which takes a single round per thread.
+static bool make_lines_plan(int L, FftPlan& p) {
+    if (!make_plan(L, p)) return false;
+    int i9 = -1, i3 = -1;
+    for (int i = 0; i < p.nst; i++) {
+        if (p.rad[i] == 9 && i9 < 0) i9 = i;
+        if (p.rad[i] == 3) i3 = i;
+    }
+    if (i9 < 0 || i3 < 0) return true;
+    p.rad[i9] = 27;
+    for (int i = i3; i + 1 < p.nst; i++) p.rad[i] = p.rad[i + 1];
+    p.nst--;
+    return true;
+}
+
 // Plan of a column pass (k_fft_cols): as make_plan, but a 5 and a lone 3
 // become one radix-15 stage placed last. The last stage of a column pass
 // reads LDS and writes HBM one butterfly at a time, so radix 15 costs no
@@ -218,6 +234,47 @@ __device__ __forceinline__ void dft_small(float2 (&u)[R]) {
             u[c + 3] = t[1];
             u[c + 6] = t[2];
         }
+    } else if constexpr (R == 27) {
+        // 3 x 9 Cooley-Tukey: j = 9a + b, k = c + 3d, W27^(jk) = W3^(ac) W27^(bc) W9^(bd):
+        // length-3 DFTs over a, twiddles W27^(bc), length-9 DFTs over b
+        constexpr float tc[17] = {1.0f, 0.973044870579823806f, 0.893632640323412275f, 0.766044443118978013f,
+                                  0.597158591702786179f, 0.396079766039156900f, 0.173648177666930414f,
+                                  -0.058144828910475774f, -0.286803232711090206f, -0.5f,
+                                  -0.686241637868733489f, -0.835487811412936265f, -0.939692620785908317f,
+                                  -0.993238357741943023f, -0.993238357741943023f, -0.939692620785908539f,
+                                  -0.835487811412936487f};
+        constexpr float ts[17] = {0.0f, 0.230615870742440165f, 0.448799180200462167f, 0.642787609686539252f,
+                                  0.802123192755043735f, 0.918216106880273997f, 0.984807753012208020f,
+                                  0.998308158271268176f, 0.957989512315488900f, 0.866025403784438708f,
+                                  0.727373641573048846f, 0.549508978070806231f, 0.342020143325668879f,
+                                  0.116092914125229929f, -0.116092914125230123f, -0.342020143325668213f,
+                                  -0.549508978070806009f};
+        const float sg = INV ? 1.0f : -1.0f;
+#pragma unroll
+        for (int b = 0; b < 9; b++) {
+            float2 t[3] = {u[b], u[9 + b], u[18 + b]};
+            dft_small<3, INV>(t);
+            u[b] = t[0];
+            if (b == 0) {
+                u[9] = t[1];
+                u[18] = t[2];
+            } else {
+                u[9 + b] = cmul(t[1], make_float2(tc[b], sg * ts[b]));
+                u[18 + b] = cmul(t[2], make_float2(tc[2 * b], sg * ts[2 * b]));
+            }
+        }
+        float2 y[27];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            float2 t[9];
+#pragma unroll
+            for (int b = 0; b < 9; b++) t[b] = u[9 * c + b];
+            dft_small<9, INV>(t);
+#pragma unroll
+            for (int d = 0; d < 9; d++) y[c + 3 * d] = t[d];
+        }
+#pragma unroll
+        for (int k = 0; k < 27; k++) u[k] = y[k];
     } else if constexpr (R == 15) {
         // Good-Thomas (3 and 5 coprime, no twiddles): input n = (5 n1 + 3 n2) mod 15,
         // output k = (10 k1 + 6 k2) mod 15, so W15^(nk) = W3^(n1 k1) W5^(n2 k2)
@@ -380,6 +437,8 @@ __device__ __forceinline__ void fft_lines(float2* __restrict__ s_, const FftPlan
             case 5: fft_stage<5, INV>(s, pl.L, nlines, p, tw); break;
             case 7: fft_stage<7, INV>(s, pl.L, nlines, p, tw); break;
             case 9: fft_stage<9, INV>(s, pl.L, nlines, p, tw); break;
+            // one round per thread at 4096 staged values; its 26 twiddles are read per round
+            case 27: fft_stage<27, INV, kFftBL, false>(s, pl.L, nlines, p, tw); break;
             default: fft_stage<8, INV>(s, pl.L, nlines, p, tw); break;
         }
         p *= R;
@@ -1272,7 +1331,7 @@ int vote3_spectral_run(const float* tensors, int in_layout, const float* lam, co
     FftPlan p0, p1, p2;  // z and y: column passes; x: lines in LDS
     make_cols_plan((int)L0, p0);
     make_cols_plan((int)L1, p1);
-    make_plan((int)L2, p2);
+    make_lines_plan((int)L2, p2);
 
     SpecGeom g{};
     g.n0 = (int)s0; g.n1 = (int)s1; g.n2 = (int)s2;
@@ -1390,7 +1449,7 @@ int tira_fft3_async(float* data, unsigned nchan, unsigned L0, unsigned L1, unsig
     TH_ARG(L0 >= 1 && L1 >= 1 && L2 >= 1 && L0 <= (unsigned)kMaxL && L1 <= (unsigned)kMaxL &&
                L2 <= (unsigned)kMaxL,
            "FFT lengths must be in [1, %d]", kMaxL);
-    TH_ARG(make_cols_plan((int)L0, p0) && make_cols_plan((int)L1, p1) && make_plan((int)L2, p2),
+    TH_ARG(make_cols_plan((int)L0, p0) && make_cols_plan((int)L1, p1) && make_lines_plan((int)L2, p2),
            "FFT lengths must have prime factors 2, 3, 5, 7 only");
     TH_ARG(is_device(data), "tira_fft3_async takes a device pointer");
     if (!nchan) return TIRA_OK;
