@@ -170,17 +170,6 @@ template <> __device__ __forceinline__ float dsin<7>(int n) {
                             -0.433883739117558120f, -0.974927912181823607f, -0.781831482468029809f};
     return t[n];
 }
-template <> __device__ __forceinline__ float dcos<8>(int n) {
-    constexpr float t[8] = {1.0f, 0.707106781186547524f, 0.0f, -0.707106781186547524f,
-                            -1.0f, -0.707106781186547524f, 0.0f, 0.707106781186547524f};
-    return t[n];
-}
-template <> __device__ __forceinline__ float dsin<8>(int n) {
-    constexpr float t[8] = {0.0f, 0.707106781186547524f, 1.0f, 0.707106781186547524f,
-                            0.0f, -0.707106781186547524f, -1.0f, -0.707106781186547524f};
-    return t[n];
-}
-
 // In-register DFT of length R: y_k = sum_j u_j W^(jk), W = e^(-+2 pi i / R).
 template <int R, bool INV>
 __device__ __forceinline__ void dft_small(float2 (&u)[R]) {
@@ -292,19 +281,56 @@ __device__ __forceinline__ void dft_small(float2 (&u)[R]) {
 #pragma unroll
             for (int a = 0; a < 3; a++) u[(10 * a + 6 * b) % 15] = t[a];
         }
+    } else if constexpr (R == 8) {
+        // two length-4 DFTs (even and odd inputs), the odd ones turned by W8^k
+        float2 e[4] = {u[0], u[2], u[4], u[6]}, o[4] = {u[1], u[3], u[5], u[7]};
+        dft_small<4, INV>(e);
+        dft_small<4, INV>(o);
+        constexpr float r = 0.707106781186547524f;
+        const float sg = INV ? 1.0f : -1.0f;  // W8 = r (1 + sg i)
+        const float2 w1 = make_float2(r * (o[1].x - sg * o[1].y), r * (o[1].y + sg * o[1].x));
+        const float2 w2 = make_float2(-sg * o[2].y, sg * o[2].x);
+        const float2 w3 = make_float2(r * (-o[3].x - sg * o[3].y), r * (-o[3].y + sg * o[3].x));
+        u[0] = cadd(e[0], o[0]);
+        u[4] = csub(e[0], o[0]);
+        u[1] = cadd(e[1], w1);
+        u[5] = csub(e[1], w1);
+        u[2] = cadd(e[2], w2);
+        u[6] = csub(e[2], w2);
+        u[3] = cadd(e[3], w3);
+        u[7] = csub(e[3], w3);
     } else {
+        // odd R (5, 7): with s_m = u_m + u_(R-m) and d_m = u_m - u_(R-m),
+        // y_k = a_k -+ i b_k and y_(R-k) = a_k +- i b_k, where
+        // a_k = u_0 + sum_m cos(2 pi m k / R) s_m and b_k = sum_m sin(2 pi m k / R) d_m
+        static_assert(R % 2 == 1, "symmetric-pair DFT needs an odd length");
+        constexpr int H = (R - 1) / 2;
+        float2 s[H], d[H];
+        float2 y0 = u[0];
+#pragma unroll
+        for (int m = 1; m <= H; m++) {
+            s[m - 1] = cadd(u[m], u[R - m]);
+            d[m - 1] = csub(u[m], u[R - m]);
+            y0 = cadd(y0, s[m - 1]);
+        }
         float2 y[R];
+        y[0] = y0;
 #pragma unroll
-        for (int k = 0; k < R; k++) {
-            float2 acc = u[0];
+        for (int k = 1; k <= H; k++) {
+            float2 a = u[0], b = make_float2(0.0f, 0.0f);
 #pragma unroll
-            for (int j = 1; j < R; j++) {
-                const int n = (j * k) % R;
-                const float c = dcos<R>(n), s = INV ? dsin<R>(n) : -dsin<R>(n);
-                acc.x = fmaf(u[j].x, c, fmaf(-u[j].y, s, acc.x));
-                acc.y = fmaf(u[j].x, s, fmaf(u[j].y, c, acc.y));
+            for (int m = 1; m <= H; m++) {
+                const int n = (m * k) % R;
+                const float c = dcos<R>(n), sn = dsin<R>(n);
+                a.x = fmaf(c, s[m - 1].x, a.x);
+                a.y = fmaf(c, s[m - 1].y, a.y);
+                b.x = fmaf(sn, d[m - 1].x, b.x);
+                b.y = fmaf(sn, d[m - 1].y, b.y);
             }
-            y[k] = acc;
+            const float2 amib = make_float2(a.x + b.y, a.y - b.x);  // a - i b
+            const float2 apib = make_float2(a.x - b.y, a.y + b.x);  // a + i b
+            y[k] = INV ? apib : amib;
+            y[R - k] = INV ? amib : apib;
         }
 #pragma unroll
         for (int k = 0; k < R; k++) u[k] = y[k];
