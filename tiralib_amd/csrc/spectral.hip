@@ -644,6 +644,7 @@ __global__ __launch_bounds__(kFT) void k_fft_cols(float2* __restrict__ data, Lin
     const int a0 = ab * B;
     const int Bv = min(B, g.na - a0);
     float2* base = data + (size_t)c * g.cstride + (size_t)b * g.bstride + (size_t)a0;
+    if (pl.nst == 0) return;  // length 1: the identity
     if (pl.nst == 1) {
         TH_RADIX_SWITCH_LAST(pl.rad[0], (cols_single<RR, INV>(base, B, Bv, g.estride, g.n_in, g.n_out)));
         return;
@@ -795,11 +796,19 @@ __global__ __launch_bounds__(256) void k_spec_mul(float2* __restrict__ Z, const 
 #pragma unroll
         for (int c = 0; c < 6; c++) Y[sl][c] = make_float2(0.0f, 0.0f);
 
+    // the next channel's eight values are in flight while this one is multiplied
+    float2 zn[8];
+#pragma unroll
+    for (int sl = 0; sl < 8; sl++) zn[sl] = Z[off[sl]];
     for (int j = 0; j < g.nz; j++) {
-        const float2* zc = Z + (size_t)j * g.cs;
         float2 z8[8];
 #pragma unroll
-        for (int sl = 0; sl < 8; sl++) z8[sl] = zc[off[sl]];
+        for (int sl = 0; sl < 8; sl++) z8[sl] = zn[sl];
+        if (j + 1 < g.nz) {
+            const float2* zq = Z + (size_t)(j + 1) * g.cs;
+#pragma unroll
+            for (int sl = 0; sl < 8; sl++) zn[sl] = zq[off[sl]];
+        }
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const int f = 2 * j + h;
