@@ -131,6 +131,18 @@ static bool make_cols_plan(int L, FftPlan& p) {
     return true;
 }
 
+// Whether `nlines` lines of plan `p` fit the register rounds of every stage
+// (fft_stage holds at most MAXQ = ceil((kFftBL / R) / kFT) butterflies per
+// thread); beyond that a stage would skip butterflies.
+static bool lines_fit(const FftPlan& p, int nlines) {
+    for (int i = 0; i < p.nst; i++) {
+        const int R = p.rad[i];
+        const long maxq = (kFftBL / R + kFT - 1) / kFT;
+        if ((long)nlines * (p.L / R) > maxq * kFT) return false;
+    }
+    return true;
+}
+
 static unsigned spectral_length(unsigned n, unsigned w) {
     const unsigned need = n + w / 2;
     for (unsigned L = std::max(need, 1u); L <= (unsigned)kMaxL; L++) {
@@ -702,9 +714,10 @@ __global__ __launch_bounds__(kFT) void k_spec_xfwd(const float4* __restrict__ ps
                                                    const float* __restrict__ tensors, int layout, SpecGeom g, int B,
                                                    int C, FftPlan pl, const float2* __restrict__ twg,
                                                    float2* __restrict__ Z) {
-    __shared__ float2 s[kFftBL];
-    __shared__ float2 tw[kMaxL];
+    extern __shared__ float2 xs_dyn[];  // C B L staged values, then L twiddles
     const int L = pl.L;
+    float2* s = xs_dyn;
+    float2* tw = xs_dyn + C * B * L;
     const int row0 = (int)blockIdx.x * B;
     const int nrows = g.n0 * g.n1;
     if (row0 >= nrows) return;
@@ -1382,7 +1395,12 @@ int vote3_spectral_run(const float* tensors, int in_layout, const float* lam, co
 
     const size_t nvox = (size_t)s0 * s1 * s2;
     const int C = std::min(g.nz, std::max(1, kFftBL / g.L2));
-    const int B = std::max(1, kFftBL / (C * g.L2));
+    int B = std::max(1, kFftBL / (C * g.L2));
+    // More rows than the 4096-value staging holds while every stage still fits
+    // its rounds (the 31 ATV features of 128-wide rows: two rows, 4320 values,
+    // so the eigen-solve keeps all 256 threads busy). Dynamic LDS holds them.
+    while (B < 64 && lines_fit(p2, C * (B + 1)) && ((size_t)C * (B + 1) * g.L2 + g.L2) * 8 <= 65536) B++;
+    const size_t xlds = ((size_t)C * B * g.L2 + g.L2) * 8;
     // one channel group: the x pass decomposes the voxels itself (no payload round trip)
     const bool fused = tensors && C == g.nz;
     if (atv && !fused) return kSpectralNotApplicable;
@@ -1406,8 +1424,10 @@ int vote3_spectral_run(const float* tensors, int in_layout, const float* lam, co
         const size_t nrows = (size_t)s0 * s1;
         const size_t nblk = ceil_div(nrows, (size_t)B);
         TH_ARG(nblk < (1ull << 31), "spectral vote: volume too large");
-        if (fused) k_spec_xfwd<true><<<(unsigned)nblk, kFT, 0, st>>>(ps, pp, tensors, in_layout, g, B, C, p2, tw2, Z);
-        else k_spec_xfwd<false><<<(unsigned)nblk, kFT, 0, st>>>(ps, pp, tensors, in_layout, g, B, C, p2, tw2, Z);
+        if (fused)
+            k_spec_xfwd<true><<<(unsigned)nblk, kFT, xlds, st>>>(ps, pp, tensors, in_layout, g, B, C, p2, tw2, Z);
+        else
+            k_spec_xfwd<false><<<(unsigned)nblk, kFT, xlds, st>>>(ps, pp, tensors, in_layout, g, B, C, p2, tw2, Z);
         TH_HIP(hipGetLastError());
     }
     {
