@@ -1394,12 +1394,20 @@ int vote3_spectral_run(const float* tensors, int in_layout, const float* lam, co
     g.ocs = (size_t)g.h0 * g.h1 * g.h2;
 
     const size_t nvox = (size_t)s0 * s1 * s2;
-    const int C = std::min(g.nz, std::max(1, kFftBL / g.L2));
+    // The fewest channel groups whose lines of one row fit every stage's
+    // register rounds and 64 KB of LDS, balanced (512-wide rows: 8 + 7
+    // channels instead of 7 + 7 + 1, one payload pass less).
+    int C = g.nz;
+    for (int ng = 1; ng <= g.nz; ng++) {
+        C = (g.nz + ng - 1) / ng;
+        if (lines_fit(p2, C) && ((size_t)C * g.L2 + g.L2) * 8 <= 65536) break;
+    }
     int B = std::max(1, kFftBL / (C * g.L2));
-    // More rows than the 4096-value staging holds while every stage still fits
-    // its rounds (the 31 ATV features of 128-wide rows: two rows, 4320 values,
-    // so the eigen-solve keeps all 256 threads busy). Dynamic LDS holds them.
-    while (B < 64 && lines_fit(p2, C * (B + 1)) && ((size_t)C * (B + 1) * g.L2 + g.L2) * 8 <= 65536) B++;
+    // More rows than the 4096-value staging holds, while a block's voxels do
+    // not yet cover its threads and every stage still fits its rounds (the 31
+    // ATV features of 128-wide rows: two rows, 4320 values, so the eigen-solve
+    // keeps all 256 threads busy). Dynamic LDS holds them.
+    while (B * g.n2 < kFT && lines_fit(p2, C * (B + 1)) && ((size_t)C * (B + 1) * g.L2 + g.L2) * 8 <= 65536) B++;
     const size_t xlds = ((size_t)C * B * g.L2 + g.L2) * 8;
     // one channel group: the x pass decomposes the voxels itself (no payload round trip)
     const bool fused = tensors && C == g.nz;
