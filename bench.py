@@ -59,9 +59,9 @@ def spectral_bytes(shape, w, in_bytes=36, out_bytes=24, nf=30):
     L0, L1, L2 = (api.spectral_length(n, w) for n in shape)
     h0, h1, h2 = L0 // 2 + 1, L1 // 2 + 1, L2 // 2 + 1
     nz, N, c = (nf + 1) // 2, n0 * n1 * n2, 8
-    # every channel in one x-pass group (15 x L2 <= 4096 complex values): the x pass
-    # eigen-decomposes the input tensors itself and no payload is written
-    fused = nz <= 4096 // L2
+    # tensor input: the x pass eigen-decomposes the input tensors itself (in every channel
+    # group) and no payload is written
+    fused = True
     return {
         "vote3_prep": 0 if fused else N * (in_bytes + 32),
         "spec_fwd_x": N * (in_bytes if fused else 32) + nz * n0 * n1 * L2 * c,

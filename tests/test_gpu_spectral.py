@@ -93,6 +93,7 @@ CASES = [
     ((3, 2, 40), 3.0, 0.0, 15, True, True),     # window larger than two axes
     ((12, 10, 14), 1.5, 0.0, 7, True, False),
     ((12, 10, 14), 1.5, 0.0, 7, False, True),
+    ((2, 3, 300), 1.5, 0.0, 7, True, True),     # x length 315: two channel groups, each eigen-solving its voxels
 ]
 
 
@@ -111,6 +112,28 @@ def test_spectral_vote_matches_oracle_f64(hip, oracle, shape, s1, s2, w, stick, 
     assert e_vs_direct <= 1e-5, e_vs_direct
     gm = gpu.reshape(-1, 3, 3)
     assert np.array_equal(gm, np.transpose(gm, (0, 2, 1)))
+
+
+def test_spectral_from_eigenpairs(hip):
+    """The eigenpair entry (_tensorvote3_tk, tensorvote.h:2741-2809) at a size
+    where TIRA_PLATE_AUTO takes the spectral vote goes through the voter
+    payload instead of the x pass's own eigen-solve; it agrees with the direct
+    gather of the same field."""
+    shape = (32, 32, 34)
+    f = _random_field(shape, seed=77)
+    lam64, vec = np.linalg.eigh(f.reshape(-1, 3, 3).astype(np.float64))
+    lam = lam64.astype(np.float32)
+    e0 = np.ascontiguousarray(vec[:, :, 0], np.float32)
+    e2 = np.ascontiguousarray(vec[:, :, 2], np.float32)
+    call("tira_hip_profile_begin")
+    got = api.tensorvote3_tk(lam, e2, e0, shape, 1.5, 0.0, 1, 9, samples=20)
+    call("tira_hip_profile_end")
+    assert _lib.profile_get("spec_mul")[1] == 1 and _lib.profile_get("vote3_prep")[1] == 1
+    direct = _vote(f, shape, 1.5, 0.0, 9, _lib.PLATE_CLOSED)
+    g, d = np.asarray(got).reshape(-1, 9)[:, SLOTS], direct.reshape(-1, 9)[:, SLOTS]
+    e = rel_frob(g, d)
+    print(f"eigenpair input, spectral vs direct gather: {e:.2e}")
+    assert e <= 1e-5, e
 
 
 def test_spectral_layouts_and_tubes(hip, oracle):
@@ -201,7 +224,7 @@ def _atv(field, shape, sigma, w, method, in_layout=_lib.LAYOUT_MAT9, out_layout=
 
 
 @pytest.mark.parametrize("shape,sigma,w", [((20, 22, 24), 2.0, 13), ((17, 9, 31), 1.5, 7), ((3, 2, 40), 3.0, 15),
-                                           ((12, 13, 14), 1.0, 6)])
+                                           ((12, 13, 14), 1.0, 6), ((2, 3, 300), 1.5, 7)])
 def test_spectral_atv_matches_oracle_f64(hip, oracle, shape, sigma, w):
     """Analytical tensor voting in the spectral form (31 features) against the
     FP64 oracle (tira::atv restated, tensorvote.h:688-760, 1443-1495) and the

@@ -702,13 +702,14 @@ __device__ __forceinline__ void mono15(const float4 P, float* __restrict__ A) {
 }
 
 // Forward x pass fused with the features. A block takes B rows (z, y) of the
-// field and the complex channels in groups of C (C B L2 <= kFftBL): every
-// element computes all features of its voxel from one payload read, channel
-// j = features (2j, 2j+1) as (real, imaginary), lines (channel, row) are
-// transformed together. Groups after the first re-read the payload from cache.
+// field and the complex channels in groups of C: every element computes all
+// features of its voxel from one payload read, channel j = features
+// (2j, 2j+1) as (real, imaginary), lines (channel, row) are transformed
+// together. Each group re-reads its voxels.
 // FROM_TENSORS: each element eigen-decomposes its voxel of the input field
-// (layout) instead of reading the payload (one channel group only, so every
-// voxel is decomposed once).
+// (layout) instead of reading a payload, once per channel group (wide rows
+// take two or more groups: a second eigen-solve costs less than writing and
+// re-reading a 32-byte payload per voxel).
 template <bool FROM_TENSORS>
 __global__ __launch_bounds__(kFT) void k_spec_xfwd(const float4* __restrict__ ps, const float4* __restrict__ pp,
                                                    const float* __restrict__ tensors, int layout, SpecGeom g, int B,
@@ -1338,9 +1339,8 @@ static LineGeom strided_geom(size_t cs, int nchan, int L, size_t estride, int na
 // a volume of at least 32^3 voxels (below that the gather costs microseconds
 // and the first call's spectra build would dominate), and device memory for
 // the spectra, their build and the feature volumes within 80% of the device.
-// With `atv`: analytical tensor voting, whose 31 features (16 complex
-// channels) must fit one x-pass group (L2 <= 256), as its features come from
-// the input tensors in the x pass only.
+// With `atv`: analytical tensor voting (31 features, 16 complex channels),
+// whose features come from the input tensors in the x pass only.
 bool spectral_preferred(unsigned s0, unsigned s1, unsigned s2, unsigned w, unsigned power, bool stick, bool plate,
                         unsigned samples, bool atv) {
     const unsigned p = power < 1 ? 1 : power;
@@ -1349,7 +1349,6 @@ bool spectral_preferred(unsigned s0, unsigned s1, unsigned s2, unsigned w, unsig
     const unsigned L0 = spectral_length(s0, w), L1 = spectral_length(s1, w), L2 = spectral_length(s2, w);
     if (!L0 || !L1 || !L2 || 3 * L2 > (unsigned)kFftBL) return false;
     const size_t nf = atv ? 2 * kMono + 1 : (stick ? kMono : 0) + (plate ? kMono : 0), nz = (nf + 1) / 2;
-    if (atv && nz * L2 > (size_t)kFftBL) return false;
     const size_t W = 2 * (w / 2) + 1;
     const size_t h0 = L0 / 2 + 1, h1 = L1 / 2 + 1, h2 = L2 / 2 + 1;
     const size_t need = nz * (size_t)L0 * L1 * L2 * 8 + 6 * nf * h0 * h1 * h2 * 4 + 6 * nf * W * h1 * h2 * 8 +
@@ -1409,9 +1408,9 @@ int vote3_spectral_run(const float* tensors, int in_layout, const float* lam, co
     // keeps all 256 threads busy). Dynamic LDS holds them.
     while (B * g.n2 < kFT && lines_fit(p2, C * (B + 1)) && ((size_t)C * (B + 1) * g.L2 + g.L2) * 8 <= 65536) B++;
     const size_t xlds = ((size_t)C * B * g.L2 + g.L2) * 8;
-    // one channel group: the x pass decomposes the voxels itself (no payload round trip)
-    const bool fused = tensors && C == g.nz;
-    if (atv && !fused) return kSpectralNotApplicable;
+    // tensor input: the x pass decomposes the voxels itself, in every channel
+    // group (no payload round trip); eigenpair input goes through the payload
+    const bool fused = tensors != nullptr;
     float4 *ps = nullptr, *pp = nullptr;
     float2* Z = nullptr;
     if (stick && !fused) TH_TRY(ws_get(WS_PAY0, nvox * 16, (void**)&ps));
