@@ -194,7 +194,7 @@ int tira_tensorvote3_atv(float* output_field, const float* lambdas, const float*
                          unsigned s0, unsigned s1, unsigned s2, float sigma, unsigned w);
 /* Native 3D analytical tensor voting on device buffers, non-blocking on the
  * library stream (layouts TIRA_LAYOUT_*). The spectral form is taken when it
- * applies (w >= 5, at least 32^3 voxels, x-rows whose FFT length is at most 256). */
+ * applies (w >= 5, at least 32^3 voxels, FFT lengths of at most 1024). */
 int tira_tensorvote3_atv_async(const float* input_field, int in_layout, float* output_field, int out_layout,
                                unsigned s0, unsigned s1, unsigned s2, float sigma, unsigned w);
 /* The same with the method named: TIRA_PLATE_AUTO, TIRA_PLATE_CLOSED (the
