@@ -238,11 +238,15 @@ __global__ __launch_bounds__(256) void k_thin_candidates_list(const unsigned cha
 // candidate p is deleted (mark 1) when it is simple in the volume where the
 // candidates before it in raster order (neighbour bits 0..12) that are
 // currently marked for deletion are removed; otherwise kept (mark 2).
+// The candidate count is read from device memory (the candidate pass wrote
+// it), so the host launches over an upper bound without reading it back.
 __global__ __launch_bounds__(256) void k_thin_resolve(const unsigned char* __restrict__ v, int s0, int s1, int s2,
-                                                      const unsigned* __restrict__ list, unsigned cnt,
+                                                      const unsigned* __restrict__ list,
+                                                      const unsigned* __restrict__ count,
                                                       const unsigned char* __restrict__ mark_in,
                                                       unsigned char* __restrict__ mark_out,
                                                       unsigned* __restrict__ changes) {
+    const unsigned cnt = *count;
     const unsigned stride = gridDim.x * blockDim.x;
     const unsigned iters = (cnt + stride - 1) / stride;
     for (unsigned it = 0; it < iters; it++) {
@@ -271,9 +275,11 @@ __global__ __launch_bounds__(256) void k_thin_resolve(const unsigned char* __res
 
 // Apply the decisions: delete marked candidates, clear both mark buffers.
 __global__ __launch_bounds__(256) void k_thin_apply(unsigned char* __restrict__ v, const unsigned* __restrict__ list,
-                                                    unsigned cnt, unsigned char* __restrict__ mark_a,
+                                                    const unsigned* __restrict__ count,
+                                                    unsigned char* __restrict__ mark_a,
                                                     unsigned char* __restrict__ mark_b, const int final_is_a,
                                                     unsigned* __restrict__ deleted) {
+    const unsigned cnt = *count;
     const unsigned stride = gridDim.x * blockDim.x;
     const unsigned iters = (cnt + stride - 1) / stride;
     for (unsigned it = 0; it < iters; it++) {
@@ -467,51 +473,56 @@ int lee_run(unsigned char* v, unsigned s0, unsigned s1, unsigned s2, unsigned* i
     k_thin_zero<<<1, 64, 0, st>>>(cnt + 3, 1);
     k_thin_objects<<<grid_for(n), 256, 0, st>>>(v, n, nullptr, 0, objA, cnt + 3);
     TH_HIP(hipGetLastError());
-    int unchanged = 0, iters = 0;
-    while (unchanged < 6) {
-        unchanged = 0;
+    // Counters: [0] candidates of the current direction, [1] [2] changes of the
+    // two sweeps of a batch, [3] object voxels, [8 + dir] deletions of this
+    // outer pass. The re-check and apply passes read the candidate count on
+    // the device (launched over the object count, an upper bound), so a
+    // direction needs one read-back per two sweeps and an outer pass one more.
+    int iters = 0;
+    unsigned host_pass[7];
+    for (;;) {
         iters++;
-        bool deleted_any = false;
+        k_thin_zero<<<1, 64, 0, st>>>(cnt + 8, 6);
         for (int dir = 0; dir < 6; dir++) {
             k_thin_zero<<<1, 64, 0, st>>>(cnt, 3);
             k_thin_candidates_list<<<grid_for(nobj), 256, 0, st>>>(v, (int)s0, (int)s1, (int)s2, dir, objA, nobj, ma,
                                                                    list, cnt);
             TH_HIP(hipGetLastError());
-            TH_HIP(hipMemcpyAsync(host, cnt, 4, hipMemcpyDeviceToHost, st));
-            TH_HIP(hipStreamSynchronize(st));
-            const unsigned c = host[0];
-            if (c == 0) { unchanged++; continue; }
             unsigned char *in = ma, *out = mb;
-            for (unsigned sweep = 0;; sweep++) {
-                // bounded: each sweep settles at least one more level of the raster-order DAG
-                TH_ARG(sweep <= c, "thinning re-check did not settle (internal error)");
-                k_thin_zero<<<1, 64, 0, st>>>(cnt + 1, 1);
-                k_thin_resolve<<<grid_for(c), 256, 0, st>>>(v, (int)s0, (int)s1, (int)s2, list, c, in, out, cnt + 1);
+            for (unsigned sweep = 0;; sweep += 2) {
+                // bounded: each sweep settles at least one more level of the raster-order DAG, and
+                // there are at most nobj candidates; a sweep that changes nothing is a fixed point,
+                // so the second sweep of a batch may repeat a settled state harmlessly
+                TH_ARG(sweep <= nobj + 2, "thinning re-check did not settle (internal error)");
+                k_thin_zero<<<1, 64, 0, st>>>(cnt + 1, 2);
+                for (int b = 0; b < 2; b++) {
+                    k_thin_resolve<<<grid_for(nobj), 256, 0, st>>>(v, (int)s0, (int)s1, (int)s2, list, cnt, in, out,
+                                                                   cnt + 1 + b);
+                    std::swap(in, out);
+                }
                 TH_HIP(hipGetLastError());
-                TH_HIP(hipMemcpyAsync(host + 1, cnt + 1, 4, hipMemcpyDeviceToHost, st));
+                TH_HIP(hipMemcpyAsync(host + 2, cnt + 2, 4, hipMemcpyDeviceToHost, st));
                 TH_HIP(hipStreamSynchronize(st));
-                std::swap(in, out);
-                if (host[1] == 0) break;
+                if (host[2] == 0) break;
             }
-            k_thin_apply<<<grid_for(c), 256, 0, st>>>(v, list, c, ma, mb, in == ma ? 1 : 0, cnt + 2);
+            k_thin_apply<<<grid_for(nobj), 256, 0, st>>>(v, list, cnt, ma, mb, in == ma ? 1 : 0, cnt + 8 + dir);
             TH_HIP(hipGetLastError());
-            TH_HIP(hipMemcpyAsync(host + 2, cnt + 2, 4, hipMemcpyDeviceToHost, st));
-            TH_HIP(hipStreamSynchronize(st));
-            if (host[2] == 0) unchanged++;
-            else deleted_any = true;
         }
-        if (deleted_any && unchanged < 6) {  // drop deleted voxels from the object list
-            k_thin_zero<<<1, 64, 0, st>>>(cnt + 3, 1);
-            k_thin_objects<<<grid_for(nobj), 256, 0, st>>>(v, n, objA, nobj, objB, cnt + 3);
-            TH_HIP(hipGetLastError());
-            TH_HIP(hipMemcpyAsync(host + 3, cnt + 3, 4, hipMemcpyDeviceToHost, st));
-            TH_HIP(hipStreamSynchronize(st));
-            nobj = host[3];
-            std::swap(objA, objB);
-            if (nobj == 0) {  // the next outer pass would find no candidate in any direction
-                iters++;
-                break;
-            }
+        // drop deleted voxels from the object list; read it with this pass's deletions
+        k_thin_zero<<<1, 64, 0, st>>>(cnt + 3, 1);
+        k_thin_objects<<<grid_for(nobj), 256, 0, st>>>(v, n, objA, nobj, objB, cnt + 3);
+        TH_HIP(hipGetLastError());
+        TH_HIP(hipMemcpyAsync(host + 3, cnt + 3, 4, hipMemcpyDeviceToHost, st));
+        TH_HIP(hipMemcpyAsync(host_pass, cnt + 8, 6 * 4, hipMemcpyDeviceToHost, st));
+        TH_HIP(hipStreamSynchronize(st));
+        bool deleted_any = false;
+        for (int d = 0; d < 6; d++) deleted_any = deleted_any || host_pass[d] != 0;
+        if (!deleted_any) break;  // a whole pass of six directions deleted nothing
+        nobj = host[3];
+        std::swap(objA, objB);
+        if (nobj == 0) {  // the next outer pass would find no candidate in any direction
+            iters++;
+            break;
         }
     }
     if (iterations) *iterations = (unsigned)iters;
