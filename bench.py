@@ -47,7 +47,7 @@ def profiled_traffic(kernel):
 SPECTRAL_KERNELS = ("spec_fwd_x", "spec_fwd_y", "spec_fwd_z", "spec_mul", "spec_inv_z", "spec_inv_y", "spec_inv_x")
 
 
-def spectral_bytes(shape, w, in_bytes=36, out_bytes=24, nf=30):
+def spectral_bytes(shape, w, in_bytes=36, out_bytes=24, nf=15):
     """Algorithmic HBM bytes per launch of each kernel of the spectral vote:
     every array a pass reads or writes, once (zero padding that the passes
     skip is not counted). Features: nf real channels packed two per complex
@@ -74,7 +74,7 @@ def spectral_bytes(shape, w, in_bytes=36, out_bytes=24, nf=30):
     }
 
 
-def spectral_roofline(shape, w, stage_ms, in_bytes=36, nf=30):
+def spectral_roofline(shape, w, stage_ms, in_bytes=36, nf=15):
     """Per-kernel achieved HBM bandwidth of the spectral vote and the dominant kernel."""
     b = spectral_bytes(shape, w, in_bytes, nf=nf)
     per = {k: {"ms": round(stage_ms[k], 4), "bytes": int(b[k]),
@@ -637,7 +637,7 @@ def bench_atv(d_in, field, shape, sigma, w, steps, warmup, cpu_seconds=None):
     # the default (TIRA_PLATE_AUTO): the spectral form at this size
     ms = run(_lib.PLATE_AUTO)
     stage_ms = {k: _lib.profile_get(k)[0] for k in SPECTRAL_KERNELS}
-    per, dom, tot = spectral_roofline(shape, w, stage_ms, in_bytes=36, nf=31)
+    per, dom, tot = spectral_roofline(shape, w, stage_ms, in_bytes=36, nf=15)
     spec_out = d_out.to_host()
     # the per-pair gather
     ms_g = run(_lib.PLATE_CLOSED)

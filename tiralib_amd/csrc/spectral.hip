@@ -11,14 +11,19 @@
 // On the unit sphere the even polynomials of degree <= 4 span exactly the 15
 // quartic monomials q_x^a q_y^b q_z^c (a + b + c = 4), so
 //     V_c(x) = sum_f sum_d K_{c,f}(d) F_f(x + d),      c = xx, xy, xz, yy, yz, zz,
-// with the voter features F_f = scale * monomial_f (15 of e2 for the stick,
-// 15 of e0 for the plate) and offset kernels K_{c,f}(d) fitted once on the
-// host in FP64. The whole field's vote is a 30-in / 6-out correlation with a
-// w^3 window: O(N log N) through FFTs instead of O(N w^3) pair evaluations.
+// with offset kernels K_{c,f}(d) of the stick vote fitted once on the host in
+// FP64. The plate vote integrates the stick vote over the great circle
+// perpendicular to n, and averaging a quartic over that circle is a fixed
+// linear map on the 15 monomials (add_circle_average), so the plate kernels
+// are the stick kernels times that map and both votes share them: the voter
+// features are F_f = s m_f(e2) + sticknorm3 t (circle average of m(e0))_f.
+// The whole field's vote is a 15-in / 6-out correlation with a w^3 window:
+// O(N log N) through FFTs instead of O(N w^3) pair evaluations.
 //
 // Pipeline of one call (device buffers, zero padding to L >= n + w/2 per axis
 // so the circular convolution equals the reference's bounded window):
-//  1. payload: eigen-decomposition of every voxel (shared with the gather path);
+//  1. payload: eigen-decomposition of every voxel, for eigenpair input only
+//     (tensor input is eigen-solved in the x pass);
 //  2. forward x pass fused with the features: rows of the field only, two
 //     real features packed as one complex line;
 //  3. forward y pass (planes z < n0), forward z pass (all columns);
@@ -681,8 +686,10 @@ struct SpecGeom {
     int n0, n1, n2;       // field (n2 fastest)
     int L0, L1, L2;       // FFT lengths
     int h0, h1, h2;       // octant extents L/2 + 1
-    int nf, nz, nst;      // real features, complex channels, stick features (15 or 0)
-    int atv;              // analytical tensor voting: features s1 m(e2), s2 m(e0), s3 (nf = 31)
+    int nf, nz;           // real features of the product (the 15 combined quartic features), complex channels
+    int stick, plate;     // which votes the features carry
+    int atv;              // analytical tensor voting: s1 m(e2) + 2 s2 circle(e0) + 3 s3 sphere
+    float alpha;          // scale of the plate part: sticknorm3 (vote) or 2 (ATV)
     int P2;               // row stride in complex values (L2 rounded up to 16)
     size_t cs;            // complex values per channel volume
     size_t ocs;           // kernel-spectrum values per channel (octant)
@@ -699,6 +706,34 @@ __device__ __forceinline__ void mono15(const float4 P, float* __restrict__ A) {
     A[6] = sx * y * y2;   A[7] = sx * y * yz;   A[8] = sx * y * z2;   A[9] = sx * z * z2;
     A[10] = sy * y * y2;  A[11] = sy * y * yz;  A[12] = sy * y * z2;  A[13] = sy * z * z2;
     A[14] = sz * z * z2;
+}
+
+// Circle average over the great circle perpendicular to n of each quartic
+// monomial, as a quartic in n: avg_f(n) = sum_g A[f][g] n^g with exact
+// rational A (42 nonzero terms). The plate vote of normal n integrates the
+// stick vote over that circle, so its offset kernels are the stick kernels
+// times sticknorm3 A and its features fold into the stick features:
+// F_f += sticknorm3 t sum_g A[f][g] m_g(e0). The ATV's e0 part is likewise
+// twice the circle average of its e2 part and its constant part three times
+// the sphere average. N.w carries the scale (t times sticknorm3, or 2 s2).
+struct CircleTerm {
+    int f, g;
+    float a;
+};
+__device__ __forceinline__ void add_circle_average(const float4 N, float* __restrict__ A) {
+    float B[kMono];
+    mono15(N, B);
+    constexpr CircleTerm t[42] = {
+        {0, 10, 0.375f}, {0, 12, 0.75f}, {0, 14, 0.375f}, {1, 6, -0.375f}, {1, 8, -0.375f}, {2, 7, -0.375f},
+        {2, 9, -0.375f}, {3, 3, 0.375f}, {3, 5, 0.125f}, {3, 12, 0.125f}, {3, 14, 0.125f}, {4, 4, 0.25f},
+        {4, 11, -0.125f}, {4, 13, -0.125f}, {5, 3, 0.125f}, {5, 5, 0.375f}, {5, 10, 0.125f},
+        {5, 12, 0.125f}, {6, 1, -0.375f}, {6, 8, -0.375f}, {7, 2, -0.125f}, {7, 7, 0.25f}, {7, 9, -0.125f},
+        {8, 1, -0.125f}, {8, 6, -0.125f}, {8, 8, 0.25f}, {9, 2, -0.375f}, {9, 7, -0.375f}, {10, 0, 0.375f},
+        {10, 5, 0.75f}, {10, 14, 0.375f}, {11, 4, -0.375f}, {11, 13, -0.375f}, {12, 0, 0.125f},
+        {12, 3, 0.125f}, {12, 5, 0.125f}, {12, 12, 0.375f}, {13, 4, -0.375f}, {13, 11, -0.375f},
+        {14, 0, 0.375f}, {14, 3, 0.75f}, {14, 10, 0.375f}};
+#pragma unroll
+    for (int i = 0; i < 42; i++) A[t[i].f] = fmaf(t[i].a, B[t[i].g], A[t[i].f]);
 }
 
 // Forward x pass fused with the features. A block takes B rows (z, y) of the
@@ -724,7 +759,6 @@ __global__ __launch_bounds__(kFT) void k_spec_xfwd(const float4* __restrict__ ps
     if (row0 >= nrows) return;
     const int Bv = min(B, nrows - row0);
     load_twiddles(tw, twg, L);
-    const bool two = g.nf >= 2 * kMono;  // stick and plate (or both ATV subspaces)
     const int npad = L - g.n2;
     // index splits below use fdiv (a float reciprocal), not integer division
     const float rB = 1.0f / (float)Bv, rn1 = 1.0f / (float)g.n1, rn2 = 1.0f / (float)g.n2,
@@ -740,31 +774,40 @@ __global__ __launch_bounds__(kFT) void k_spec_xfwd(const float4* __restrict__ ps
         }
         for (int idx = (int)threadIdx.x; idx < Bv * g.n2; idx += kFT) {
             const int b = fdiv(idx, g.n2, rn2), e = idx - b * g.n2;
-            float A[2 * kMono + 2];
+            float A[kMono + 1];  // the 15 combined features and a zero (the last channel's imaginary part)
 #pragma unroll
-            for (int f = 0; f < 2 * kMono + 2; f++) A[f] = 0.0f;
+            for (int f = 0; f <= kMono; f++) A[f] = 0.0f;
             {
                 const size_t vox = (size_t)(row0 + b) * g.n2 + e;
-                float4 S, N;
+                float4 S = make_float4(0.0f, 0.0f, 0.0f, 0.0f), N = S;
+                float s3 = 0.0f;
                 if (FROM_TENSORS) {
                     float l0, l1, l2, e0[3], e2[3];
                     voxel_eig<false>(tensors, layout, vox, l0, l1, l2, e0, e2);
                     if (g.atv) {  // (e2, l2 - l1), (e0, l1 - l0), l0 (the ATV eigen-subspace weights)
                         S = make_float4(e2[0], e2[1], e2[2], l2 - l1);
                         N = make_float4(e0[0], e0[1], e0[2], l1 - l0);
-                        A[2 * kMono] = l0;
+                        s3 = l0;
                     } else {
                         stick_plate_payload(l0, l1, l2, e0, e2, S, N);
                     }
                 } else {
-                    if (g.nst) S = ps[vox];
-                    if (g.nf > g.nst) N = pp[vox];
+                    if (g.stick) S = ps[vox];
+                    if (g.plate) N = pp[vox];
                 }
-                mono15(g.nst ? S : N, A);
-                if (two) mono15(N, A + kMono);
+                if (g.stick) mono15(S, A);
+                if (g.plate) add_circle_average(make_float4(N.x, N.y, N.z, N.w * g.alpha), A);
+                if (g.atv) {  // three times the sphere averages: 1/5 for x^4, 1/15 for x^2 y^2
+                    A[0] = fmaf(0.6f, s3, A[0]);
+                    A[3] = fmaf(0.2f, s3, A[3]);
+                    A[5] = fmaf(0.2f, s3, A[5]);
+                    A[10] = fmaf(0.6f, s3, A[10]);
+                    A[12] = fmaf(0.2f, s3, A[12]);
+                    A[14] = fmaf(0.6f, s3, A[14]);
+                }
             }
 #pragma unroll
-            for (int j = 0; j < kMono + 1; j++)
+            for (int j = 0; j < (kMono + 1) / 2; j++)
                 if (j >= j0 && j < j0 + Cv) s[((j - j0) * Bv + b) * L + e] = make_float2(A[2 * j], A[2 * j + 1]);
         }
         __syncthreads();
@@ -1232,7 +1275,18 @@ static int spectra_for(const SpecKey& key, const SpecGeom& g, hipStream_t st, co
     // offset kernels, parities and trig tables
     std::vector<double> K;
     int nf = 0;
-    TH_TRY(offset_kernels((unsigned)key.w, key.s1, key.s2, key.stick != 0, key.plate != 0, key.atv != 0, K, nf));
+    // the stick (ATV: e2) part only: the plate part, and the ATV's e0 and
+    // constant parts, are linear maps of it and fold into the features
+    TH_TRY(offset_kernels((unsigned)key.w, key.s1, key.s2, true, false, key.atv != 0, K, nf));
+    if (nf > kMono) {  // ATV: keep the first 15 features of every component
+        const size_t noff = K.size() / ((size_t)6 * nf);
+        std::vector<double> K15((size_t)6 * kMono * noff);
+        for (int c = 0; c < 6; c++)
+            std::copy(K.begin() + (size_t)c * nf * noff, K.begin() + ((size_t)c * nf + kMono) * noff,
+                      K15.begin() + (size_t)c * kMono * noff);
+        K.swap(K15);
+        nf = kMono;
+    }
     const int w = 2 * (key.w / 2) + 1, r = key.w / 2;
     const int nch = 6 * nf;
     std::vector<unsigned char> par(nch);
@@ -1348,7 +1402,7 @@ bool spectral_preferred(unsigned s0, unsigned s1, unsigned s2, unsigned w, unsig
     if (w < 5 || (size_t)s0 * s1 * s2 < 32768) return false;
     const unsigned L0 = spectral_length(s0, w), L1 = spectral_length(s1, w), L2 = spectral_length(s2, w);
     if (!L0 || !L1 || !L2 || 3 * L2 > (unsigned)kFftBL) return false;
-    const size_t nf = atv ? 2 * kMono + 1 : (stick ? kMono : 0) + (plate ? kMono : 0), nz = (nf + 1) / 2;
+    const size_t nf = kMono, nz = (nf + 1) / 2;  // the combined features
     const size_t W = 2 * (w / 2) + 1;
     const size_t h0 = L0 / 2 + 1, h1 = L1 / 2 + 1, h2 = L2 / 2 + 1;
     const size_t need = nz * (size_t)L0 * L1 * L2 * 8 + 6 * nf * h0 * h1 * h2 * 4 + 6 * nf * W * h1 * h2 * 8 +
@@ -1384,10 +1438,12 @@ int vote3_spectral_run(const float* tensors, int in_layout, const float* lam, co
     g.n0 = (int)s0; g.n1 = (int)s1; g.n2 = (int)s2;
     g.L0 = (int)L0; g.L1 = (int)L1; g.L2 = (int)L2;
     g.h0 = g.L0 / 2 + 1; g.h1 = g.L1 / 2 + 1; g.h2 = g.L2 / 2 + 1;
-    g.nf = (stick ? kMono : 0) + (plate ? kMono : 0) + (atv ? 1 : 0);
+    g.nf = kMono;  // stick, plate and ATV parts combined into 15 quartic features
     g.nz = (g.nf + 1) / 2;
-    g.nst = stick ? kMono : 0;
+    g.stick = stick ? 1 : 0;
+    g.plate = plate ? 1 : 0;
     g.atv = atv ? 1 : 0;
+    g.alpha = atv ? 2.0f : sticknorm3_host(sigma1, sigma2, 1);
     g.P2 = ((int)L2 + 15) & ~15;  // rows start on 128-byte lines
     g.cs = (size_t)L0 * L1 * g.P2;
     g.ocs = (size_t)g.h0 * g.h1 * g.h2;
@@ -1416,7 +1472,8 @@ int vote3_spectral_run(const float* tensors, int in_layout, const float* lam, co
     if (stick && !fused) TH_TRY(ws_get(WS_PAY0, nvox * 16, (void**)&ps));
     if (plate && !fused) TH_TRY(ws_get(WS_PAY1, nvox * 16, (void**)&pp));
     TH_TRY(ws_get(WS_SPEC0, (size_t)g.nz * g.cs * 8, (void**)&Z));
-    const SpecKey key{g.L0, g.L1, g.L2, (int)w, sigma1, sigma2, stick ? 1 : 0, plate ? 1 : 0, atv ? 1 : 0};
+    // the spectra are those of the stick part whatever the stick / plate selection
+    const SpecKey key{g.L0, g.L1, g.L2, (int)w, sigma1, sigma2, 1, 1, atv ? 1 : 0};
     const float2 *tw0, *tw1, *tw2;
     const float* kh;
     TH_TRY(spectra_for(key, g, st, &tw0, &tw1, &tw2, &kh));
