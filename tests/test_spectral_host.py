@@ -123,3 +123,59 @@ def test_atv_offset_kernels_reproduce_pair_votes(oracle, w, sigma):
             worst = max(worst, np.abs(got - ref).max())
             scale = max(scale, np.abs(ref).max())
     assert worst <= 1e-12 * max(scale, 1.0), (worst, scale)
+
+
+def _circle_average_map():
+    """A[f, g] with mean over the great circle perpendicular to n of monomial f
+    = sum_g A[f, g] n^g: a 16-point rule on the circle (exact for the degree-4
+    trigonometric polynomials), projected onto the monomials at random unit n."""
+    rng = np.random.default_rng(0)
+    ns = rng.normal(size=(64, 3))
+    ns /= np.linalg.norm(ns, axis=1, keepdims=True)
+    avg, mn = [], []
+    for n in ns:
+        a = np.array([1.0, 0.0, 0.0]) if abs(n[0]) < 0.9 else np.array([0.0, 1.0, 0.0])
+        u = np.cross(n, a)
+        u /= np.linalg.norm(u)
+        v = np.cross(n, u)
+        b = np.arange(16) * 2.0 * np.pi / 16
+        avg.append(np.mean([monos(np.cos(t) * u + np.sin(t) * v) for t in b], axis=0))
+        mn.append(monos(n))
+    A = np.linalg.lstsq(np.array(mn), np.array(avg), rcond=None)[0].T
+    assert np.abs(np.array(mn) @ A.T - np.array(avg)).max() < 1e-13
+    return A
+
+
+def _flat(K, f0, f1):  # rows (component, offset), columns features f0..f1-1
+    return K[:, f0:f1].transpose(0, 2, 3, 4, 1).reshape(-1, f1 - f0)
+
+
+@pytest.mark.parametrize("w,s1,s2", [(13, 2.0, 0.0), (9, 1.5, 0.7), (5, 0.4, 0.0)])
+def test_plate_kernels_are_circle_averages_of_stick_kernels(oracle, w, s1, s2):
+    """The identity the 15-feature spectral vote rests on: the plate vote of
+    normal n integrates the stick vote over the great circle perpendicular to n
+    (tensorvote.h:1651-1695), so the plate offset kernels are the stick ones
+    times sticknorm3 A, A the circle average of the quartic monomials. A is
+    built here from its definition, independently of the library; it has 42
+    nonzero entries, all multiples of 1/8 (the table the x pass applies)."""
+    A = _circle_average_map()
+    assert np.allclose(A * 8, np.round(A * 8), atol=1e-12) and np.count_nonzero(np.round(A * 8)) == 42
+    K = api.spectral_offset_kernels(w, s1, s2)
+    sn = oracle.sticknorm3(float(np.float32(s1)), float(np.float32(s2)), 1)
+    Ks, Kp = _flat(K, 0, 15), _flat(K, 15, 30)
+    assert np.abs(Ks @ (sn * A) - Kp).max() <= 1e-12 * np.abs(Kp).max()
+
+
+@pytest.mark.parametrize("w,sigma", [(13, 2.0), (6, 1.0)])
+def test_atv_parts_fold_into_the_e2_part(w, sigma):
+    """Analytical voting: the (l1 - l0) e0 part is twice the circle average of
+    the (l2 - l1) e2 part and the l0 part three times its sphere average
+    (x^4: 1/5, x^2 y^2: 1/15), so all 31 features fold into 15."""
+    A = _circle_average_map()
+    mu = np.array([1 / 5 if 4 in (a, b, c) else (1 / 15 if sorted((a, b, c)) == [0, 2, 2] else 0.0)
+                   for a, b, c in MONO])
+    K = api.spectral_offset_kernels_atv(w, sigma)
+    K1, K2, K3 = _flat(K, 0, 15), _flat(K, 15, 30), _flat(K, 30, 31)[:, 0]
+    scale = np.abs(K).max()
+    assert np.abs(K1 @ (2 * A) - K2).max() <= 1e-12 * scale
+    assert np.abs(K1 @ (3 * mu) - K3).max() <= 1e-12 * scale
