@@ -1457,12 +1457,22 @@ int vote3_spectral_run(const float* tensors, int in_layout, const float* lam, co
         C = (g.nz + ng - 1) / ng;
         if (lines_fit(p2, C) && ((size_t)C * g.L2 + g.L2) * 8 <= 65536) break;
     }
-    int B = std::max(1, kFftBL / (C * g.L2));
-    // More rows than the 4096-value staging holds, while a block's voxels do
-    // not yet cover its threads and every stage still fits its rounds (the 31
-    // ATV features of 128-wide rows: two rows, 4320 values, so the eigen-solve
-    // keeps all 256 threads busy). Dynamic LDS holds them.
-    while (B * g.n2 < kFT && lines_fit(p2, C * (B + 1)) && ((size_t)C * (B + 1) * g.L2 + g.L2) * 8 <= 65536) B++;
+    // Rows per block: among the counts whose lines fit every stage's rounds
+    // and 64 KB of dynamic LDS, the one whose voxels best fill whole passes
+    // of the block's threads in the eigen-solve (ties: more rows). 128-wide
+    // rows of 8 channels take 4 rows (512 voxels, 4320 values), not the 3
+    // that fit 4096 values (384 voxels: half the threads solving twice).
+    int B = 1;
+    {
+        double best = -1.0;
+        for (int b = 1; b <= 64; b++) {
+            if (!lines_fit(p2, C * b) || ((size_t)C * b * g.L2 + g.L2) * 8 > 65536) break;
+            const int vox = b * g.n2, passes = (vox + kFT - 1) / kFT;
+            const double fill = (double)vox / ((double)passes * kFT);
+            if (fill >= best) { best = fill; B = b; }
+            if (vox >= 4 * kFT) break;
+        }
+    }
     const size_t xlds = ((size_t)C * B * g.L2 + g.L2) * 8;
     // tensor input: the x pass decomposes the voxels itself, in every channel
     // group (no payload round trip); eigenpair input goes through the payload
