@@ -1531,8 +1531,22 @@ int vote3_spectral_run(const float* tensors, int in_layout, const float* lam, co
     }
     {
         ProfScope prof("spec_inv_x", st);
-        const int B = std::max(1, kFftBL / (3 * g.L2));
+        // Rows per block: up to the staging's 4096 values, the count whose grid
+        // best fills whole rounds of the resident blocks (four per CU). 128^2
+        // rows of L = 135 take 8 rows (2048 blocks, two full rounds), not 10
+        // (1639 blocks: a second round 60% full).
         const size_t nrows = (size_t)s0 * s1;
+        const int Bmax = std::max(1, kFftBL / (3 * g.L2));
+        const double slots = 4.0 * cu_count();
+        int B = Bmax;
+        {
+            double best = -1.0;
+            for (int b = Bmax; b >= std::max(1, Bmax / 2); b--) {
+                const double rounds = (double)ceil_div(nrows, (size_t)b) / slots;
+                const double fill = rounds / std::ceil(rounds);
+                if (fill > best + 0.02) { best = fill; B = b; }
+            }
+        }
         k_spec_xinv<<<ceil_div(nrows, (size_t)B), kFT, 0, st>>>(Z, g, B, p2, tw2, out, out_layout);
         TH_HIP(hipGetLastError());
     }
