@@ -91,3 +91,24 @@ def test_vote2_odd_shapes(hip, oracle):
         a = api.tensorvote(g, 2.0, 0.0, 1, 9, True, True, 10)
         b = oracle.tensorvote2(g, 2.0, 0.0, 1, 9, True, True, 10)
         assert rel_frob(a, b) <= 1e-4
+
+
+def test_vote2_table_kept_between_calls(hip):
+    """The device copy of the neighbour table is reused while the table is
+    unchanged: repeated calls, a parameter change, an ATV call in between and
+    a workspace release all give the results of fresh calls."""
+    from tiralib_amd._lib import call
+    f = _fields64(3)["mixed"]
+    a1 = api.tensorvote(f, 4.0, 1.0, 1, 25, True, True, 20)
+    a2 = api.tensorvote(f, 4.0, 1.0, 1, 25, True, True, 20)
+    np.testing.assert_array_equal(a1, a2)
+    b = api.tensorvote(f, 3.0, 0.0, 1, 19, True, True, 20)  # another table
+    assert not np.array_equal(a1, b)
+    api.tensorvote(f, 2.0, 0.0, 1, 13, True, True, 20, is_atv=True)
+    a3 = api.tensorvote(f, 4.0, 1.0, 1, 25, True, True, 20)
+    np.testing.assert_array_equal(a1, a3)
+    call("tira_hip_release_workspace")
+    a4 = api.tensorvote(f, 4.0, 1.0, 1, 25, True, True, 20)
+    np.testing.assert_array_equal(a1, a4)
+    b2 = api.tensorvote(f, 3.0, 0.0, 1, 19, True, True, 20)
+    np.testing.assert_array_equal(b, b2)

@@ -79,6 +79,11 @@ static constexpr int kMaxDevices = 16;
 // Deliberately not freed at thread exit: the HIP runtime may already be torn
 // down then. tira_hip_release_workspace() frees explicitly.
 static thread_local Workspace* g_ws[kMaxDevices] = {};
+// Bumped whenever a workspace buffer is allocated or released, so that
+// callers caching the contents of a slot can tell it is still the same buffer.
+static thread_local unsigned long long g_ws_gen = 1;
+
+unsigned long long ws_generation() { return g_ws_gen; }
 
 int ws_get(int slot, size_t bytes, void** p) {
     int dev = 0;
@@ -98,6 +103,7 @@ int ws_get(int slot, size_t bytes, void** p) {
             w.cap[slot] = 0;
         }
         size_t b = (bytes + 4095) & ~size_t(4095);
+        g_ws_gen++;
         TH_HIP(hipMalloc(&w.ptr[slot], b));
         w.cap[slot] = b;
     }
@@ -311,6 +317,7 @@ int tira_hip_release_workspace(void) {
     int dev = 0;
     TH_HIP(hipGetDevice(&dev));
     if (dev >= 0 && dev < kMaxDevices && g_ws[dev]) {
+        g_ws_gen++;
         TH_HIP(hipStreamSynchronize(g_stream));
         for (int s = 0; s < WS_NSLOTS; s++)
             if (g_ws[dev]->ptr[s]) TH_HIP(hipFree(g_ws[dev]->ptr[s]));
