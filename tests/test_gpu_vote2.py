@@ -104,11 +104,15 @@ def test_vote2_table_kept_between_calls(hip):
     np.testing.assert_array_equal(a1, a2)
     b = api.tensorvote(f, 3.0, 0.0, 1, 19, True, True, 20)  # another table
     assert not np.array_equal(a1, b)
-    api.tensorvote(f, 2.0, 0.0, 1, 13, True, True, 20, is_atv=True)
+    t1 = api.tensorvote(f, 2.0, 0.0, 1, 13, True, True, 20, is_atv=True)
     a3 = api.tensorvote(f, 4.0, 1.0, 1, 25, True, True, 20)
     np.testing.assert_array_equal(a1, a3)
+    t2 = api.tensorvote(f, 3.0, 0.0, 1, 19, True, True, 20, is_atv=True)  # another ATV table
+    assert not np.array_equal(t1, t2)
     call("tira_hip_release_workspace")
     a4 = api.tensorvote(f, 4.0, 1.0, 1, 25, True, True, 20)
     np.testing.assert_array_equal(a1, a4)
     b2 = api.tensorvote(f, 3.0, 0.0, 1, 19, True, True, 20)
     np.testing.assert_array_equal(b, b2)
+    np.testing.assert_array_equal(t1, api.tensorvote(f, 2.0, 0.0, 1, 13, True, True, 20, is_atv=True))
+    np.testing.assert_array_equal(t2, api.tensorvote(f, 3.0, 0.0, 1, 19, True, True, 20, is_atv=True))

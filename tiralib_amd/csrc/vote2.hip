@@ -112,6 +112,36 @@ int build_neighbors2d(int w, float s1, float s2, unsigned power, unsigned sample
     return (int)nb.size();
 }
 
+// Device copy of a host table in workspace `slot`, kept while its bytes and
+// the buffer (same device, address and workspace generation) are unchanged:
+// repeated calls with the same parameters skip the stream wait and the copy.
+struct TableCache {
+    int dev = -1;
+    const void* buf = nullptr;
+    unsigned long long gen = 0;
+    std::vector<unsigned char> bytes;
+};
+static int table_to_device(int slot, const void* host, size_t nbytes, TableCache& c, hipStream_t st, void** out) {
+    void* d = nullptr;
+    TH_TRY(ws_get(slot, nbytes ? nbytes : 1, &d));
+    int dev = 0;
+    TH_HIP(hipGetDevice(&dev));
+    const bool same = nbytes > 0 && c.dev == dev && c.buf == d && c.gen == ws_generation() &&
+                      c.bytes.size() == nbytes && std::memcmp(c.bytes.data(), host, nbytes) == 0;
+    if (nbytes > 0 && !same) {
+        // pageable source; the buffer may still be read by earlier work on the stream
+        TH_HIP(hipStreamSynchronize(st));
+        c.dev = -1;
+        TH_HIP(hipMemcpy(d, host, nbytes, hipMemcpyHostToDevice));
+        c.bytes.assign((const unsigned char*)host, (const unsigned char*)host + nbytes);
+        c.buf = d;
+        c.gen = ws_generation();
+        c.dev = dev;
+    }
+    *out = d;
+    return TIRA_OK;
+}
+
 // ---------------------------------------------------------------- kernels
 // Voter payload from the tensor field [a b b c]: eigenvector of the larger
 // eigenvalue, eta_s-scaled stick scale (|l1| - |l0|, negated when l1 < 0,
@@ -286,9 +316,8 @@ static int vote2_atv_run(const float* tin, const float* lam, const float* th, fl
     std::vector<Atv2> tab;
     build_atv2_table(w, sigma, tab);
     Atv2* dtab = nullptr;
-    TH_TRY(ws_get(WS_TABLE2, tab.size() * sizeof(Atv2), (void**)&dtab));
-    TH_HIP(hipStreamSynchronize(st));  // pageable source, table slot may still be read by earlier work
-    TH_HIP(hipMemcpy(dtab, tab.data(), tab.size() * sizeof(Atv2), hipMemcpyHostToDevice));
+    static thread_local TableCache atv_cache;
+    TH_TRY(table_to_device(WS_TABLE2, tab.data(), tab.size() * sizeof(Atv2), atv_cache, st, (void**)&dtab));
     dim3 grid(ceil_div(s1, 64), ceil_div(s0, 4));
     {
         ProfScope prof("vote2_atv", st);
@@ -336,32 +365,8 @@ static int vote2_run(const float* tin, const float* lam, const float* th, float*
     }
     TH_HIP(hipGetLastError());
     Nb2* dnb = nullptr;
-    TH_TRY(ws_get(WS_NB2, (size_t)(nbc > 0 ? nbc : 1) * sizeof(Nb2), (void**)&dnb));
-    // The device copy of the table is kept while the table (compared entry by
-    // entry) and its buffer (same device, address and workspace generation)
-    // are unchanged: repeated votes with the same parameters skip the stream
-    // wait and the copy.
-    static thread_local struct {
-        int dev = -1;
-        const void* buf = nullptr;
-        unsigned long long gen = 0;
-        std::vector<Nb2> tab;
-    } last;
-    int dev = 0;
-    TH_HIP(hipGetDevice(&dev));
-    const bool same = nbc > 0 && last.dev == dev && last.buf == dnb && last.gen == ws_generation() &&
-                      last.tab.size() == (size_t)nbc &&
-                      std::memcmp(last.tab.data(), host_nb, (size_t)nbc * sizeof(Nb2)) == 0;
-    if (nbc > 0 && !same) {
-        // pageable source: copy synchronously after prior work on the stream
-        TH_HIP(hipStreamSynchronize(st));
-        last.dev = -1;
-        TH_HIP(hipMemcpy(dnb, host_nb, (size_t)nbc * sizeof(Nb2), hipMemcpyHostToDevice));
-        last.tab.assign(host_nb, host_nb + nbc);
-        last.buf = dnb;
-        last.gen = ws_generation();
-        last.dev = dev;
-    }
+    static thread_local TableCache nb_cache;
+    TH_TRY(table_to_device(WS_NB2, host_nb, (size_t)(nbc > 0 ? nbc : 0) * sizeof(Nb2), nb_cache, st, (void**)&dnb));
     dim3 grid(ceil_div(s1n, 64), ceil_div(s0, 4));
     {
         ProfScope prof("vote2", st);
