@@ -101,6 +101,9 @@ static bool make_plan(int L, FftPlan& p) {
 
 // Plan of a line transform in LDS (fft_lines): as make_plan, but a 9 and a
 // lone 3 become one radix-27 stage, which takes a single round per thread.
+// That stage goes first: at p = 1 its 26 twiddles per butterfly are all W^0
+// and are skipped, while the radix after it reads its fewer twiddles once
+// per stage (fft_stage's fixed mapping).
 static bool make_lines_plan(int L, FftPlan& p) {
     if (!make_plan(L, p)) return false;
     int i9 = -1, i3 = -1;
@@ -109,9 +112,11 @@ static bool make_lines_plan(int L, FftPlan& p) {
         if (p.rad[i] == 3) i3 = i;
     }
     if (i9 < 0 || i3 < 0) return true;
-    p.rad[i9] = 27;
-    for (int i = i3; i + 1 < p.nst; i++) p.rad[i] = p.rad[i + 1];
-    p.nst--;
+    int k = 0;
+    for (int i = 0; i < p.nst; i++)
+        if (i != i9 && i != i3) p.rad[1 + k++] = p.rad[i];
+    p.rad[0] = 27;
+    p.nst = k + 1;
     return true;
 }
 
@@ -421,8 +426,10 @@ __device__ __noinline__ void fft_stage(lds_f2* __restrict__ s, int L, int nlines
         for (int q = 0; q < MAXQ; q++) {
             const int line = l0 + q * G;
             if (act && line < nlines) {
+                if (p > 1) {  // p = 1: every twiddle is W^0
 #pragma unroll
-                for (int r = 1; r < R; r++) u[q][r] = cmul(u[q][r], w[r]);
+                    for (int r = 1; r < R; r++) u[q][r] = cmul(u[q][r], w[r]);
+                }
                 dft_small<R, INV>(u[q]);
                 lds_f2* dst = s + line * L + (i - k) * R + k;
 #pragma unroll
@@ -449,11 +456,13 @@ __device__ __noinline__ void fft_stage(lds_f2* __restrict__ s, int L, int nlines
         if (t < nb) {
             const int line = fdiv(t, m, rm), i = t - line * m;
             const int k = i - fdiv(i, p, rp) * p;
+            if (p > 1) {  // p = 1: every twiddle is W^0
 #pragma unroll
-            for (int r = 1; r < R; r++) {
-                float2 w = ld2(tw + r * k * step);  // r k step < L
-                if (INV) w.y = -w.y;
-                u[q][r] = cmul(u[q][r], w);
+                for (int r = 1; r < R; r++) {
+                    float2 w = ld2(tw + r * k * step);  // r k step < L
+                    if (INV) w.y = -w.y;
+                    u[q][r] = cmul(u[q][r], w);
+                }
             }
             dft_small<R, INV>(u[q]);
             lds_f2* dst = s + line * L + (i - k) * R + k;
