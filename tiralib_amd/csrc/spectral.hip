@@ -392,7 +392,9 @@ __device__ __forceinline__ int fdiv(int n, int d, float rd) {
 // the butterfly alone; otherwise the butterflies of the block are dealt out
 // in order, kFT per round. FIXED = false compiles the second mapping only
 // (the column passes, whose B lines rarely fill whole rounds the first way).
-template <int R, bool INV, int BL = kFftBL, bool FIXED = true>
+// TW = false compiles no twiddles at all, for a stage that only ever runs
+// first (p = 1).
+template <int R, bool INV, int BL = kFftBL, bool FIXED = true, bool TW = true>
 __device__ __noinline__ void fft_stage(lds_f2* __restrict__ s, int L, int nlines, int p,
                                        const lds_f2* __restrict__ tw) {
     constexpr int MAXQ = (BL / R + kFT - 1) / kFT;
@@ -409,7 +411,7 @@ __device__ __noinline__ void fft_stage(lds_f2* __restrict__ s, int L, int nlines
         float2 w[R];
 #pragma unroll
         for (int r = 1; r < R; r++) {
-            w[r] = act ? ld2(tw + r * k * step) : make_float2(1.0f, 0.0f);  // r k step < L
+            w[r] = (TW && act) ? ld2(tw + r * k * step) : make_float2(1.0f, 0.0f);  // r k step < L
             if (INV) w[r].y = -w[r].y;
         }
 #pragma unroll
@@ -426,7 +428,7 @@ __device__ __noinline__ void fft_stage(lds_f2* __restrict__ s, int L, int nlines
         for (int q = 0; q < MAXQ; q++) {
             const int line = l0 + q * G;
             if (act && line < nlines) {
-                if (p > 1) {  // p = 1: every twiddle is W^0
+                if (TW && p > 1) {  // p = 1: every twiddle is W^0
 #pragma unroll
                     for (int r = 1; r < R; r++) u[q][r] = cmul(u[q][r], w[r]);
                 }
@@ -456,7 +458,7 @@ __device__ __noinline__ void fft_stage(lds_f2* __restrict__ s, int L, int nlines
         if (t < nb) {
             const int line = fdiv(t, m, rm), i = t - line * m;
             const int k = i - fdiv(i, p, rp) * p;
-            if (p > 1) {  // p = 1: every twiddle is W^0
+            if (TW && p > 1) {  // p = 1: every twiddle is W^0
 #pragma unroll
                 for (int r = 1; r < R; r++) {
                     float2 w = ld2(tw + r * k * step);  // r k step < L
@@ -489,8 +491,9 @@ __device__ __forceinline__ void fft_lines(float2* __restrict__ s_, const FftPlan
             case 5: fft_stage<5, INV>(s, pl.L, nlines, p, tw); break;
             case 7: fft_stage<7, INV>(s, pl.L, nlines, p, tw); break;
             case 9: fft_stage<9, INV>(s, pl.L, nlines, p, tw); break;
-            // one round per thread at 4096 staged values; its 26 twiddles are read per round
-            case 27: fft_stage<27, INV, kFftBL, false>(s, pl.L, nlines, p, tw); break;
+            // one round per thread at 4096 staged values; make_lines_plan puts it
+            // first (p = 1), so it has no twiddles, and none hold registers
+            case 27: fft_stage<27, INV, kFftBL, false, false>(s, pl.L, nlines, p, tw); break;
             default: fft_stage<8, INV>(s, pl.L, nlines, p, tw); break;
         }
         p *= R;
@@ -754,8 +757,11 @@ __device__ __forceinline__ void add_circle_average(const float4 N, float* __rest
 // (layout) instead of reading a payload, once per channel group (wide rows
 // take two or more groups: a second eigen-solve costs less than writing and
 // re-reading a 32-byte payload per voxel).
+// Four waves per SIMD (at most 128 VGPRs): unbounded, the eigen-solve and the
+// features take 136 and three waves; the bound costs a 16-VGPR spill and
+// gains 10% at 128^3.
 template <bool FROM_TENSORS>
-__global__ __launch_bounds__(kFT) void k_spec_xfwd(const float4* __restrict__ ps, const float4* __restrict__ pp,
+__global__ __launch_bounds__(kFT, 4) void k_spec_xfwd(const float4* __restrict__ ps, const float4* __restrict__ pp,
                                                    const float* __restrict__ tensors, int layout, SpecGeom g, int B,
                                                    int C, FftPlan pl, const float2* __restrict__ twg,
                                                    float2* __restrict__ Z) {
