@@ -655,12 +655,19 @@ __device__ __forceinline__ void cols_single(float2* __restrict__ base, int B, in
         default: TH_RADIX_SWITCH(R, CALL);                 \
     }
 
+// At least six waves per SIMD: the bound reaches the stage functions only this
+// kernel calls, which then fit 73 VGPRs without spilling (101 unbounded, five
+// waves); eight waves spill and run 25% slower.
 template <bool INV, int BL>
-__global__ __launch_bounds__(kFT) void k_fft_cols(float2* __restrict__ data, LineGeom g, FftPlan pl,
+__global__ __launch_bounds__(kFT, 6) void k_fft_cols(float2* __restrict__ data, LineGeom g, FftPlan pl,
                                                   const float2* __restrict__ twg) {
-    __shared__ float2 s_[BL];
-    __shared__ float2 tw_[kMaxL];
+    // B L staged values, then L twiddles: sized at launch (B L <= BL / 2),
+    // 18 KB at L = 135 where a static BL-value array took 40 KB and held the
+    // pass to four blocks per CU
+    extern __shared__ float2 cols_dyn[];
     const int L = pl.L;
+    float2* s_ = cols_dyn;
+    float2* tw_ = cols_dyn + g.B * L;
     // XCD-major order: neighbouring column batches (which share 128-byte
     // lines when B < 16) run on one XCD and meet in its L2
     unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
@@ -1370,12 +1377,14 @@ static int launch_lines(float2* data, const LineGeom& lg, const FftPlan& pl, con
     if (nblk == 0) return TIRA_OK;
     TH_ARG(nblk < (1ull << 31), "spectral vote: volume too large");
     const unsigned grid = (unsigned)nblk;
+    const size_t clds = ((size_t)lg.B * pl.L + pl.L) * sizeof(float2);  // k_fft_cols staging + twiddles
+    TH_ARG(rows || (size_t)lg.B * pl.L <= (size_t)kFftBL, "spectral vote: column batch exceeds the staging");
     if (inv) {
         if (rows) k_fft_lines<true, true><<<grid, kFT, 0, st>>>(data, lg, pl, tw);
-        else k_fft_cols<true, kFftBL><<<grid, kFT, 0, st>>>(data, lg, pl, tw);
+        else k_fft_cols<true, kFftBL><<<grid, kFT, clds, st>>>(data, lg, pl, tw);
     } else {
         if (rows) k_fft_lines<false, true><<<grid, kFT, 0, st>>>(data, lg, pl, tw);
-        else k_fft_cols<false, kFftBL><<<grid, kFT, 0, st>>>(data, lg, pl, tw);
+        else k_fft_cols<false, kFftBL><<<grid, kFT, clds, st>>>(data, lg, pl, tw);
     }
     TH_HIP(hipGetLastError());
     return TIRA_OK;
