@@ -393,8 +393,9 @@ __device__ __forceinline__ int fdiv(int n, int d, float rd) {
 // in order, kFT per round. FIXED = false compiles the second mapping only
 // (the column passes, whose B lines rarely fill whole rounds the first way).
 // TW = false compiles no twiddles at all, for a stage that only ever runs
-// first (p = 1).
-template <int R, bool INV, int BL = kFftBL, bool FIXED = true, bool TW = true>
+// first (p = 1). TAG gives one kernel stage functions of its own, so that its
+// launch bound reaches them (a bound does not constrain a shared callee).
+template <int R, bool INV, int BL = kFftBL, bool FIXED = true, bool TW = true, int TAG = 0>
 __device__ __noinline__ void fft_stage(lds_f2* __restrict__ s, int L, int nlines, int p,
                                        const lds_f2* __restrict__ tw) {
     constexpr int MAXQ = (BL / R + kFT - 1) / kFT;
@@ -655,11 +656,14 @@ __device__ __forceinline__ void cols_single(float2* __restrict__ base, int B, in
         default: TH_RADIX_SWITCH(R, CALL);                 \
     }
 
-// At least six waves per SIMD: the bound reaches the stage functions only this
-// kernel calls, which then fit 73 VGPRs without spilling (101 unbounded, five
-// waves); eight waves spill and run 25% slower.
-template <bool INV, int BL>
-__global__ __launch_bounds__(kFT, 6) void k_fft_cols(float2* __restrict__ data, LineGeom g, FftPlan pl,
+// MINW: waves per SIMD the kernel is compiled for. Batches of 16 columns take
+// 6: the bound reaches the stage functions that only this instantiation calls
+// (TAG), which then fit 73 VGPRs without spilling (101 unbounded, five waves),
+// 1% faster at L = 135; eight waves spill and run 25% slower. Narrower batches
+// (L > 256) take no bound: there the bounded code runs 8% slower and fetches
+// 1.3x the algorithmic bytes.
+template <bool INV, int BL, int MINW>
+__global__ __launch_bounds__(kFT, MINW) void k_fft_cols(float2* __restrict__ data, LineGeom g, FftPlan pl,
                                                   const float2* __restrict__ twg) {
     // B L staged values, then L twiddles: sized at launch (B L <= BL / 2),
     // 18 KB at L = 135 where a static BL-value array took 40 KB and held the
@@ -693,7 +697,7 @@ __global__ __launch_bounds__(kFT, 6) void k_fft_cols(float2* __restrict__ data, 
     int p = pl.rad[0];
     for (int st = 1; st < pl.nst - 1; st++) {
         const int R = pl.rad[st];
-        TH_RADIX_SWITCH(R, (fft_stage<RR, INV, BL, false>(s, L, B, p, tw)));
+        TH_RADIX_SWITCH(R, (fft_stage<RR, INV, BL, false, true, MINW>(s, L, B, p, tw)));
         p *= R;
     }
     TH_RADIX_SWITCH_LAST(pl.rad[pl.nst - 1],
@@ -1381,10 +1385,12 @@ static int launch_lines(float2* data, const LineGeom& lg, const FftPlan& pl, con
     TH_ARG(rows || (size_t)lg.B * pl.L <= (size_t)kFftBL, "spectral vote: column batch exceeds the staging");
     if (inv) {
         if (rows) k_fft_lines<true, true><<<grid, kFT, 0, st>>>(data, lg, pl, tw);
-        else k_fft_cols<true, kFftBL><<<grid, kFT, clds, st>>>(data, lg, pl, tw);
+        else if (lg.B >= 16) k_fft_cols<true, kFftBL, 6><<<grid, kFT, clds, st>>>(data, lg, pl, tw);
+        else k_fft_cols<true, kFftBL, 1><<<grid, kFT, clds, st>>>(data, lg, pl, tw);
     } else {
         if (rows) k_fft_lines<false, true><<<grid, kFT, 0, st>>>(data, lg, pl, tw);
-        else k_fft_cols<false, kFftBL><<<grid, kFT, clds, st>>>(data, lg, pl, tw);
+        else if (lg.B >= 16) k_fft_cols<false, kFftBL, 6><<<grid, kFT, clds, st>>>(data, lg, pl, tw);
+        else k_fft_cols<false, kFftBL, 1><<<grid, kFT, clds, st>>>(data, lg, pl, tw);
     }
     TH_HIP(hipGetLastError());
     return TIRA_OK;
