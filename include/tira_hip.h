@@ -186,6 +186,17 @@ int tira_tensorvote3_tk(float* output_field, const float* lambdas, const float* 
 int tira_tensorvote3_async(const float* input_field, int in_layout, float* output_field, int out_layout,
                            unsigned s0, unsigned s1, unsigned s2, float sigma1, float sigma2, unsigned w,
                            unsigned power, int stick, int plate, unsigned samples, int plate_mode);
+/* tira_tensorvote3_async followed by tira_eig3_async(output_field, s0 s1 s2,
+ * out_layout, evals, NULL, NULL, eig_flags, n_refined_dev): the same field,
+ * bit for bit, and eigenvalues (n x 3) of the same solver to FP32 rounding
+ * (a few ulps of the field's largest entry). No reference counterpart: the
+ * reference's callers vote, then decompose. When the spectral vote runs, its
+ * last pass writes the eigenvalues of every voxel it writes (one read of the
+ * field and one launch fewer); otherwise eig3 runs after the vote. */
+int tira_tensorvote3_eig_async(const float* input_field, int in_layout, float* output_field, int out_layout,
+                               unsigned s0, unsigned s1, unsigned s2, float sigma1, float sigma2, unsigned w,
+                               unsigned power, int stick, int plate, unsigned samples, int plate_mode, float* evals,
+                               unsigned eig_flags, unsigned long long* n_refined_dev);
 /* tira::cuda::_tensorvote3_atv(out, lambdas, evec1, evec2, s0, s1, s2, sigma, w),
  * tensorvote.h:2695-2739: 3D analytical tensor voting from eigenpairs
  * (lambdas n x 3 ascending, evec1 middle and evec2 largest eigenvectors n x 3,

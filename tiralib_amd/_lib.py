@@ -88,6 +88,7 @@ SIGNATURES = {
     "tira_evals3_symmetric": [_vp, _sz, _vp, _i],
     "tira_evecs3spherical_symmetric": [_vp, _vp, _sz, _vp, _i],
     "tira_eig3_async": [_vp, _sz, _i, _vp, _vp, _vp, _u, _vp],
+    "tira_tensorvote3_eig_async": [_vp, _i, _vp, _i, _u, _u, _u, _f, _f, _u, _u, _i, _i, _u, _i, _vp, _u, _vp],
     "tira_eig3_batch_async": [_vp, _vp, _i, _i, _vp, _vp, _vp, _u, _vp],
     "tira_build_neighbors2d": [_i, _f, _f, _u, _u, _i, _i, _vp, _i, C.POINTER(_i)],
     "tira_tensorvote2": [_vp, _vp, _f, _f, _i, _u, _u, _u, _i, _i, _u, _i],

@@ -16,6 +16,7 @@
 //  * results are staged in LDS and written back with 16-byte stores.
 #include "common.hpp"
 #include "eig_device.hpp"
+#include "eig_store.hpp"
 
 #include <cfloat>
 #include <cmath>
@@ -27,19 +28,6 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kEigThreads = 256;
 constexpr int kEigPerThread = 2;
 constexpr int kEigTile = kEigThreads * kEigPerThread;  // tensors per block
-
-// Power-of-two prescale so that the largest |entry| lies in [0.5, 1): exact
-// in floating point, and it turns the reference's absolute thresholds into
-// relative ones. Returns the exponent to scale eigenvalues back with.
-__device__ __forceinline__ int prescale6(float& a, float& b, float& c, float& d, float& e, float& f) {
-    float m = fmaxf(fmaxf(fmaxf(fabsf(a), fabsf(b)), fmaxf(fabsf(c), fabsf(d))), fmaxf(fabsf(e), fabsf(f)));
-    if (!(m > 0.0f) || !isfinite(m)) return 0;
-    int ex;
-    frexpf(m, &ex);
-    a = ldexpf(a, -ex); b = ldexpf(b, -ex); c = ldexpf(c, -ex);
-    d = ldexpf(d, -ex); e = ldexpf(e, -ex); f = ldexpf(f, -ex);
-    return ex;
-}
 
 // Reads tensor `i` of a staged LDS tile as (a, b, c, d, e, f) = (xx, xy, yy, xz, yz, zz).
 template <int LAYOUT>
@@ -64,20 +52,6 @@ __device__ __forceinline__ void global_tensor(const float* g, size_t i, float& a
         const float* m = g + i * 6;
         a = m[0]; b = m[1]; d = m[2]; c = m[3]; e = m[4]; f = m[5];
     }
-}
-
-// Eigenvalues l0 <= l1 <= l2 (mode 0), the saliency maps (l2 - l1, l1 - l0,
-// l0) -- stick, plate and ball saliency of a voted field (mode 1) -- or the
-// same with the eigenvalues ranked by magnitude, m0 <= m1 <= m2 of |l|
-// (mode 2, the ranking of reconstruct.py:14-34).
-__device__ __forceinline__ void store_eig(float* o, float l0, float l1, float l2, int mode) {
-    if (mode == 2) {
-        const float a = fabsf(l0), b = fabsf(l1), c = fabsf(l2);
-        const float m2 = fmaxf(a, fmaxf(b, c)), m0 = fminf(a, fminf(b, c));
-        const float m1 = __builtin_amdgcn_fmed3f(a, b, c);
-        o[0] = m2 - m1; o[1] = m1 - m0; o[2] = m0;
-    } else if (mode == 1) { o[0] = l2 - l1; o[1] = l1 - l0; o[2] = l0; }
-    else { o[0] = l0; o[1] = l1; o[2] = l2; }
 }
 
 // Segment table: one launch covers up to kEigMaxSeg independent tensor

@@ -44,6 +44,7 @@
 #include "common.hpp"
 #include "vote3.hpp"
 #include "voxel_payload.hpp"
+#include "eig_store.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -933,7 +934,8 @@ __global__ __launch_bounds__(256) void k_spec_mul(float2* __restrict__ Z, const 
 // W_2 = (yz, zz) as (real, imaginary) parts.
 __global__ __launch_bounds__(kFT) void k_spec_xinv(const float2* __restrict__ Z, SpecGeom g, int B, FftPlan pl,
                                                    const float2* __restrict__ twg, float* __restrict__ out,
-                                                   int layout) {
+                                                   int layout, float* __restrict__ evals, unsigned eflags,
+                                                   unsigned long long* __restrict__ n_refined) {
     __shared__ float2 s[kFftBL];
     __shared__ float2 tw[kMaxL];
     const int L = pl.L;
@@ -966,6 +968,9 @@ __global__ __launch_bounds__(kFT) void k_spec_xinv(const float2* __restrict__ Z,
             float* o = out + v * 6;
             o[0] = w0.x; o[1] = w0.y; o[2] = w1.x; o[3] = w1.y; o[4] = w2.x; o[5] = w2.y;
         }
+        // eigenvalues of the tensor just written, from the same floats
+        // (xx, xy, yy, xz, yz, zz) = (w0.x, w0.y, w1.y, w1.x, w2.x, w2.y)
+        if (evals) eig3_tensor_values(w0.x, w0.y, w1.y, w1.x, w2.x, w2.y, eflags, evals + v * 3, n_refined);
     }
 }
 
@@ -1448,7 +1453,7 @@ bool spectral_preferred(unsigned s0, unsigned s1, unsigned s2, unsigned w, unsig
 int vote3_spectral_run(const float* tensors, int in_layout, const float* lam, const float* ev2, const float* ev0,
                        float* out, int out_layout, unsigned s0, unsigned s1, unsigned s2, float sigma1,
                        float sigma2, unsigned w, unsigned power, bool stick, bool plate, unsigned samples,
-                       bool atv, hipStream_t st) {
+                       bool atv, hipStream_t st, EigOut* eig) {
     const unsigned p = power < 1 ? 1 : power;
     if (atv) {  // sigma2, power, stick, plate and samples do not apply to ATV
         stick = plate = true;
@@ -1577,8 +1582,11 @@ int vote3_spectral_run(const float* tensors, int in_layout, const float* lam, co
                 if (fill > best + 0.02) { best = fill; B = b; }
             }
         }
-        k_spec_xinv<<<ceil_div(nrows, (size_t)B), kFT, 0, st>>>(Z, g, B, p2, tw2, out, out_layout);
+        k_spec_xinv<<<ceil_div(nrows, (size_t)B), kFT, 0, st>>>(Z, g, B, p2, tw2, out, out_layout,
+                                                                eig ? eig->evals : nullptr, eig ? eig->flags : 0u,
+                                                                eig ? eig->n_refined : nullptr);
         TH_HIP(hipGetLastError());
+        if (eig) eig->done = true;
     }
     return TIRA_OK;
 }

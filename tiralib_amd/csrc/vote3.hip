@@ -1015,7 +1015,8 @@ static void launch_closed(bool stick, bool plate, unsigned grid, hipStream_t st,
 // lam/ev2/ev0 eigenpairs are given.
 int vote3_run(const float* tensors, int in_layout, const float* lam, const float* ev2, const float* ev0, float* out,
               int out_layout, unsigned s0, unsigned s1, unsigned s2, float sigma1, float sigma2, unsigned w,
-              unsigned power, bool stick, bool plate, unsigned samples, int plate_mode, hipStream_t st) {
+              unsigned power, bool stick, bool plate, unsigned samples, int plate_mode, hipStream_t st,
+              EigOut* eig) {
     const size_t nvox = (size_t)s0 * s1 * s2;
     if (nvox == 0) return TIRA_OK;
     if (!stick && !plate) {
@@ -1028,7 +1029,7 @@ int vote3_run(const float* tensors, int in_layout, const float* lam, const float
         plate_mode = TIRA_VOTE_SPECTRAL;
     if (plate_mode == TIRA_VOTE_SPECTRAL) {
         const int rc = vote3_spectral_run(tensors, in_layout, lam, ev2, ev0, out, out_layout, s0, s1, s2, sigma1,
-                                          sigma2, w, power, stick, plate, samples, false, st);
+                                          sigma2, w, power, stick, plate, samples, false, st, eig);
         if (rc != kSpectralNotApplicable) return rc;
         plate_mode = TIRA_PLATE_AUTO;  // power, samples or size outside the spectral method: direct gather
     }
@@ -1330,6 +1331,23 @@ int tira_tensorvote3_async(const float* input_field, int in_layout, float* outpu
     TH_TRY(check_vote_args(s0, s1, s2, w, sigma1, sigma2));
     return vote3_run(input_field, in_layout, nullptr, nullptr, nullptr, output_field, out_layout, s0, s1, s2, sigma1,
                      sigma2, w, power, stick != 0, plate != 0, samples, plate_mode, stream());
+}
+
+int tira_tensorvote3_eig_async(const float* input_field, int in_layout, float* output_field, int out_layout,
+                               unsigned s0, unsigned s1, unsigned s2, float sigma1, float sigma2, unsigned w,
+                               unsigned power, int stick, int plate, unsigned samples, int plate_mode, float* evals,
+                               unsigned eig_flags, unsigned long long* n_refined_dev) {
+    TH_ARG(input_field && output_field && evals, "NULL pointer");
+    TH_ARG(in_layout == TIRA_LAYOUT_MAT9 || in_layout == TIRA_LAYOUT_PACKED6, "bad input layout");
+    TH_ARG(out_layout == TIRA_LAYOUT_MAT9 || out_layout == TIRA_LAYOUT_PACKED6, "bad output layout");
+    TH_ARG((plate_mode >= TIRA_PLATE_AUTO && plate_mode <= TIRA_PLATE_CLOSED) || plate_mode == TIRA_VOTE_SPECTRAL, "bad plate mode %d", plate_mode);
+    TH_TRY(check_vote_args(s0, s1, s2, w, sigma1, sigma2));
+    EigOut eo{evals, eig_flags, n_refined_dev, false};
+    TH_TRY(vote3_run(input_field, in_layout, nullptr, nullptr, nullptr, output_field, out_layout, s0, s1, s2, sigma1,
+                     sigma2, w, power, stick != 0, plate != 0, samples, plate_mode, stream(), &eo));
+    if (eo.done) return TIRA_OK;
+    return tira_eig3_async(output_field, (size_t)s0 * s1 * s2, out_layout, evals, nullptr, nullptr, eig_flags,
+                           n_refined_dev);
 }
 
 // _tensorvote3_atv (tensorvote.h:2695-2739): ATV from eigenpairs.

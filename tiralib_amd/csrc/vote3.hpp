@@ -24,15 +24,25 @@ int vote3_payload_flat(const float* tensors, int in_layout, const float* lam, co
                        unsigned s0, unsigned s1, unsigned s2, float4* ps, float4* pp, hipStream_t st);
 bool spectral_preferred(unsigned s0, unsigned s1, unsigned s2, unsigned w, unsigned power, bool stick, bool plate,
                         unsigned samples, bool atv = false);
+// Eigenvalues of the voted field requested with the vote: the spectral vote
+// writes them from its last pass and sets `done`; otherwise the caller runs
+// eig3 on the field.
+struct EigOut {
+    float* evals;
+    unsigned flags;
+    unsigned long long* n_refined;
+    bool done;
+};
 // Status of vote3_spectral_run when the request is outside the spectral method.
 constexpr int kSpectralNotApplicable = -1;
 int vote3_spectral_run(const float* tensors, int in_layout, const float* lam, const float* ev2, const float* ev0,
                        float* out, int out_layout, unsigned s0, unsigned s1, unsigned s2, float sigma1,
                        float sigma2, unsigned w, unsigned power, bool stick, bool plate, unsigned samples,
-                       bool atv, hipStream_t st);
+                       bool atv, hipStream_t st, EigOut* eig = nullptr);
 int resolve_plate_mode(int plate_mode, unsigned power, unsigned samples);
 int vote3_run(const float* tensors, int in_layout, const float* lam, const float* ev2, const float* ev0, float* out,
               int out_layout, unsigned s0, unsigned s1, unsigned s2, float sigma1, float sigma2, unsigned w,
-              unsigned power, bool stick, bool plate, unsigned samples, int plate_mode, hipStream_t st);
+              unsigned power, bool stick, bool plate, unsigned samples, int plate_mode, hipStream_t st,
+              EigOut* eig = nullptr);
 
 }  // namespace th
