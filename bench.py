@@ -243,7 +243,7 @@ def bench_mlp(steps, check=True, log=None, keep_net=False):
     vol = synth.curved_sheets_volume_device(shape, 12, 0.02, seed=synth.SEED_CFG4)
     net = api.PlateMLP(sigma, 0.0, 1, 64, seed=synth.SEED_CFG4)
     t = time.perf_counter()
-    tr, va = net.train(nsamples=1 << 22, epochs=100, batch=8192, lr=1e-2, lr_final=1e-5, seed=synth.SEED_CFG4)
+    tr, va = net.train(nsamples=1 << 22, epochs=100, batch=4096, lr=3e-3, lr_final=1e-6, seed=synth.SEED_CFG4)
     train_s = time.perf_counter() - t
     be = net.bin_errors()
     say(f"mlp: trained in {train_s:.1f} s, validation error {va:.2e}")

@@ -360,16 +360,42 @@ int tira_plate_mlp_step(void* handle, const void* X, const float* T, size_t n, f
  * NULL) receives it, `set` (may be NULL) replaces it first. Training sets it
  * from the training targets. */
 int tira_plate_mlp_affine(void* handle, float* get, const float* set);
-/* Per-orientation relative error of the network (TIRA_MLP_NBINS floats). */
+/* Per-orientation relative error of the network evaluated per pair
+ * (TIRA_MLP_NBINS floats). */
 int tira_plate_mlp_bin_errors(void* handle, float* errs);
+/* The same for the tabulated network (the default vote method). */
+int tira_plate_mlp_table_errors(void* handle, float* errs);
 int tira_plate_mlp_refresh_bins(void* handle);
+/* Host copy of the tabulated network the vote reads: *rows distance rows of
+ * *cells cells, each cell 8 floats ((A, B, g, D/2) at x_j = (j + 0.5) / cells
+ * and the difference to the next cell), the plate vote / c1 being
+ * A I + B n n^T + s g (n d^T + d n^T) + D d d^T at s = n.d, x = 1 - s^2.
+ * Row 0 is the voter itself, rows 1..K the squared lattice distances
+ * shells[0..K-1] (|d| < 3 sigma1), row K + 1 every farther offset. Any
+ * output may be NULL; `tab` needs rows x cells x 8 floats (cap_floats). */
+int tira_plate_mlp_vote_table(void* handle, float* tab, size_t cap_floats, int* rows, int* cells, int* shells);
+/* Replaces the vote table with a caller-built one of the same shape (host,
+ * rows x cells x 8 floats, layout as above) until the next weight change:
+ * a test hook that lets the table vote be judged with a table of the exact
+ * integral. */
+int tira_plate_mlp_set_vote_table(void* handle, const float* tab, size_t nfloats);
+
+/* Evaluation of the network in the vote. */
+#define TIRA_MLP_TABLE    0   /* tabulated once per weight change over (distance row, x = 1 - (n.d)^2) on
+                                 MFMA, linear in x per pair (default) */
+#define TIRA_MLP_PER_PAIR 1   /* one network evaluation per (receiver, voter) pair on MFMA */
 /* 3D stick + learned plate vote (device pointers, non-blocking unless
  * n_fallback is given): the network's sigma1, sigma2 and power; voters whose
  * orientation bin's error exceeds `tol` take the integrator's plate vote.
- * n_fallback (host, may be NULL) receives the number of such voters. */
+ * n_fallback (host, may be NULL) receives the number of such voters. Uses
+ * TIRA_MLP_TABLE (windows up to 63; larger windows evaluate per pair). */
 int tira_tensorvote3_mlp(void* mlp, const float* input_field, int in_layout, float* output_field, int out_layout,
                          unsigned s0, unsigned s1, unsigned s2, unsigned w, int stick, float tol,
                          unsigned long long* n_fallback);
+/* The same with the evaluation method chosen (TIRA_MLP_*). */
+int tira_tensorvote3_mlp_method(void* mlp, const float* input_field, int in_layout, float* output_field,
+                                int out_layout, unsigned s0, unsigned s1, unsigned s2, unsigned w, int stick,
+                                float tol, unsigned long long* n_fallback, int method);
 
 /* ------------------------------------------------------------------ saliency, threshold, thinning */
 /* Stick saliency |l|max - |l|min of n 2x2 tensors [a b b c] (reconstruct.py:14-34),

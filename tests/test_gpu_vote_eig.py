@@ -1,6 +1,8 @@
 """tira_tensorvote3_eig_async against tira_tensorvote3_async followed by
-tira_eig3_async on its output: the same field bit for bit, and eigenvalues of
-the same solver to FP32 rounding (the solver inlined into another kernel may
+tira_eig3_async on its output, and against the FP64 oracle's eigenvalues of
+the voted tensors (<= 1e-6 normwise with the FP64 re-solve and
+scale-invariant thresholds): the same field
+bit for bit, and eigenvalues of the same solver to FP32 rounding (the solver inlined into another kernel may
 contract a few operations differently: 0.6% of values differ, by one or two
 ulps), with the FP64 re-solve taken for the same tensors up to the borderline
 ones, for both output layouts and the eigen flags the pipelines use. Sizes from 32^3 voxels take the spectral vote,
@@ -34,7 +36,7 @@ def _field(shape, seed):
 @pytest.mark.parametrize("out_layout", [_lib.LAYOUT_PACKED6, _lib.LAYOUT_MAT9])
 @pytest.mark.parametrize("flags", [_lib.EIG_REFINE_FP64 | _lib.EIG_SCALE_INVARIANT, 0, _lib.EIG_REFINE_FP64,
                                    _lib.EIG_REFINE_FP64 | _lib.EIG_SALIENCY | _lib.EIG_MAGNITUDE])
-def test_vote_eig_equals_vote_then_eig3(hip, shape, sigma, w, out_layout, flags):
+def test_vote_eig_equals_vote_then_eig3(hip, oracle, shape, sigma, w, out_layout, flags):
     n = int(np.prod(shape))
     print(f"vote+eig shape={shape} out_layout={out_layout} flags={flags}", flush=True)
     comps = 9 if out_layout == _lib.LAYOUT_MAT9 else 6
@@ -62,3 +64,14 @@ def test_vote_eig_equals_vote_then_eig3(hip, shape, sigma, w, out_layout, flags)
     assert np.abs(ef - er).max() <= 4e-6 * scale, np.abs(ef - er).max() / scale
     assert abs(int(c[0]) - int(c[1])) <= max(2, int(c[0]) // 100)
     assert np.isfinite(ef).all()
+    if flags == _lib.EIG_REFINE_FP64 | _lib.EIG_SCALE_INVARIANT:
+        # against the FP64 oracle: eigenvalues of the same voted tensors in
+        # double (with the reference's absolute thresholds, flags without
+        # SCALE_INVARIANT, tiny tensors may take its diagonal shortcut)
+        t6 = ff if out_layout == _lib.LAYOUT_PACKED6 else ff[:, [0, 1, 2, 4, 5, 8]]
+        own = oracle.eig3_packed6_f64(t6)
+        scale = np.maximum(np.abs(own).max(axis=1), 1e-30)
+        live = np.abs(t6).max(axis=1) > 0
+        err = (np.abs(ef.astype(np.float64) - own).max(axis=1) / scale)[live]
+        print(f"fused eigenvalues vs FP64: normwise max {err.max():.2e}")
+        assert err.max() <= 1e-6

@@ -92,3 +92,84 @@ def test_features_and_input_row():
     row = M.input_row(f)
     assert np.allclose(row[:5] + row[5:10], f, rtol=0, atol=2e-5)
     assert row[10] == 1.0 and not row[11:].any()
+
+
+def _exact_row(oracle, power=1, rho=1.0, M_=M.TAB_M):
+    """Table row built from the FP64 integral (64 samples) in place of the
+    network: the plate-frame targets at the grid's features."""
+    Y = []
+    for f in M.table_features(rho, M_):
+        sn, _, s = f[0], f[1], f[2]
+        d = np.array([sn, 0.0, s]) * (rho * 2.0)  # |d| = rho sigma1 with sigma1 = 2
+        t, _ = M.target(d, np.array([0.0, 0.0, 1.0]), 2.0, 0.0, power, 64, oracle.plate3_sum_f64)
+        Y.append(t)
+    return M.table_row(np.array(Y), M_)
+
+
+def _random_unit(rng):
+    v = rng.normal(size=3)
+    return v / np.linalg.norm(v)
+
+
+def test_table_conversion_exact_at_grid_points(oracle):
+    """At the cell centres the tabulated form A I + B nn + s g (nd + dn) + D dd
+    rebuilt from the exact integral's plate-frame components equals the FP64
+    integral in any orientation: a dropped term, a wrong sign of s or of the
+    odd part, or a lost factor 2 in D/2 fails."""
+    rng = np.random.default_rng(3)
+    for power in (1, 2):
+        row = _exact_row(oracle, power)
+        x = M.table_grid()
+        for j in (0, 1, 17, 40, 62, 63):
+            for sg in (1.0, -1.0):
+                s = sg * np.sqrt(1.0 - x[j])
+                n = _random_unit(rng)
+                w = _random_unit(rng)
+                w -= (w @ n) * n
+                w /= np.linalg.norm(w)
+                d = s * n + np.sqrt(x[j]) * w
+                got = M.table_vote(row[j], n, d)
+                want = oracle.plate3_sum_f64(d, 1.0, 0.0, power, n, 64)
+                assert np.allclose(got, want, rtol=0, atol=1e-11 * np.abs(want).max()), (power, j, sg)
+
+
+def test_table_interpolation_error_bound(oracle):
+    """Between and beyond the cell centres (linear in x, extrapolated over the
+    end half cells) the tabulated exact integral stays within the
+    interpolation bound: RMS 2e-4 and at most 1e-3 relative per pair for
+    p = 1 at 64 cells (x-polynomials of degree <= 2), at s = 0 and s = +-1 too."""
+    rng = np.random.default_rng(4)
+    row = _exact_row(oracle, 1)
+    num = den = 0.0
+    worst = 0.0
+    for s in np.concatenate([rng.uniform(-1, 1, 400), [0.0, 1.0, -1.0, 1e-4, -0.9999]]):
+        n = _random_unit(rng)
+        w = _random_unit(rng)
+        w -= (w @ n) * n
+        w /= np.linalg.norm(w)
+        d = s * n + np.sqrt(max(0.0, 1.0 - s * s)) * w
+        d /= np.linalg.norm(d)
+        got = M.table_vote(M.table_lookup(row, 1.0 - (n @ d) ** 2), n, d)
+        want = oracle.plate3_sum_f64(d, 1.0, 0.0, 1, n, 64)
+        e = M.COMP_W @ (got - want) ** 2
+        t = M.COMP_W @ want ** 2
+        num += e
+        den += t
+        worst = max(worst, np.sqrt(e / t))
+    assert np.sqrt(num / den) <= 2e-4 and worst <= 1e-3, (np.sqrt(num / den), worst)
+
+
+def test_table_shells_are_sums_of_three_squares():
+    """Rows 1..K: the distinct squared lattice distances inside 3 sigma1,
+    checked against Legendre's three-square theorem (n is a sum of three
+    squares unless n = 4^a (8b + 7))."""
+    for sigma in (1.0, 1.5, 2.0, 4.0):
+        got = M.shells(sigma)
+        want = []
+        for l2 in range(1, int(9 * sigma * sigma) + 2):
+            q = l2
+            while q % 4 == 0:
+                q //= 4
+            if q % 8 != 7 and np.sqrt(np.float32(l2)) / np.float32(sigma) < np.float32(3.0):
+                want.append(l2)
+        assert np.array_equal(got, want), sigma

@@ -37,6 +37,8 @@ EIG_MAGNITUDE = 8
 
 MLP_NPARAM = 12288
 MLP_NBINS = 2048
+MLP_TABLE = 0      # TIRA_MLP_TABLE: network tabulated over (distance row, x = 1 - (n.d)^2)
+MLP_PER_PAIR = 1   # TIRA_MLP_PER_PAIR: one network evaluation per pair
 
 
 class TiraError(RuntimeError):
@@ -138,6 +140,10 @@ SIGNATURES = {
     "tira_plate_mlp_bin_errors": [_vp, _vp],
     "tira_plate_mlp_refresh_bins": [_vp],
     "tira_tensorvote3_mlp": [_vp, _vp, _i, _vp, _i, _u, _u, _u, _u, _i, _f, C.POINTER(C.c_ulonglong)],
+    "tira_tensorvote3_mlp_method": [_vp, _vp, _i, _vp, _i, _u, _u, _u, _u, _i, _f, C.POINTER(C.c_ulonglong), _i],
+    "tira_plate_mlp_table_errors": [_vp, _vp],
+    "tira_plate_mlp_vote_table": [_vp, _vp, _sz, C.POINTER(_i), C.POINTER(_i), _vp],
+    "tira_plate_mlp_set_vote_table": [_vp, _vp, _sz],
 }
 
 _lib = None

@@ -386,7 +386,7 @@ class PlateMLP:
         self.handle = h.value
         self.sigma1, self.sigma2, self.power = float(sigma1), float(sigma2), int(power)
 
-    def train(self, nsamples=1 << 22, epochs=300, batch=8192, lr=1e-2, lr_final=0.0, seed=1004):
+    def train(self, nsamples=1 << 22, epochs=100, batch=4096, lr=3e-3, lr_final=1e-6, seed=1004):
         tr, va = C.c_float(), C.c_float()
         call("tira_plate_mlp_train", self.handle, int(nsamples), int(epochs), int(batch), float(lr), float(lr_final),
              int(seed), C.byref(tr), C.byref(va))
@@ -439,16 +439,38 @@ class PlateMLP:
         call("tira_plate_mlp_bin_errors", self.handle, e.ctypes.data)
         return e
 
+    def table_errors(self):
+        e = np.empty(_lib.MLP_NBINS, np.float32)
+        call("tira_plate_mlp_table_errors", self.handle, e.ctypes.data)
+        return e
+
     def refresh_bins(self):
         call("tira_plate_mlp_refresh_bins", self.handle)
 
+    def vote_table(self):
+        """(table (rows, cells, 8) float32, shells (squared distances of rows 1..K))
+        of the tabulated network the vote reads (tira_plate_mlp_vote_table)."""
+        rows, cells = C.c_int(), C.c_int()
+        call("tira_plate_mlp_vote_table", self.handle, None, 0, C.byref(rows), C.byref(cells), None)
+        tab = np.empty((rows.value, cells.value, 8), np.float32)
+        shells = np.zeros(max(rows.value - 2, 1), np.int32)
+        call("tira_plate_mlp_vote_table", self.handle, tab.ctypes.data, tab.size, None, None, shells.ctypes.data)
+        return tab, shells[:rows.value - 2].copy()
+
+    def set_vote_table(self, tab):
+        """Replaces the vote table (rows, cells, 8) until the next weight change (test hook)."""
+        tab = np.ascontiguousarray(tab, np.float32)
+        call("tira_plate_mlp_set_vote_table", self.handle, tab.ctypes.data, tab.size)
+
     def vote(self, field, out, shape, w, stick=True, tol=1e-3, in_layout=_lib.LAYOUT_MAT9,
-             out_layout=_lib.LAYOUT_PACKED6, count_fallback=False):
+             out_layout=_lib.LAYOUT_PACKED6, count_fallback=False, method=_lib.MLP_TABLE):
         """Stick + learned plate vote of a device field; returns the number of
-        integrator-fallback voters when count_fallback (synchronises)."""
+        integrator-fallback voters when count_fallback (synchronises).
+        method: _lib.MLP_TABLE (tabulated network, default) or _lib.MLP_PER_PAIR."""
         nf = C.c_ulonglong(0)
-        call("tira_tensorvote3_mlp", self.handle, field.ptr, in_layout, out.ptr, out_layout, *[int(s) for s in shape],
-             int(w), int(stick), float(tol), C.byref(nf) if count_fallback else None)
+        call("tira_tensorvote3_mlp_method", self.handle, field.ptr, in_layout, out.ptr, out_layout,
+             *[int(s) for s in shape], int(w), int(stick), float(tol), C.byref(nf) if count_fallback else None,
+             int(method))
         return nf.value if count_fallback else None
 
     def close(self):

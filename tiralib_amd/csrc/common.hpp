@@ -48,6 +48,7 @@ enum Slot {
     WS_PAY0, WS_PAY1, WS_TABLE, WS_TABLE2, WS_SCRATCH0, WS_SCRATCH1, WS_SCRATCH2, WS_COUNTER,
     WS_TABLE3, WS_PAY2, WS_PIPE0, WS_PIPE1, WS_SPEC0, WS_SPEC1, WS_THIN0, WS_THIN1,
     WS_NB2,  // the 2D neighbour table of the last stick/plate vote (kept while unchanged)
+    WS_TABLE4,  // offset table of the tabulated-network vote
     WS_NSLOTS
 };
 int ws_get(int slot, size_t bytes, void** p);

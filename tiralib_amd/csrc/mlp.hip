@@ -179,6 +179,9 @@ __global__ __launch_bounds__(256) void k_mlp_samples(size_t n, unsigned long lon
     }
 }
 
+// Frobenius weight of the 6 packed components (off-diagonals count twice).
+__device__ __forceinline__ float comp_weight(int o) { return (o == 1 || o == 2 || o == 4) ? 2.0f : 1.0f; }
+
 // Centre normal of orientation bin b (the bins of mlp_bin).
 __device__ __forceinline__ void bin_normal(int b, double nv[3]) {
     const double pi = 3.14159265358979323846;
@@ -218,13 +221,123 @@ __global__ __launch_bounds__(256) void k_mlp_bin_inputs(size_t nrows, size_t nva
     }
 }
 
+// Per-row weighted errors of the error table against the FP64 integrator
+// targets T: the network evaluated per pair (Y, plate-frame outputs) into
+// rs_net, the tabulated network (lookup of the offset's distance row, offs.w,
+// at s = n.d) into rs_tab; (sum w (pred - T)^2 c1^2, sum w T^2 c1^2) per row.
+__global__ __launch_bounds__(256) void k_mlp_bin_rows(size_t nvalid, int b0, int noff, const float4* __restrict__ offs,
+                                                      const float* __restrict__ Y, const float* __restrict__ T,
+                                                      const float* __restrict__ wgt, MlpAffine aff,
+                                                      const float4* __restrict__ vtab, float* __restrict__ rs_net,
+                                                      float* __restrict__ rs_tab) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nvalid; i += (size_t)gridDim.x * blockDim.x) {
+        const int b = b0 + (int)(i / noff);
+        const float4 o = offs[i % noff];
+        double nv[3];
+        bin_normal(b, nv);
+        const float n0 = (float)nv[0], n1 = (float)nv[1], n2 = (float)nv[2];
+        const float il = rsqrtf(o.x * o.x + o.y * o.y + o.z * o.z);
+        const float s = fmaf(n0, o.x * il, fmaf(n1, o.y * il, n2 * (o.z * il)));
+        const float x = fmaf(-s, s, 1.0f);
+        const float4 v = mlp_tab_lookup(vtab + (size_t)o.w * 2 * kMlpTabM, x);
+        const float sn = sqrtf(fmaxf(x, 0.0f));
+        const float tab[kMlpOut] = {fmaf(2.0f * v.w, x, v.x), 0.0f, sn * s * (v.z + 2.0f * v.w), v.x, 0.0f,
+                                    v.x + v.y + 2.0f * s * s * (v.z + v.w)};
+        const float* t = T + i * 8;
+        float en = 0.0f, et = 0.0f, tt = 0.0f;
+        for (int c = 0; c < kMlpOut; c++) {
+            const float w = comp_weight(c);
+            const float yn = fmaf(aff.sd[c], Y[i * kMlpOutPad + c], aff.mu[c]);
+            en += w * (yn - t[c]) * (yn - t[c]);
+            et += w * (tab[c] - t[c]) * (tab[c] - t[c]);
+            tt += w * t[c] * t[c];
+        }
+        const float rw = wgt[i];
+        rs_net[2 * i] = en * rw;
+        rs_net[2 * i + 1] = tt * rw;
+        rs_tab[2 * i] = et * rw;
+        rs_tab[2 * i + 1] = tt * rw;
+    }
+}
+
+// ---------------------------------------------------------------- tabulated network
+// Input rows of the vote table: row i = (distance row k, cell j, sign) with
+// k = i / (2 M), j = (i / 2) % M, s = +-sqrt(1 - x_j), x_j = (j + 0.5) / M;
+// the features of an offset at s in the offset-aligned frame (see
+// plate_target): (sqrt(x_j), 0, s, rho_k, 2 s^2 - 1). Rows past nvalid are
+// zero.
+__global__ __launch_bounds__(256) void k_mlp_tab_inputs(size_t nrows, size_t nvalid, const float* __restrict__ rho,
+                                                        u16* __restrict__ X) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nrows; i += (size_t)gridDim.x * blockDim.x) {
+        float f[kMlpFeat] = {0, 0, 0, 0, 0};
+        if (i < nvalid) {
+            const size_t k = i / (2 * kMlpTabM);
+            const int j = (int)((i / 2) % kMlpTabM);
+            const double x = (j + 0.5) / kMlpTabM;
+            const double s = (i & 1) ? -sqrt(1.0 - x) : sqrt(1.0 - x);
+            f[0] = (float)sqrt(x);
+            f[2] = (float)s;
+            f[3] = rho[k];
+            f[4] = (float)(2.0 * s * s - 1.0);
+        }
+        write_input_row(X + i * kMlpIn, f);
+    }
+}
+
+// (A, B, g, D/2) of distance row k at cell j from the network outputs Y (the
+// rows k_mlp_tab_inputs wrote), symmetrised under s -> -s.
+__device__ double4 tab_coeffs(const float* __restrict__ Y, const MlpAffine& aff, int k, int j) {
+    const size_t ip = ((size_t)k * kMlpTabM + j) * 2, im = ip + 1;
+    double yp[kMlpOut], ym[kMlpOut];
+    for (int c = 0; c < kMlpOut; c++) {
+        yp[c] = (double)fmaf(aff.sd[c], Y[ip * kMlpOutPad + c], aff.mu[c]);
+        ym[c] = (double)fmaf(aff.sd[c], Y[im * kMlpOutPad + c], aff.mu[c]);
+    }
+    const double uu = 0.5 * (yp[0] + ym[0]), un = 0.5 * (yp[2] - ym[2]);
+    const double vv = 0.5 * (yp[3] + ym[3]), nn = 0.5 * (yp[5] + ym[5]);
+    const double x = (j + 0.5) / kMlpTabM, s = sqrt(1.0 - x), sn = sqrt(x);
+    const double D = (uu - vv) / x;
+    const double g = -D + un / (s * sn);
+    const double B = (nn - vv) - s * s * D - 2.0 * s * s * g;
+    return make_double4(vv, B, g, 0.5 * D);
+}
+
+// Vote table rows 0..nrows-1: cell j holds (A, B, g, D/2) at x_j and the
+// difference to x_(j+1). Row 0 (the voter itself, d = 0) is the integrator's
+// self vote (pi / 2)(I - n n^T).
+__global__ __launch_bounds__(256) void k_mlp_tab_convert(const float* __restrict__ Y, int nrows, MlpAffine aff,
+                                                         float4* __restrict__ tab) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrows * kMlpTabM) return;
+    const int k = i / kMlpTabM, j = i % kMlpTabM;
+    float4* cell = tab + 2 * (size_t)i;
+    if (k == 0) {
+        const float hp = (float)(0.5 * 3.14159265358979323846);
+        cell[0] = make_float4(hp, -hp, 0.0f, 0.0f);
+        cell[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    const double4 v = tab_coeffs(Y, aff, k, j);
+    double4 dl = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (j + 1 < kMlpTabM) {
+        const double4 n = tab_coeffs(Y, aff, k, j + 1);
+        dl = make_double4(n.x - v.x, n.y - v.y, n.z - v.z, n.w - v.w);
+    }
+    cell[0] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
+    cell[1] = make_float4((float)dl.x, (float)dl.y, (float)dl.z, (float)dl.w);
+}
+
 // ---------------------------------------------------------------- GEMMs
-enum { kEpiF32 = 0, kEpiReluBf16 = 1, kEpiMaskBf16 = 2 };
+enum { kEpiF32 = 0, kEpiReluSplit = 1, kEpiMaskBf16 = 2 };
 
 // C[rows x cols] = A[rows x K] B[cols x K]^T (BF16 in, FP32 accumulate).
-// One 32x32 tile per wave, four waves per block along rows. Epilogues:
-// FP32 store; ReLU + BF16; BF16 masked by H > 0 (the ReLU derivative).
-template <int EPI>
+// One 32x32 tile per wave, four waves per block along rows. SPLIT_A: A holds
+// hidden activations as a BF16 high part (columns 0..K-1) and a BF16
+// residual (columns K..2K-1) of the same row; both meet the same B columns,
+// so the product sees ~16-bit activations. Epilogues: FP32 store; ReLU
+// split into high part (C[row][col]) and residual (C[row][kMlpHid + col]);
+// BF16 masked by H > 0 (the ReLU derivative, on H's high part).
+template <int EPI, bool SPLIT_A>
 __global__ __launch_bounds__(256) void k_gemm_nt(const u16* __restrict__ A, int lda, const u16* __restrict__ B,
                                                  int ldb, int K, int rows, void* __restrict__ C, int ldc,
                                                  const u16* __restrict__ H, int ldh) {
@@ -237,9 +350,13 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const u16* __restrict__ A, int 
     const u16* pa = A + (size_t)(row0 + m) * lda + 8 * h;
     const u16* pb = B + (size_t)(col0 + m) * ldb + 8 * h;
     for (int s = 0; s < K / 16; s++) {
-        const bf16x8 a = *reinterpret_cast<const bf16x8*>(pa + 16 * s);
         const bf16x8 b = *reinterpret_cast<const bf16x8*>(pb + 16 * s);
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(pa + 16 * s);
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        if (SPLIT_A) {
+            const bf16x8 al = *reinterpret_cast<const bf16x8*>(pa + K + 16 * s);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, b, acc, 0, 0, 0);
+        }
     }
 #pragma unroll
     for (int r = 0; r < 16; r++) {
@@ -247,8 +364,11 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const u16* __restrict__ A, int 
         const size_t o = (size_t)row * ldc + col0 + m;
         if (EPI == kEpiF32) {
             reinterpret_cast<float*>(C)[o] = acc[r];
-        } else if (EPI == kEpiReluBf16) {
-            reinterpret_cast<u16*>(C)[o] = to_bf16(fmaxf(acc[r], 0.0f));
+        } else if (EPI == kEpiReluSplit) {
+            const float v = fmaxf(acc[r], 0.0f);
+            const u16 hi = to_bf16(v);
+            reinterpret_cast<u16*>(C)[o] = hi;
+            reinterpret_cast<u16*>(C)[o + kMlpHid] = to_bf16(v - from_bf16(hi));
         } else {
             const short hv = (short)H[(size_t)row * ldh + col0 + m];
             reinterpret_cast<u16*>(C)[o] = hv > 0 ? to_bf16(acc[r]) : (u16)0;
@@ -258,62 +378,82 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const u16* __restrict__ A, int 
 
 constexpr int kTnChunk = 256;
 constexpr int kTnPad = kTnChunk + 8;
+constexpr int kTnMaxSplit = 64;  // row splits of a weight-gradient product
 
-// C[N x K] += sum_b A[b, n] B[b, k] over b in [0, rows): 32x32 output tile
-// (blockIdx.y, blockIdx.z), 256-sample chunk blockIdx.x.
+// Weight-gradient product, split over the batch without atomics:
+//   part[split][n][k] = sum over chunks c = split, split + nsplit, ... of
+//                       sum_{b in chunk c} A[b, n] B[b, k]
+// for the 32x32 output tile (blockIdx.y, blockIdx.z), split blockIdx.x.
+// Every chunk is summed in a fixed order, so the result does not depend on
+// how the blocks are scheduled; k_tn_reduce then adds the splits in order.
 __global__ __launch_bounds__(256) void k_gemm_tn(const u16* __restrict__ A, int lda, const u16* __restrict__ B,
-                                                 int ldb, int rows, float* __restrict__ C, int ldc) {
+                                                 int ldb, int rows, float* __restrict__ part, int ldc,
+                                                 size_t part_stride) {
     __shared__ __attribute__((aligned(16))) u16 sA[32 * kTnPad];
     __shared__ __attribute__((aligned(16))) u16 sB[32 * kTnPad];
     __shared__ float sC[4 * 32 * 32];
     const int tid = threadIdx.x;
     const int n0 = blockIdx.y * 32, k0 = blockIdx.z * 32;
-    const int b0 = blockIdx.x * kTnChunk;
-    for (int seg = tid; seg < kTnChunk * 4; seg += 256) {
-        const int b = seg >> 2, c8 = (seg & 3) * 8;
-        uint4 va = make_uint4(0, 0, 0, 0), vb = make_uint4(0, 0, 0, 0);
-        if (b0 + b < rows) {
-            va = *reinterpret_cast<const uint4*>(A + (size_t)(b0 + b) * lda + n0 + c8);
-            vb = *reinterpret_cast<const uint4*>(B + (size_t)(b0 + b) * ldb + k0 + c8);
-        }
-        const u16* ea = reinterpret_cast<const u16*>(&va);
-        const u16* eb = reinterpret_cast<const u16*>(&vb);
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            sA[(c8 + j) * kTnPad + b] = ea[j];
-            sB[(c8 + j) * kTnPad + b] = eb[j];
-        }
-    }
-    __syncthreads();
     const int wave = tid >> 6, l = tid & 63, m = l & 31, h = l >> 5;
     f32x16 acc = {};
+    for (int b0 = blockIdx.x * kTnChunk; b0 < rows; b0 += gridDim.x * kTnChunk) {
+        for (int seg = tid; seg < kTnChunk * 4; seg += 256) {
+            const int b = seg >> 2, c8 = (seg & 3) * 8;
+            uint4 va = make_uint4(0, 0, 0, 0), vb = make_uint4(0, 0, 0, 0);
+            if (b0 + b < rows) {
+                va = *reinterpret_cast<const uint4*>(A + (size_t)(b0 + b) * lda + n0 + c8);
+                vb = *reinterpret_cast<const uint4*>(B + (size_t)(b0 + b) * ldb + k0 + c8);
+            }
+            const u16* ea = reinterpret_cast<const u16*>(&va);
+            const u16* eb = reinterpret_cast<const u16*>(&vb);
 #pragma unroll
-    for (int s = 0; s < kTnChunk / 64; s++) {
-        const int bb = wave * (kTnChunk / 4) + 16 * s + 8 * h;
-        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sA[m * kTnPad + bb]);
-        const bf16x8 b = *reinterpret_cast<const bf16x8*>(&sB[m * kTnPad + bb]);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+            for (int j = 0; j < 8; j++) {
+                sA[(c8 + j) * kTnPad + b] = ea[j];
+                sB[(c8 + j) * kTnPad + b] = eb[j];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < kTnChunk / 64; s++) {
+            const int bb = wave * (kTnChunk / 4) + 16 * s + 8 * h;
+            const bf16x8 a = *reinterpret_cast<const bf16x8*>(&sA[m * kTnPad + bb]);
+            const bf16x8 b = *reinterpret_cast<const bf16x8*>(&sB[m * kTnPad + bb]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+        __syncthreads();
     }
 #pragma unroll
     for (int r = 0; r < 16; r++) sC[wave * 1024 + (8 * (r >> 2) + 4 * h + (r & 3)) * 32 + m] = acc[r];
     __syncthreads();
+    float* dst = part + (size_t)blockIdx.x * part_stride;
     for (int e = tid; e < 1024; e += 256) {
-        const float v = sC[e] + sC[1024 + e] + sC[2048 + e] + sC[3072 + e];
-        atomicAdd(&C[(size_t)(n0 + (e >> 5)) * ldc + k0 + (e & 31)], v);
+        const float v = ((sC[e] + sC[1024 + e]) + sC[2048 + e]) + sC[3072 + e];
+        dst[(size_t)(n0 + (e >> 5)) * ldc + k0 + (e & 31)] = v;
     }
 }
 
+// C[e] = sum over s = 0 .. nsplit-1 (in order) of part[s][e].
+__global__ __launch_bounds__(256) void k_tn_reduce(const float* __restrict__ part, int nsplit, size_t stride,
+                                                   int n, float* __restrict__ C) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    float v = 0.0f;
+    for (int s = 0; s < nsplit; s++) v += part[(size_t)s * stride + e];
+    C[e] = v;
+}
+
 // ---------------------------------------------------------------- loss, Adam, fills
-__device__ __forceinline__ float comp_weight(int o) { return (o == 1 || o == 2 || o == 4) ? 2.0f : 1.0f; }
 
 // Frobenius-weighted squared error on the 6 components (off-diagonals count
-// twice) of mu + sd * Y against T. dY (optional) = gscale * d/dY. sums[0] += sum w (Y-T)^2 * rw,
-// sums[1] += sum w T^2 * rw, rw = row weight (1 when rowwgt is NULL).
-// perbin (optional): the same two sums per group of `group` rows.
+// twice) of mu + sd * Y against T. dY (optional) = gscale * d/dY. Block b
+// writes its two sums, sum w (Y-T)^2 * rw and sum w T^2 * rw (rw = row
+// weight, 1 when rowwgt is NULL), to part[2 (blk0 + b) .. + 1]; the block sum
+// is a fixed tree, so the partials (and k_sum_partials over them) do not
+// depend on scheduling. rowsums (optional): the same two sums per row.
 __global__ __launch_bounds__(256) void k_mlp_loss(const float* __restrict__ Y, const float* __restrict__ T, int rows,
-                                                  u16* __restrict__ dY, float gscale, float* __restrict__ sums,
-                                                  const float* __restrict__ rowwgt, float* __restrict__ perbin,
-                                                  int group, MlpAffine aff) {
+                                                  u16* __restrict__ dY, float gscale, float* __restrict__ part,
+                                                  int blk0, const float* __restrict__ rowwgt,
+                                                  float* __restrict__ rowsums, MlpAffine aff) {
     __shared__ float s_e[256], s_t[256];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     float e = 0.0f, t = 0.0f;
@@ -332,9 +472,9 @@ __global__ __launch_bounds__(256) void k_mlp_loss(const float* __restrict__ Y, c
                 dY[(size_t)i * kMlpOutPad + o] = 0;
             }
         }
-        if (perbin) {
-            atomicAdd(&perbin[2 * (i / group)], e);
-            atomicAdd(&perbin[2 * (i / group) + 1], t);
+        if (rowsums) {
+            rowsums[2 * (size_t)i] = e;
+            rowsums[2 * (size_t)i + 1] = t;
         }
     }
     s_e[threadIdx.x] = e;
@@ -347,9 +487,38 @@ __global__ __launch_bounds__(256) void k_mlp_loss(const float* __restrict__ Y, c
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0 && sums) {
-        atomicAdd(&sums[0], s_e[0]);
-        atomicAdd(&sums[1], s_t[0]);
+    if (threadIdx.x == 0 && part) {
+        part[2 * (size_t)(blk0 + blockIdx.x)] = s_e[0];
+        part[2 * (size_t)(blk0 + blockIdx.x) + 1] = s_t[0];
+    }
+}
+
+// Sums pairs of partials in a fixed order: out[2 g + c] = sum over the
+// `count` pairs i of group g (pairs g * count .. g * count + count - 1) of
+// in[2 i + c]. One block per group; each thread adds a strided run in
+// order, then a fixed tree combines the threads.
+__global__ __launch_bounds__(256) void k_sum_partials(const float* __restrict__ in, size_t count,
+                                                      float* __restrict__ out) {
+    __shared__ float s_a[256], s_b[256];
+    const float* src = in + 2 * (size_t)blockIdx.x * count;
+    float a = 0.0f, b = 0.0f;
+    for (size_t i = threadIdx.x; i < count; i += 256) {
+        a += src[2 * i];
+        b += src[2 * i + 1];
+    }
+    s_a[threadIdx.x] = a;
+    s_b[threadIdx.x] = b;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            s_a[threadIdx.x] += s_a[threadIdx.x + s];
+            s_b[threadIdx.x] += s_b[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[2 * (size_t)blockIdx.x] = s_a[0];
+        out[2 * (size_t)blockIdx.x + 1] = s_b[0];
     }
 }
 
@@ -424,75 +593,109 @@ static const Layer kLayers[4] = {
 };
 constexpr int kMlpNWt = 2 * kMlpHid * kMlpHid + kMlpHid * kMlpOutPad;
 
-// Activations of one batch.
+// Activations of one batch, plus the partial-sum buffers of the
+// deterministic reductions: `part` holds kTnMaxSplit split tiles of every
+// layer's weight gradient, `lpart` one (error, target) pair per loss block.
 struct Acts {
     size_t cap = 0;
     u16 *h1 = nullptr, *h2 = nullptr, *h3 = nullptr, *dy = nullptr, *dz1 = nullptr, *dz2 = nullptr, *dz3 = nullptr;
     float* y = nullptr;
     float* sums = nullptr;
-    ~Acts() {
+    float* part = nullptr;
+    float* lpart = nullptr;
+    ~Acts() { release(); hipFree(sums); hipFree(part); }
+    void release() {
         hipFree(h1); hipFree(h2); hipFree(h3); hipFree(dy); hipFree(dz1); hipFree(dz2); hipFree(dz3);
-        hipFree(y); hipFree(sums);
+        hipFree(y); hipFree(lpart);
+        h1 = h2 = h3 = dy = dz1 = dz2 = dz3 = nullptr;
+        y = lpart = nullptr;
     }
     int reserve(size_t rows) {
         if (rows <= cap) return TIRA_OK;
-        hipFree(h1); hipFree(h2); hipFree(h3); hipFree(dy); hipFree(dz1); hipFree(dz2); hipFree(dz3); hipFree(y);
-        h1 = h2 = h3 = dy = dz1 = dz2 = dz3 = nullptr;
-        y = nullptr;
-        const size_t hb = rows * kMlpHid * 2;
-        TH_HIP(hipMalloc(&h1, hb)); TH_HIP(hipMalloc(&h2, hb)); TH_HIP(hipMalloc(&h3, hb));
+        release();
+        cap = 0;
+        const size_t hb = rows * kMlpHid * 2, hs = rows * kMlpHidSplit * 2;
+        TH_HIP(hipMalloc(&h1, hs)); TH_HIP(hipMalloc(&h2, hs)); TH_HIP(hipMalloc(&h3, hs));
         TH_HIP(hipMalloc(&dz1, hb)); TH_HIP(hipMalloc(&dz2, hb)); TH_HIP(hipMalloc(&dz3, hb));
         TH_HIP(hipMalloc(&dy, rows * kMlpOutPad * 2));
         TH_HIP(hipMalloc(&y, rows * kMlpOutPad * 4));
+        TH_HIP(hipMalloc(&lpart, ((rows + 255) / 256) * 8));
         if (!sums) TH_HIP(hipMalloc(&sums, 64));
+        if (!part) TH_HIP(hipMalloc(&part, (size_t)kTnMaxSplit * kMlpNParam * 4));
         cap = rows;
         return TIRA_OK;
     }
 };
 
-static void launch_nt(int epi, const u16* A, int lda, const u16* B, int ldb, int K, int rows, int cols, void* C,
-                      int ldc, const u16* H, int ldh, hipStream_t st) {
+static void launch_nt(int epi, bool split_a, const u16* A, int lda, const u16* B, int ldb, int K, int rows, int cols,
+                      void* C, int ldc, const u16* H, int ldh, hipStream_t st) {
     const dim3 grid(ceil_div(rows, 128), cols / 32);
-    if (epi == kEpiF32) k_gemm_nt<kEpiF32><<<grid, 256, 0, st>>>(A, lda, B, ldb, K, rows, C, ldc, H, ldh);
-    else if (epi == kEpiReluBf16) k_gemm_nt<kEpiReluBf16><<<grid, 256, 0, st>>>(A, lda, B, ldb, K, rows, C, ldc, H, ldh);
-    else k_gemm_nt<kEpiMaskBf16><<<grid, 256, 0, st>>>(A, lda, B, ldb, K, rows, C, ldc, H, ldh);
+    if (epi == kEpiF32) {
+        if (split_a) k_gemm_nt<kEpiF32, true><<<grid, 256, 0, st>>>(A, lda, B, ldb, K, rows, C, ldc, H, ldh);
+        else k_gemm_nt<kEpiF32, false><<<grid, 256, 0, st>>>(A, lda, B, ldb, K, rows, C, ldc, H, ldh);
+    } else if (epi == kEpiReluSplit) {
+        if (split_a) k_gemm_nt<kEpiReluSplit, true><<<grid, 256, 0, st>>>(A, lda, B, ldb, K, rows, C, ldc, H, ldh);
+        else k_gemm_nt<kEpiReluSplit, false><<<grid, 256, 0, st>>>(A, lda, B, ldb, K, rows, C, ldc, H, ldh);
+    } else {
+        k_gemm_nt<kEpiMaskBf16, false><<<grid, 256, 0, st>>>(A, lda, B, ldb, K, rows, C, ldc, H, ldh);
+    }
 }
 
+// C[N x K] = A^T B over `rows` samples (N = ncols_a, K = ncols_b): split
+// partial tiles into `part`, then their ordered sum into C.
 static void launch_tn(const u16* A, int lda, int ncols_a, const u16* B, int ldb, int ncols_b, int rows, float* C,
-                      int ldc, hipStream_t st) {
-    const dim3 grid(ceil_div(rows, kTnChunk), ncols_a / 32, ncols_b / 32);
-    k_gemm_tn<<<grid, 256, 0, st>>>(A, lda, B, ldb, rows, C, ldc);
+                      int ldc, float* part, hipStream_t st) {
+    const int nsplit = (int)std::min<size_t>(kTnMaxSplit, ceil_div(rows, kTnChunk));
+    const size_t stride = (size_t)ncols_a * ldc;
+    const dim3 grid(nsplit, ncols_a / 32, ncols_b / 32);
+    k_gemm_tn<<<grid, 256, 0, st>>>(A, lda, B, ldb, rows, part, ldc, stride);
+    const int n = ncols_a * ldc;
+    k_tn_reduce<<<ceil_div(n, 256), 256, 0, st>>>(part, nsplit, stride, n, C);
+}
+
+// Loss sums of `rows` rows into sums[0..1] (deterministic: block partials,
+// then their ordered sum).
+static void loss_sums(const float* Y, const float* T, int rows, u16* dY, float gscale, const MlpAffine& aff,
+                      Acts& a, hipStream_t st) {
+    const unsigned nb = ceil_div(rows, 256);
+    k_mlp_loss<<<nb, 256, 0, st>>>(Y, T, rows, dY, gscale, a.lpart, 0, nullptr, nullptr, aff);
+    k_sum_partials<<<1, 256, 0, st>>>(a.lpart, nb, a.sums);
 }
 
 // Forward pass of `rows` samples (a multiple of 32): Y (rows x 32 FP32).
+// Hidden activations H1..H3 are rows of kMlpHidSplit BF16: the high parts of
+// the 64 units, then their residuals.
 static void mlp_forward(PlateMlp* m, const u16* X, int rows, Acts& a, hipStream_t st) {
     const u16* W = m->wb;
-    launch_nt(kEpiReluBf16, X, kMlpIn, W + kMlpW1, kMlpIn, kMlpIn, rows, kMlpHid, a.h1, kMlpHid, nullptr, 0, st);
-    launch_nt(kEpiReluBf16, a.h1, kMlpHid, W + kMlpW2, kMlpHid, kMlpHid, rows, kMlpHid, a.h2, kMlpHid, nullptr, 0, st);
-    launch_nt(kEpiReluBf16, a.h2, kMlpHid, W + kMlpW3, kMlpHid, kMlpHid, rows, kMlpHid, a.h3, kMlpHid, nullptr, 0, st);
-    launch_nt(kEpiF32, a.h3, kMlpHid, W + kMlpW4, kMlpHid, kMlpHid, rows, kMlpOutPad, a.y, kMlpOutPad, nullptr, 0, st);
+    launch_nt(kEpiReluSplit, false, X, kMlpIn, W + kMlpW1, kMlpIn, kMlpIn, rows, kMlpHid, a.h1, kMlpHidSplit, nullptr,
+              0, st);
+    launch_nt(kEpiReluSplit, true, a.h1, kMlpHidSplit, W + kMlpW2, kMlpHid, kMlpHid, rows, kMlpHid, a.h2,
+              kMlpHidSplit, nullptr, 0, st);
+    launch_nt(kEpiReluSplit, true, a.h2, kMlpHidSplit, W + kMlpW3, kMlpHid, kMlpHid, rows, kMlpHid, a.h3,
+              kMlpHidSplit, nullptr, 0, st);
+    launch_nt(kEpiF32, true, a.h3, kMlpHidSplit, W + kMlpW4, kMlpHid, kMlpHid, rows, kMlpOutPad, a.y, kMlpOutPad,
+              nullptr, 0, st);
 }
 
 // Forward + backward of one batch (rows a multiple of 256): gradients into
-// m->grads (zeroed here), loss sums into a.sums.
+// m->grads, loss sums into a.sums. Weight gradients take the activations'
+// high parts; every reduction runs in a fixed order, so the same weights and
+// batch give bit-identical gradients on every run.
 static void mlp_fwd_bwd(PlateMlp* m, const u16* X, const float* T, int rows, Acts& a, hipStream_t st) {
     mlp_forward(m, X, rows, a, st);
-    fill(m->grads, kMlpNParam, 0.0f, st);
-    fill(a.sums, 2, 0.0f, st);
-    k_mlp_loss<<<ceil_div(rows, 256), 256, 0, st>>>(a.y, T, rows, a.dy, 1.0f / (float)rows, a.sums, nullptr,
-                                                    nullptr, 1, m->aff);
+    loss_sums(a.y, T, rows, a.dy, 1.0f / (float)rows, m->aff, a, st);
     const u16* WT = m->wtb;
     float* G = m->grads;
-    launch_tn(a.dy, kMlpOutPad, kMlpOutPad, a.h3, kMlpHid, kMlpHid, rows, G + kMlpW4, kMlpHid, st);
-    launch_nt(kEpiMaskBf16, a.dy, kMlpOutPad, WT + kLayers[3].wt_off, kMlpOutPad, kMlpOutPad, rows, kMlpHid, a.dz3,
-              kMlpHid, a.h3, kMlpHid, st);
-    launch_tn(a.dz3, kMlpHid, kMlpHid, a.h2, kMlpHid, kMlpHid, rows, G + kMlpW3, kMlpHid, st);
-    launch_nt(kEpiMaskBf16, a.dz3, kMlpHid, WT + kLayers[2].wt_off, kMlpHid, kMlpHid, rows, kMlpHid, a.dz2, kMlpHid,
-              a.h2, kMlpHid, st);
-    launch_tn(a.dz2, kMlpHid, kMlpHid, a.h1, kMlpHid, kMlpHid, rows, G + kMlpW2, kMlpHid, st);
-    launch_nt(kEpiMaskBf16, a.dz2, kMlpHid, WT + kLayers[1].wt_off, kMlpHid, kMlpHid, rows, kMlpHid, a.dz1, kMlpHid,
-              a.h1, kMlpHid, st);
-    launch_tn(a.dz1, kMlpHid, kMlpHid, X, kMlpIn, kMlpIn, rows, G + kMlpW1, kMlpIn, st);
+    launch_tn(a.dy, kMlpOutPad, kMlpOutPad, a.h3, kMlpHidSplit, kMlpHid, rows, G + kMlpW4, kMlpHid, a.part, st);
+    launch_nt(kEpiMaskBf16, false, a.dy, kMlpOutPad, WT + kLayers[3].wt_off, kMlpOutPad, kMlpOutPad, rows, kMlpHid,
+              a.dz3, kMlpHid, a.h3, kMlpHidSplit, st);
+    launch_tn(a.dz3, kMlpHid, kMlpHid, a.h2, kMlpHidSplit, kMlpHid, rows, G + kMlpW3, kMlpHid, a.part, st);
+    launch_nt(kEpiMaskBf16, false, a.dz3, kMlpHid, WT + kLayers[2].wt_off, kMlpHid, kMlpHid, rows, kMlpHid, a.dz2,
+              kMlpHid, a.h2, kMlpHidSplit, st);
+    launch_tn(a.dz2, kMlpHid, kMlpHid, a.h1, kMlpHidSplit, kMlpHid, rows, G + kMlpW2, kMlpHid, a.part, st);
+    launch_nt(kEpiMaskBf16, false, a.dz2, kMlpHid, WT + kLayers[1].wt_off, kMlpHid, kMlpHid, rows, kMlpHid, a.dz1,
+              kMlpHid, a.h1, kMlpHidSplit, st);
+    launch_tn(a.dz1, kMlpHid, kMlpHid, X, kMlpIn, kMlpIn, rows, G + kMlpW1, kMlpIn, a.part, st);
 }
 
 static void mlp_adam(PlateMlp* m, float lr, hipStream_t st) {
@@ -557,6 +760,66 @@ static void init_params(std::vector<float>& p, std::vector<unsigned char>& mask,
         }
 }
 
+// ---------------------------------------------------------------- vote table
+void mlp_shells(float sigma1, std::vector<int>& l2s) {
+    l2s.clear();
+    const int top = (int)std::ceil(9.0 * (double)sigma1 * sigma1) + 1;
+    for (int l2 = 1; l2 <= top; l2++) {
+        // Legendre: l2 is a sum of three squares unless l2 = 4^a (8 b + 7)
+        int q = l2;
+        while (q % 4 == 0) q /= 4;
+        if (q % 8 == 7) continue;
+        if (std::min(std::sqrt((float)l2) / sigma1, kMlpRhoMax) < kMlpRhoMax) l2s.push_back(l2);
+    }
+}
+
+int mlp_shell_row(const std::vector<int>& l2s, int l2) {
+    if (l2 == 0) return 0;
+    const auto it = std::lower_bound(l2s.begin(), l2s.end(), l2);
+    if (it != l2s.end() && *it == l2) return 1 + (int)(it - l2s.begin());
+    return (int)l2s.size() + 1;
+}
+
+// Evaluates the network on the grid of the vote table (every distance row,
+// kMlpTabM cells, +s and -s) with the MFMA forward pass and writes vtab.
+static int mlp_build_vote_table(PlateMlp* m, hipStream_t st) {
+    mlp_shells(m->sigma1, m->shells);
+    const int nrows = (int)m->shells.size() + 2;
+    std::vector<float> rho(nrows, 0.0f);
+    for (size_t k = 0; k < m->shells.size(); k++) rho[k + 1] = std::sqrt((float)m->shells[k]) / m->sigma1;
+    rho[nrows - 1] = kMlpRhoMax;
+    const size_t nvalid = (size_t)nrows * kMlpTabM * 2, rows = (nvalid + 255) / 256 * 256;
+    if (m->vtab_rows != nrows) {
+        hipFree(m->vtab);
+        m->vtab = nullptr;
+        m->vtab_rows = 0;
+        TH_HIP(hipMalloc(&m->vtab, (size_t)nrows * kMlpTabM * 32));
+        m->vtab_rows = nrows;
+    }
+    float* drho = nullptr;
+    u16* X = nullptr;
+    Acts a;
+    int rc = TIRA_OK;
+    do {
+        if (hipMalloc(&drho, nrows * 4) != hipSuccess || hipMalloc(&X, rows * kMlpIn * 2) != hipSuccess) {
+            rc = TIRA_ERR_OUT_OF_MEMORY;
+            set_error("MLP vote table: out of device memory");
+            break;
+        }
+        if ((rc = a.reserve(rows)) != TIRA_OK) break;
+        if (hipMemcpy(drho, rho.data(), nrows * 4, hipMemcpyHostToDevice) != hipSuccess) { rc = TIRA_ERR_HIP; break; }
+        k_mlp_tab_inputs<<<grid_for(rows), 256, 0, st>>>(rows, nvalid, drho, X);
+        mlp_forward(m, X, (int)rows, a, st);
+        k_mlp_tab_convert<<<ceil_div((size_t)nrows * kMlpTabM, 256), 256, 0, st>>>(a.y, nrows, m->aff, m->vtab);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
+            rc = TIRA_ERR_HIP;
+            set_error("MLP vote table: launch failed");
+        }
+    } while (0);
+    hipFree(drho); hipFree(X);
+    return rc;
+}
+
 // Per-lane MFMA operand layout of the vote kernel (v_mfma_f32_32x32x16_bf16:
 // lane l holds A[l % 32][8 (l / 32) + j], j = 0..7). Layer 1 takes the input
 // slots in natural order. Layers 2-4 take the previous layer's accumulator
@@ -590,6 +853,8 @@ int mlp_pack_inference(PlateMlp* m, hipStream_t st) {
             }
     if (!m->wreg) TH_HIP(hipMalloc(&m->wreg, pk.size() * 2));
     TH_HIP(hipMemcpy(m->wreg, pk.data(), pk.size() * 2, hipMemcpyHostToDevice));
+    TH_TRY(mlp_build_vote_table(m, st));
+    m->version++;
     return TIRA_OK;
 }
 
@@ -601,13 +866,13 @@ static int mlp_alloc(PlateMlp* m) {
     TH_HIP(hipMalloc(&m->mask, kMlpNParam));
     TH_HIP(hipMalloc(&m->wb, kMlpNParam * 2));
     TH_HIP(hipMalloc(&m->wtb, kMlpNWt * 2));
-    TH_HIP(hipMalloc(&m->bin_err, kMlpBins * 4));
+    TH_HIP(hipMalloc(&m->bin_err, 2 * kMlpBins * 4));
     return TIRA_OK;
 }
 
 static void mlp_free(PlateMlp* m) {
     hipFree(m->params); hipFree(m->grads); hipFree(m->adam_m); hipFree(m->adam_v); hipFree(m->mask);
-    hipFree(m->wb); hipFree(m->wtb); hipFree(m->wreg); hipFree(m->bin_err);
+    hipFree(m->wb); hipFree(m->wtb); hipFree(m->wreg); hipFree(m->bin_err); hipFree(m->vtab);
 }
 
 static int mlp_reset(PlateMlp* m, unsigned long long seed, hipStream_t st) {
@@ -616,19 +881,22 @@ static int mlp_reset(PlateMlp* m, unsigned long long seed, hipStream_t st) {
     init_params(p, mask, seed);
     TH_TRY(mlp_sync_bf16(m, p, st));
     TH_HIP(hipMemcpy(m->mask, mask.data(), kMlpNParam, hipMemcpyHostToDevice));
-    std::vector<float> z(kMlpNParam, 0.0f), ones(kMlpBins, 1.0f);
+    std::vector<float> z(kMlpNParam, 0.0f), ones(2 * kMlpBins, 1.0f);
     TH_HIP(hipMemcpy(m->adam_m, z.data(), kMlpNParam * 4, hipMemcpyHostToDevice));
     TH_HIP(hipMemcpy(m->adam_v, z.data(), kMlpNParam * 4, hipMemcpyHostToDevice));
-    TH_HIP(hipMemcpy(m->bin_err, ones.data(), kMlpBins * 4, hipMemcpyHostToDevice));
+    TH_HIP(hipMemcpy(m->bin_err, ones.data(), 2 * kMlpBins * 4, hipMemcpyHostToDevice));
     m->adam_t = 0;
     m->trained = false;
     m->aff = MlpAffine{{0, 0, 0, 0, 0, 0}, {1, 1, 1, 1, 1, 1}};
     return mlp_pack_inference(m, st);
 }
 
-// Relative Frobenius error of the network against the integrator for every
-// orientation bin: bin centre normal, every lattice offset 0 < |d| <= 3 sigma1,
-// weighted by c1^2 (the squared attenuation the vote applies).
+// Relative Frobenius error against the integrator for every orientation bin
+// of the plate normal: bin centre normal, every lattice offset
+// 0 < |d| <= 3 sigma1, weighted by c1^2 (the squared attenuation the vote
+// applies); for the network evaluated per pair (bin_err[0 .. kMlpBins)) and
+// for the tabulated network (bin_err[kMlpBins .. 2 kMlpBins)). Every sum runs
+// in a fixed order (per-row errors, then one ordered block sum per bin).
 static int mlp_bin_table(PlateMlp* m, hipStream_t st) {
     std::vector<float4> offs;
     const double R = kMlpRhoMax * m->sigma1;
@@ -636,9 +904,10 @@ static int mlp_bin_table(PlateMlp* m, hipStream_t st) {
     for (int dw = -ri; dw <= ri; dw++)
         for (int dv = -ri; dv <= ri; dv++)
             for (int du = -ri; du <= ri; du++) {
-                const double r2 = (double)du * du + (double)dv * dv + (double)dw * dw;
+                const int l2 = du * du + dv * dv + dw * dw;
+                const double r2 = (double)l2;
                 if (r2 == 0.0 || r2 > R * R) continue;
-                offs.push_back(make_float4((float)du, (float)dv, (float)dw, 0.0f));
+                offs.push_back(make_float4((float)du, (float)dv, (float)dw, (float)mlp_shell_row(m->shells, l2)));
             }
     const int noff = (int)offs.size();
     if (noff == 0) return TIRA_OK;
@@ -646,36 +915,38 @@ static int mlp_bin_table(PlateMlp* m, hipStream_t st) {
     const size_t cap = ((size_t)bins_per * noff + 255) / 256 * 256;
     float4* doffs = nullptr;
     u16* X = nullptr;
-    float *T = nullptr, *W = nullptr, *pb = nullptr;
+    float *T = nullptr, *W = nullptr, *pb = nullptr, *rs = nullptr;
     Acts a;
     int rc = TIRA_OK;
     do {
         if (hipMalloc(&doffs, offs.size() * 16) != hipSuccess || hipMalloc(&X, cap * kMlpIn * 2) != hipSuccess ||
             hipMalloc(&T, cap * 32) != hipSuccess || hipMalloc(&W, cap * 4) != hipSuccess ||
-            hipMalloc(&pb, (size_t)kMlpBins * 8) != hipSuccess) {
+            hipMalloc(&rs, cap * 16) != hipSuccess || hipMalloc(&pb, (size_t)kMlpBins * 16) != hipSuccess) {
             rc = TIRA_ERR_OUT_OF_MEMORY;
             set_error("MLP error table: out of device memory");
             break;
         }
         if ((rc = a.reserve(cap)) != TIRA_OK) break;
         if (hipMemcpy(doffs, offs.data(), offs.size() * 16, hipMemcpyHostToDevice) != hipSuccess) { rc = TIRA_ERR_HIP; break; }
-        fill(pb, (size_t)kMlpBins * 2, 0.0f, st);
+        float* rs_tab = rs + 2 * cap;
         for (int b0 = 0; b0 < kMlpBins; b0 += bins_per) {
             const int nb = std::min(bins_per, kMlpBins - b0);
             const size_t valid = (size_t)nb * noff, rows = (valid + 255) / 256 * 256;
             k_mlp_bin_inputs<<<grid_for(rows), 256, 0, st>>>(rows, valid, b0, noff, doffs, m->sigma1, m->sigma2,
                                                              m->power, m->nsamp, X, T, W);
             mlp_forward(m, X, (int)rows, a, st);
-            k_mlp_loss<<<ceil_div(valid, 256), 256, 0, st>>>(a.y, T, (int)valid, nullptr, 0.0f, nullptr, W,
-                                                              pb + 2 * (size_t)b0, noff, m->aff);
+            k_mlp_bin_rows<<<grid_for(valid), 256, 0, st>>>(valid, b0, noff, doffs, a.y, T, W, m->aff, m->vtab, rs,
+                                                            rs_tab);
+            k_sum_partials<<<nb, 256, 0, st>>>(rs, noff, pb + 2 * (size_t)b0);
+            k_sum_partials<<<nb, 256, 0, st>>>(rs_tab, noff, pb + 2 * ((size_t)kMlpBins + b0));
         }
-        std::vector<float> h((size_t)kMlpBins * 2), e(kMlpBins);
+        std::vector<float> h((size_t)kMlpBins * 4), e(2 * kMlpBins);
         if (hipStreamSynchronize(st) != hipSuccess ||
             hipMemcpy(h.data(), pb, h.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) { rc = TIRA_ERR_HIP; break; }
-        for (int b = 0; b < kMlpBins; b++) e[b] = h[2 * b + 1] > 0 ? std::sqrt(h[2 * b] / h[2 * b + 1]) : 0.0f;
-        if (hipMemcpy(m->bin_err, e.data(), kMlpBins * 4, hipMemcpyHostToDevice) != hipSuccess) rc = TIRA_ERR_HIP;
+        for (int b = 0; b < 2 * kMlpBins; b++) e[b] = h[2 * b + 1] > 0 ? std::sqrt(h[2 * b] / h[2 * b + 1]) : 0.0f;
+        if (hipMemcpy(m->bin_err, e.data(), 2 * kMlpBins * 4, hipMemcpyHostToDevice) != hipSuccess) rc = TIRA_ERR_HIP;
     } while (0);
-    hipFree(doffs); hipFree(X); hipFree(T); hipFree(W); hipFree(pb);
+    hipFree(doffs); hipFree(X); hipFree(T); hipFree(W); hipFree(pb); hipFree(rs);
     return rc;
 }
 
@@ -683,19 +954,20 @@ static int mlp_train(PlateMlp* m, size_t nsamples, unsigned epochs, unsigned bat
                      unsigned long long seed, float* loss_out, float* val_out, hipStream_t st) {
     const size_t B = batch;
     const size_t steps = nsamples / B;
-    const size_t nval = 1 << 18;
+    const size_t nval = 1 << 18, vchunk = 65536;
     u16 *X = nullptr, *XV = nullptr;
-    float *T = nullptr, *TV = nullptr;
+    float *T = nullptr, *TV = nullptr, *vpart = nullptr;
     Acts a;
     int rc = TIRA_OK;
     do {
         if (hipMalloc(&X, steps * B * kMlpIn * 2) != hipSuccess || hipMalloc(&T, steps * B * 32) != hipSuccess ||
-            hipMalloc(&XV, nval * kMlpIn * 2) != hipSuccess || hipMalloc(&TV, nval * 32) != hipSuccess) {
+            hipMalloc(&XV, nval * kMlpIn * 2) != hipSuccess || hipMalloc(&TV, nval * 32) != hipSuccess ||
+            hipMalloc(&vpart, (nval / 256) * 8) != hipSuccess) {
             rc = TIRA_ERR_OUT_OF_MEMORY;
             set_error("MLP training: out of device memory for %zu samples", steps * B);
             break;
         }
-        if ((rc = a.reserve(std::max(B, (size_t)65536))) != TIRA_OK) break;
+        if ((rc = a.reserve(std::max(B, vchunk))) != TIRA_OK) break;
         const double radius = kMlpRhoMax * (double)m->sigma1;
         k_mlp_samples<<<grid_for(steps * B), 256, 0, st>>>(steps * B, seed, m->sigma1, m->sigma2, m->power,
                                                           m->nsamp, radius, X, T, nullptr);
@@ -737,13 +1009,13 @@ static int mlp_train(PlateMlp* m, size_t nsamples, unsigned epochs, unsigned bat
         }
         if (rc != TIRA_OK) break;
         if (hipGetLastError() != hipSuccess) { rc = TIRA_ERR_HIP; set_error("MLP training launch failed"); break; }
-        // validation
-        fill(a.sums, 2, 0.0f, st);
-        for (size_t v0 = 0; v0 < nval; v0 += 65536) {
-            mlp_forward(m, XV + v0 * kMlpIn, 65536, a, st);
-            k_mlp_loss<<<65536 / 256, 256, 0, st>>>(a.y, TV + v0 * 8, 65536, nullptr, 0.0f, a.sums, nullptr, nullptr, 1,
-                                                    m->aff);
+        // validation: block partials of every chunk, then one ordered sum
+        for (size_t v0 = 0; v0 < nval; v0 += vchunk) {
+            mlp_forward(m, XV + v0 * kMlpIn, (int)vchunk, a, st);
+            k_mlp_loss<<<vchunk / 256, 256, 0, st>>>(a.y, TV + v0 * 8, (int)vchunk, nullptr, 0.0f, vpart,
+                                                     (int)(v0 / 256), nullptr, nullptr, m->aff);
         }
+        k_sum_partials<<<1, 256, 0, st>>>(vpart, nval / 256, a.sums);
         if (hipMemcpyAsync(hs.data(), a.sums, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipStreamSynchronize(st) != hipSuccess) { rc = TIRA_ERR_HIP; break; }
         m->last_loss = tgt_sum > 0 ? std::sqrt(loss_sum / tgt_sum) : 0.0f;
@@ -751,10 +1023,10 @@ static int mlp_train(PlateMlp* m, size_t nsamples, unsigned epochs, unsigned bat
         if (loss_out) *loss_out = m->last_loss;
         if (val_out) *val_out = m->last_val;
     } while (0);
-    hipFree(X); hipFree(T); hipFree(XV); hipFree(TV);
+    hipFree(X); hipFree(T); hipFree(XV); hipFree(TV); hipFree(vpart);
     if (rc != TIRA_OK) return rc;
-    TH_TRY(mlp_bin_table(m, st));
     TH_TRY(mlp_pack_inference(m, st));
+    TH_TRY(mlp_bin_table(m, st));
     m->trained = true;
     return TIRA_OK;
 }
@@ -902,12 +1174,15 @@ int tira_plate_mlp_affine(void* handle, float* get, const float* set) {
     PlateMlp* m;
     TH_TRY(mlp_check(handle, &m));
     if (set) {
-        for (int o = 0; o < kMlpOut; o++) {
+        for (int o = 0; o < kMlpOut; o++)
             TH_ARG(std::isfinite(set[o]) && std::isfinite(set[kMlpOut + o]) && set[kMlpOut + o] != 0.0f,
                    "affine: finite means and non-zero scales expected");
+        for (int o = 0; o < kMlpOut; o++) {
             m->aff.mu[o] = set[o];
             m->aff.sd[o] = set[kMlpOut + o];
         }
+        TH_TRY(mlp_build_vote_table(m, stream()));
+        m->version++;
     }
     if (get)
         for (int o = 0; o < kMlpOut; o++) {
@@ -923,6 +1198,43 @@ int tira_plate_mlp_bin_errors(void* handle, float* errs) {
     TH_ARG(errs, "NULL pointer");
     TH_HIP(hipStreamSynchronize(stream()));
     TH_HIP(hipMemcpy(errs, m->bin_err, kMlpBins * 4, hipMemcpyDeviceToHost));
+    return TIRA_OK;
+}
+
+int tira_plate_mlp_table_errors(void* handle, float* errs) {
+    PlateMlp* m;
+    TH_TRY(mlp_check(handle, &m));
+    TH_ARG(errs, "NULL pointer");
+    TH_HIP(hipStreamSynchronize(stream()));
+    TH_HIP(hipMemcpy(errs, m->bin_err + kMlpBins, kMlpBins * 4, hipMemcpyDeviceToHost));
+    return TIRA_OK;
+}
+
+int tira_plate_mlp_set_vote_table(void* handle, const float* tab, size_t nfloats) {
+    PlateMlp* m;
+    TH_TRY(mlp_check(handle, &m));
+    TH_ARG(tab, "NULL pointer");
+    const size_t n = (size_t)m->vtab_rows * kMlpTabM * 8;
+    TH_ARG(nfloats == n, "vote table has %zu floats (got %zu)", n, nfloats);
+    TH_HIP(hipStreamSynchronize(stream()));
+    TH_HIP(hipMemcpy(m->vtab, tab, n * 4, hipMemcpyHostToDevice));
+    m->version++;
+    return TIRA_OK;
+}
+
+int tira_plate_mlp_vote_table(void* handle, float* tab, size_t cap_floats, int* rows, int* cells, int* shells) {
+    PlateMlp* m;
+    TH_TRY(mlp_check(handle, &m));
+    if (rows) *rows = m->vtab_rows;
+    if (cells) *cells = kMlpTabM;
+    if (shells)
+        for (size_t k = 0; k < m->shells.size(); k++) shells[k] = m->shells[k];
+    if (tab) {
+        const size_t n = (size_t)m->vtab_rows * kMlpTabM * 8;
+        TH_ARG(cap_floats >= n, "vote table needs %zu floats (got %zu)", n, cap_floats);
+        TH_HIP(hipStreamSynchronize(stream()));
+        TH_HIP(hipMemcpy(tab, m->vtab, n * 4, hipMemcpyDeviceToHost));
+    }
     return TIRA_OK;
 }
 

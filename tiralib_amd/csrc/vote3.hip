@@ -144,6 +144,33 @@ void build_closed_table(int P, int w, float s1, float s2, unsigned power, std::v
             }
 }
 
+// Offset table of the tabulated-network vote, kMlpTabStride floats per
+// offset: d (3), 2 d (3), the stick constants as in the closed table (2),
+// c1 / pi (c1 as cpu::build_neighbors3d computes it) and the offset's
+// distance row of the network table (mlp_shell_row).
+void build_mlptab_table(int P, int w, float s1, float s2, unsigned power, std::vector<float>& tab) {
+    const int hw = w / 2;
+    tab.assign((size_t)w * w * w * kMlpTabStride, 0.0f);
+    const double sn = 1.0 / (double)sticknorm3_host(s1, s2, power);
+    const float snf = 1.0f / sticknorm3_host(s1, s2, power);
+    std::vector<int> shells;
+    mlp_shells(s1, shells);
+    size_t k = 0;
+    for (int dw = -hw; dw <= hw; dw++)
+        for (int dv = -hw; dv <= hw; dv++)
+            for (int du = -hw; du <= hw; du++, k++) {
+                float d[3], c1, c2;
+                offset_geometry(du, dv, dw, s1, s2, d, c1, c2);
+                float* t = &tab[k * kMlpTabStride];
+                t[0] = d[0]; t[1] = d[1]; t[2] = d[2];
+                t[3] = 2 * d[0]; t[4] = 2 * d[1]; t[5] = 2 * d[2];
+                if (P == 1) { t[6] = snf * c1; t[7] = (float)(sn * ((double)c2 - (double)c1)); }
+                else { t[6] = snf * c1; t[7] = snf * c2; }
+                t[8] = (float)(c1 / M_PI);
+                t[9] = (float)mlp_shell_row(shells, du * du + dv * dv + dw * dw);
+            }
+}
+
 // Literal-sum table: d, stick c1/c2 (sticknorm folded), plate c1/c2.
 void build_sum_table(int w, float s1, float s2, unsigned power, std::vector<float>& tab) {
     const int hw = w / 2;
@@ -228,23 +255,12 @@ __device__ __forceinline__ void stick_pair(Acc& a, const float4 S, const float* 
     a.yy = fmaf(f1, r1, a.yy); a.yz = fmaf(f1, r2, a.yz); a.zz = fmaf(f2, r2, a.zz);
 }
 
-// Plate vote of voter N = (n, scale) at offset entry e, closed form:
-//   scale * [alpha I + beta n n^T + gamma (n d^T + d n^T) + delta d d^T]
-// with gamma(n d^T + d n^T) + delta d d^T = h d^T + d h^T, h = gamma n + delta/2 d.
-template <int P>
-__device__ __forceinline__ void plate_pair(Acc& a, const float4 N, const float* __restrict__ e) {
+// Accumulates A I + B n n^T + G (n d^T + d n^T) + 2 D d d^T for the voter
+// normal n = N.xyz and the offset entry e (d = e[0..2], 2 d = e[3..5]):
+// G (n d^T + d n^T) + 2 D d d^T = h d^T + d h^T with h = G n + D d.
+__device__ __forceinline__ void plate_accumulate(Acc& a, const float4 N, const float* __restrict__ e, float A,
+                                                 float B, float G, float D) {
     const float d0 = e[0], d1 = e[1], d2 = e[2];
-    const float s = fmaf(N.x, d0, fmaf(N.y, d1, N.z * d2));
-    const float x = fmaf(-s, s, 1.0f);
-    const float* al = e + 8;
-    const float* be = al + (P + 1);
-    const float* ga = be + (P + 1);
-    const float* de = ga + (P + 1);
-    const float sp = N.w;
-    const float A = sp * horner<P>(al, x);
-    const float B = sp * horner<P>(be, x);
-    const float G = (sp * s) * horner<P>(ga, x);
-    const float D = sp * horner<P + 1>(de, x);
     a.ii += A;
     const float b0 = B * N.x, b1 = B * N.y, b2 = B * N.z;
     a.xx = fmaf(b0, N.x, a.xx); a.xy = fmaf(b0, N.y, a.xy); a.xz = fmaf(b0, N.z, a.xz);
@@ -254,6 +270,34 @@ __device__ __forceinline__ void plate_pair(Acc& a, const float4 N, const float* 
     a.xy = fmaf(h0, d1, fmaf(h1, d0, a.xy));
     a.xz = fmaf(h0, d2, fmaf(h2, d0, a.xz));
     a.yz = fmaf(h1, d2, fmaf(h2, d1, a.yz));
+}
+
+// Plate vote of voter N = (n, scale) at offset entry e, closed form:
+//   scale * [alpha I + beta n n^T + gamma (n d^T + d n^T) + delta d d^T]
+// (the table holds delta / 2, see plate_accumulate).
+template <int P>
+__device__ __forceinline__ void plate_pair(Acc& a, const float4 N, const float* __restrict__ e) {
+    const float s = fmaf(N.x, e[0], fmaf(N.y, e[1], N.z * e[2]));
+    const float x = fmaf(-s, s, 1.0f);
+    const float* al = e + 8;
+    const float* be = al + (P + 1);
+    const float* ga = be + (P + 1);
+    const float* de = ga + (P + 1);
+    const float sp = N.w;
+    plate_accumulate(a, N, e, sp * horner<P>(al, x), sp * horner<P>(be, x), (sp * s) * horner<P>(ga, x),
+                     sp * horner<P + 1>(de, x));
+}
+
+// Plate vote of voter N = (n, scale) from the tabulated network: the row of
+// the offset's distance (LDS), looked up at x = 1 - (n.d)^2, gives
+// (A, B, g, D/2) of the vote / c1; c1pi = c1 / pi of the offset.
+__device__ __forceinline__ void tab_plate_pair(Acc& a, const float4 N, const float* __restrict__ e,
+                                               const float4* row, float c1pi) {
+    const float s = fmaf(N.x, e[0], fmaf(N.y, e[1], N.z * e[2]));
+    const float x = fmaf(-s, s, 1.0f);
+    const float4 v = mlp_tab_lookup(row, x);
+    const float sp = N.w * c1pi;
+    plate_accumulate(a, N, e, sp * v.x, sp * v.y, (sp * s) * v.z, sp * v.w);
 }
 
 // Literal N-sample plate sum (plate3_numerical_device, tensorvote.h:2457-2485):
@@ -307,16 +351,17 @@ __device__ __forceinline__ void stick_pair_rt(Acc& a, const float4 S, const floa
     a.yy = fmaf(f1, r1, a.yy); a.yz = fmaf(f1, r2, a.yz); a.zz = fmaf(f2, r2, a.zz);
 }
 
-__device__ __forceinline__ void write_out(float* __restrict__ out, int layout, size_t v, const Acc& a) {
+__device__ __forceinline__ void write_out(float* __restrict__ out, int layout, size_t v, const Acc& a,
+                                          bool add = false) {
     const float xx = a.xx + a.ii, yy = a.yy + a.ii, zz = a.zz + a.ii;
     if (layout == TIRA_LAYOUT_MAT9) {
         float* o = out + v * 9;
-        o[0] = xx; o[1] = a.xy; o[2] = a.xz;
-        o[3] = a.xy; o[4] = yy; o[5] = a.yz;
-        o[6] = a.xz; o[7] = a.yz; o[8] = zz;
+        const float t[9] = {xx, a.xy, a.xz, a.xy, yy, a.yz, a.xz, a.yz, zz};
+        for (int k = 0; k < 9; k++) o[k] = add ? o[k] + t[k] : t[k];
     } else {
         float* o = out + v * 6;
-        o[0] = xx; o[1] = a.xy; o[2] = a.xz; o[3] = yy; o[4] = a.yz; o[5] = zz;
+        const float t[6] = {xx, a.xy, a.xz, yy, a.yz, zz};
+        for (int k = 0; k < 6; k++) o[k] = add ? o[k] + t[k] : t[k];
     }
 }
 
@@ -332,12 +377,17 @@ __device__ __forceinline__ void tile_coords(const VoteGeom& g, int bz_tile, unsi
 
 // Closed-form stick + plate gather. Payloads are padded volumes of
 // (P0, P1, P2) voxels; receiver (z, y, x) reads voter (z+dw+r, y+dv+r, x+du+r).
+// gate (optional): the launch does nothing while *gate == 0; accumulate:
+// add the votes to `out` instead of writing them.
 template <int P, bool STICK, bool PLATE, int BZ>
 __global__ __launch_bounds__(kThreads) void k_vote3_closed(const float4* __restrict__ pay_s,
                                                            const float4* __restrict__ pay_p,
                                                            const float* __restrict__ tab, VoteGeom g,
-                                                           float* __restrict__ out, int out_layout) {
+                                                           float* __restrict__ out, int out_layout,
+                                                           const unsigned long long* __restrict__ gate,
+                                                           int accumulate) {
     constexpr int STRIDE = (4 * P + 13 + 3) & ~3;
+    if (gate && *gate == 0ull) return;
     unsigned bx, by, bz;
     tile_coords(g, BZ, bx, by, bz);
     const int x = (int)bx * kTileX + (int)(threadIdx.x % kTileX);
@@ -374,7 +424,7 @@ __global__ __launch_bounds__(kThreads) void k_vote3_closed(const float4* __restr
 #pragma unroll
         for (int j = 0; j < BZ; j++) {
             const int z = z0 + j;
-            if (z < g.s0) write_out(out, out_layout, ((size_t)z * g.s1 + y) * g.s2 + x, acc[j]);
+            if (z < g.s0) write_out(out, out_layout, ((size_t)z * g.s1 + y) * g.s2 + x, acc[j], accumulate != 0);
         }
     }
 }
@@ -516,10 +566,12 @@ __global__ __launch_bounds__(256) void k_vote3_prep_tk(const float* __restrict__
 // fallback flag) with u the reference's plate-basis vector (used when an
 // offset runs along n); `flag` marks voters whose orientation bin failed the
 // surrogate's error check -- their plate votes come from the integrator (the
-// closed form of the N-sample sum).
+// closed form of the N-sample sum). Without pay_u (the tabulated network)
+// a flagged voter gets plate scale 0 in pay_n and its (n, scale) in pay_f,
+// which is zero for every other voter.
 __global__ __launch_bounds__(256) void k_vote3_prep_mlp(const float* __restrict__ in, int layout, VoteGeom g,
                                                         float4* __restrict__ pay_s, float4* __restrict__ pay_n,
-                                                        float4* __restrict__ pay_u,
+                                                        float4* __restrict__ pay_u, float4* __restrict__ pay_f,
                                                         const float* __restrict__ bin_err, float tol,
                                                         unsigned long long* __restrict__ n_fallback) {
     const size_t total = (size_t)g.P0 * g.P1 * g.P2;
@@ -530,7 +582,9 @@ __global__ __launch_bounds__(256) void k_vote3_prep_mlp(const float* __restrict_
         const int x = px - g.r, y = py - g.r, z = pz - g.r;
         if (x < 0 || y < 0 || z < 0 || x >= g.s2 || y >= g.s1 || z >= g.s0) {
             if (pay_s) pay_s[p] = zero;
-            pay_n[p] = zero; pay_u[p] = zero;
+            pay_n[p] = zero;
+            if (pay_u) pay_u[p] = zero;
+            if (pay_f) pay_f[p] = zero;
             continue;
         }
         const size_t v = ((size_t)z * g.s1 + y) * g.s2 + x;
@@ -546,8 +600,15 @@ __global__ __launch_bounds__(256) void k_vote3_prep_mlp(const float* __restrict_
         plate_basis(n, u, w);
         const bool fb = sp != 0.0f && bin_err[mlp_bin(n[0], n[1], n[2])] > tol;
         nf += fb ? 1u : 0u;
-        pay_n[p] = make_float4(n[0], n[1], n[2], sp);
-        pay_u[p] = make_float4(u[0], u[1], u[2], fb ? 1.0f : 0.0f);
+        if (pay_u) {
+            pay_n[p] = make_float4(n[0], n[1], n[2], sp);
+            pay_u[p] = make_float4(u[0], u[1], u[2], fb ? 1.0f : 0.0f);
+        } else {
+            // tabulated network: flagged voters leave this payload and vote
+            // through the fallback payload (the integrator's pass)
+            pay_n[p] = make_float4(n[0], n[1], n[2], fb ? 0.0f : sp);
+            pay_f[p] = fb ? make_float4(n[0], n[1], n[2], sp) : zero;
+        }
     }
     if (n_fallback && nf) atomicAdd(n_fallback, (unsigned long long)nf);
 }
@@ -556,24 +617,53 @@ typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 typedef float f32x2_t __attribute__((ext_vector_type(2)));
-typedef short s16x2_t __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ unsigned pack_bf16(float a, float b) {  // v_cvt_pk_bf16_f32, element 0 low
     const bf16x2_t h = __builtin_convertvector((f32x2_t){a, b}, bf16x2_t);
     return __builtin_bit_cast(unsigned, h);
 }
-__device__ __forceinline__ unsigned relu_bf16x2(unsigned x) {  // max(x, 0) of two BF16 as 16-bit integers
-    const s16x2_t s = __builtin_elementwise_max(__builtin_bit_cast(s16x2_t, x), (s16x2_t){0, 0});
-    return __builtin_bit_cast(unsigned, s);
-}
 __device__ __forceinline__ bf16x8_t op4(unsigned a, unsigned b, unsigned c, unsigned d) {
     return __builtin_bit_cast(bf16x8_t, make_uint4(a, b, c, d));
 }
-__device__ __forceinline__ bf16x8_t act8(const f32x16_t& c, int o) {
-    return op4(relu_bf16x2(pack_bf16(c[o + 0], c[o + 1])), relu_bf16x2(pack_bf16(c[o + 2], c[o + 3])),
-               relu_bf16x2(pack_bf16(c[o + 4], c[o + 5])), relu_bf16x2(pack_bf16(c[o + 6], c[o + 7])));
+// ReLU of 8 accumulator registers as MFMA operands: the BF16 high parts and
+// the BF16 residuals (the activation is their sum, ~16 mantissa bits).
+__device__ __forceinline__ void act8(const f32x16_t& c, int o, bf16x8_t& hi, bf16x8_t& lo) {
+    unsigned h[4], l[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const float r0 = fmaxf(c[o + 2 * i], 0.0f), r1 = fmaxf(c[o + 2 * i + 1], 0.0f);
+        h[i] = pack_bf16(r0, r1);
+        l[i] = pack_bf16(r0 - __uint_as_float(h[i] << 16), r1 - __uint_as_float(h[i] & 0xffff0000u));
+    }
+    hi = op4(h[0], h[1], h[2], h[3]);
+    lo = op4(l[0], l[1], l[2], l[3]);
 }
 #define TH_MFMA(a, b, c) __builtin_amdgcn_mfma_f32_32x32x16_bf16((a), (b), (c), 0, 0, 0)
+
+// Hidden layer of 64 units from the previous layer's 4 K-steps of high parts
+// and residuals: two 32-row output tiles (weights wl[0..3], wl[4..7]).
+__device__ __forceinline__ void mlp_layer(const bf16x8_t* __restrict__ wl, const bf16x8_t h[4], const bf16x8_t l[4],
+                                          f32x16_t& t0, f32x16_t& t1) {
+    const f32x16_t z = {};
+    t0 = TH_MFMA(wl[0], h[0], z);
+    t1 = TH_MFMA(wl[4], h[0], z);
+    t0 = TH_MFMA(wl[0], l[0], t0);
+    t1 = TH_MFMA(wl[4], l[0], t1);
+#pragma unroll
+    for (int s = 1; s < 4; s++) {
+        t0 = TH_MFMA(wl[s], h[s], t0);
+        t1 = TH_MFMA(wl[4 + s], h[s], t1);
+        t0 = TH_MFMA(wl[s], l[s], t0);
+        t1 = TH_MFMA(wl[4 + s], l[s], t1);
+    }
+}
+
+__device__ __forceinline__ void acts(const f32x16_t& t0, const f32x16_t& t1, bf16x8_t h[4], bf16x8_t l[4]) {
+    act8(t0, 0, h[0], l[0]);
+    act8(t0, 8, h[1], l[1]);
+    act8(t1, 0, h[2], l[2]);
+    act8(t1, 8, h[3], l[3]);
+}
 
 // The 64 pairs of a wave: lanes 0-31 carry pairs 0-31 as batch A, lanes
 // 32-63 pairs 32-63 as batch B. The two batches are staggered by half a
@@ -586,34 +676,31 @@ __device__ __forceinline__ void mlp_round(const bf16x8_t* __restrict__ w, bf16x8
     const f32x16_t z = {};
     f32x16_t a0 = TH_MFMA(w[0], xa, z), a1 = TH_MFMA(w[1], xa, z);
     f32x16_t b0 = TH_MFMA(w[0], xb, z), b1 = TH_MFMA(w[1], xb, z);
-    bf16x8_t ha[4] = {act8(a0, 0), act8(a0, 8), act8(a1, 0), act8(a1, 8)};
+    bf16x8_t ha[4], la[4], hb[4], lb[4];
+    acts(a0, a1, ha, la);
 #pragma unroll
     for (int layer = 0; layer < 2; layer++) {
         const bf16x8_t* wl = w + 2 + 8 * layer;
-        a0 = TH_MFMA(wl[0], ha[0], z);
-        a1 = TH_MFMA(wl[4], ha[0], z);
-#pragma unroll
-        for (int s = 1; s < 4; s++) {
-            a0 = TH_MFMA(wl[s], ha[s], a0);
-            a1 = TH_MFMA(wl[4 + s], ha[s], a1);
-        }
-        const bf16x8_t hb[4] = {act8(b0, 0), act8(b0, 8), act8(b1, 0), act8(b1, 8)};
-        b0 = TH_MFMA(wl[0], hb[0], z);
-        b1 = TH_MFMA(wl[4], hb[0], z);
-#pragma unroll
-        for (int s = 1; s < 4; s++) {
-            b0 = TH_MFMA(wl[s], hb[s], b0);
-            b1 = TH_MFMA(wl[4 + s], hb[s], b1);
-        }
-        ha[0] = act8(a0, 0); ha[1] = act8(a0, 8); ha[2] = act8(a1, 0); ha[3] = act8(a1, 8);
+        mlp_layer(wl, ha, la, a0, a1);
+        acts(b0, b1, hb, lb);
+        mlp_layer(wl, hb, lb, b0, b1);
+        acts(a0, a1, ha, la);
     }
     f32x16_t ya = TH_MFMA(w[18], ha[0], z);
+    ya = TH_MFMA(w[18], la[0], ya);
 #pragma unroll
-    for (int s = 1; s < 4; s++) ya = TH_MFMA(w[18 + s], ha[s], ya);
-    const bf16x8_t hb[4] = {act8(b0, 0), act8(b0, 8), act8(b1, 0), act8(b1, 8)};
+    for (int s = 1; s < 4; s++) {
+        ya = TH_MFMA(w[18 + s], ha[s], ya);
+        ya = TH_MFMA(w[18 + s], la[s], ya);
+    }
+    acts(b0, b1, hb, lb);
     f32x16_t yb = TH_MFMA(w[18], hb[0], z);
+    yb = TH_MFMA(w[18], lb[0], yb);
 #pragma unroll
-    for (int s = 1; s < 4; s++) yb = TH_MFMA(w[18 + s], hb[s], yb);
+    for (int s = 1; s < 4; s++) {
+        yb = TH_MFMA(w[18 + s], hb[s], yb);
+        yb = TH_MFMA(w[18 + s], lb[s], yb);
+    }
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const float t = __shfl_xor(yb[k], 32);
@@ -743,6 +830,72 @@ __global__ __launch_bounds__(kThreads) void k_vote3_mlp(const float4* __restrict
         else mlp_accumulate(acc, o, N.w * m0.w, Ua, Va, N);
     }
     if (x < g.s2 && y < g.s1 && z < g.s0) write_out(out, out_layout, ((size_t)z * g.s1 + y) * g.s2 + x, acc);
+}
+
+// Stick + tabulated-network plate gather. Same tiling, padding, z register
+// blocking and XCD order as k_vote3_closed. Per offset column (dv, du) the
+// block stages in LDS the network-table rows of the column's distances
+// (r + 1 rows: |dw| = 0..r, kMlpTabM cells of two float4 each), so every
+// pair reads its row from LDS. tab: kMlpTabStride floats per offset
+// (build_mlptab_table). Voters flagged by the error check carry plate scale
+// 0 here; their plates come from the integrator's pass.
+template <int P, bool STICK, int BZ>
+__global__ __launch_bounds__(kThreads) void k_vote3_mlptab(const float4* __restrict__ pay_s,
+                                                           const float4* __restrict__ pay_p,
+                                                           const float* __restrict__ tab,
+                                                           const float4* __restrict__ vtab, int nrows,
+                                                           VoteGeom g, float* __restrict__ out, int out_layout) {
+    constexpr int STRIDE = kMlpTabStride;
+    constexpr int C1PI = 8, ROW = 9;
+    constexpr int CELLS2 = 2 * kMlpTabM;  // float4 per row
+    extern __shared__ float4 s_rows[];
+    unsigned bx, by, bz;
+    tile_coords(g, BZ, bx, by, bz);
+    const int x = (int)bx * kTileX + (int)(threadIdx.x % kTileX);
+    const int y = (int)by * kTileY + (int)(threadIdx.x / kTileX);
+    const int z0 = (int)bz * BZ;
+    const int r = g.r, w = 2 * g.r + 1;
+    const size_t plane = (size_t)g.P1 * g.P2;
+    const int nst = (r + 1) * CELLS2;
+
+    Acc acc[BZ];
+#pragma unroll
+    for (int j = 0; j < BZ; j++) acc[j] = Acc{0, 0, 0, 0, 0, 0, 0};
+
+    for (int dv = -r; dv <= r; dv++) {
+        for (int du = -r; du <= r; du++) {
+            const int ebase = (dv + r) * w + (du + r);
+            __syncthreads();  // the previous column's rows are no longer read
+            for (int i = threadIdx.x; i < nst; i += kThreads) {
+                const int slot = i / CELLS2;  // |dw|: offsets +-dw share the distance
+                const int row = min(max((int)tab[(size_t)((slot + r) * w * w + ebase) * STRIDE + ROW], 0), nrows - 1);
+                s_rows[i] = vtab[(size_t)row * CELLS2 + (i - slot * CELLS2)];
+            }
+            __syncthreads();
+            const size_t col = ((size_t)z0 * g.P1 + (size_t)(y + r + dv)) * g.P2 + (size_t)(x + r + du);
+            for (int zz = 0; zz < BZ + 2 * r; zz++) {
+                float4 S;
+                if (STICK) S = pay_s[col + (size_t)zz * plane];
+                const float4 N = pay_p[col + (size_t)zz * plane];
+#pragma unroll
+                for (int j = 0; j < BZ; j++) {
+                    const int dw = zz - r - j;
+                    if (dw >= -r && dw <= r) {
+                        const float* e = tab + (size_t)((dw + r) * w * w + ebase) * STRIDE;
+                        if (STICK) stick_pair<P>(acc[j], S, e);
+                        tab_plate_pair(acc[j], N, e, s_rows + (dw < 0 ? -dw : dw) * CELLS2, e[C1PI]);
+                    }
+                }
+            }
+        }
+    }
+    if (x < g.s2 && y < g.s1) {
+#pragma unroll
+        for (int j = 0; j < BZ; j++) {
+            const int z = z0 + j;
+            if (z < g.s0) write_out(out, out_layout, ((size_t)z * g.s1 + y) * g.s2 + x, acc[j]);
+        }
+    }
 }
 
 // ---------------------------------------------------------------- analytical tensor voting (ATV)
@@ -965,7 +1118,7 @@ static int table_for(const TableKey& key, int P, hipStream_t st, char** dtab, fl
     size_t tab_bytes, cs_bytes = 0;
     if (key.sum == 2) {
         tab_bytes = (size_t)key.w * key.w * key.w * kAtvStride * 4;
-    } else if (key.sum) {
+    } else if (key.sum == 1) {
         tab_bytes = (size_t)key.w * key.w * key.w * kSumStride * 4;
         cs_bytes = (size_t)key.nsamp * 8;
     } else {
@@ -979,7 +1132,7 @@ static int table_for(const TableKey& key, int P, hipStream_t st, char** dtab, fl
     if (g_tab.key == key && g_tab.device == dev && g_tab.ptr == buf) return TIRA_OK;
     if (key.sum == 2) {
         build_atv3_table(key.w, key.s1, tab);
-    } else if (key.sum) {
+    } else if (key.sum == 1) {
         build_sum_table(key.w, key.s1, key.s2, key.power, tab);
         cs.resize(key.nsamp);
         const float db = (float)M_PI / (float)key.nsamp;  // dbeta of tensorvote.h:2473
@@ -1006,9 +1159,9 @@ constexpr int kBZ = 4;
 template <int P>
 static void launch_closed(bool stick, bool plate, unsigned grid, hipStream_t st, const float4* ps, const float4* pp,
                           const float* tab, const VoteGeom& g, float* out, int layout) {
-    if (stick && plate) k_vote3_closed<P, true, true, kBZ><<<grid, kThreads, 0, st>>>(ps, pp, tab, g, out, layout);
-    else if (stick) k_vote3_closed<P, true, false, kBZ><<<grid, kThreads, 0, st>>>(ps, pp, tab, g, out, layout);
-    else k_vote3_closed<P, false, true, kBZ><<<grid, kThreads, 0, st>>>(ps, pp, tab, g, out, layout);
+    if (stick && plate) k_vote3_closed<P, true, true, kBZ><<<grid, kThreads, 0, st>>>(ps, pp, tab, g, out, layout, nullptr, 0);
+    else if (stick) k_vote3_closed<P, true, false, kBZ><<<grid, kThreads, 0, st>>>(ps, pp, tab, g, out, layout, nullptr, 0);
+    else k_vote3_closed<P, false, true, kBZ><<<grid, kThreads, 0, st>>>(ps, pp, tab, g, out, layout, nullptr, 0);
 }
 
 // Runs the vote on device buffers. `eig_in`: tensors (layout) or NULL when
@@ -1173,6 +1326,36 @@ static int mlp_table_for(int w, float s1, float s2, hipStream_t st, float4** dta
     return TIRA_OK;
 }
 
+struct MlpCompactCache {
+    int P = 0, w = 0;
+    float s1 = 0, s2 = 0;
+    unsigned power = 0;
+    int device = -1;
+    void* ptr = nullptr;
+};
+static thread_local MlpCompactCache g_ctab;
+
+// Device copy of the tabulated-network vote's offset table, rebuilt and
+// uploaded only when its parameters change.
+static int mlptab_table_for(int P, int w, float s1, float s2, unsigned power, hipStream_t st, float** dtab) {
+    int dev = 0;
+    TH_HIP(hipGetDevice(&dev));
+    const size_t bytes = (size_t)w * w * w * kMlpTabStride * 4;
+    void* buf = nullptr;
+    TH_TRY(ws_get(WS_TABLE4, bytes, &buf));
+    *dtab = (float*)buf;
+    if (g_ctab.P == P && g_ctab.w == w && g_ctab.s1 == s1 && g_ctab.s2 == s2 && g_ctab.power == power &&
+        g_ctab.device == dev && g_ctab.ptr == buf)
+        return TIRA_OK;
+    std::vector<float> t;
+    build_mlptab_table(P, w, s1, s2, power, t);
+    TH_HIP(hipStreamSynchronize(st));
+    TH_HIP(hipMemcpy(buf, t.data(), bytes, hipMemcpyHostToDevice));
+    g_ctab.P = P; g_ctab.w = w; g_ctab.s1 = s1; g_ctab.s2 = s2; g_ctab.power = power; g_ctab.device = dev;
+    g_ctab.ptr = buf;
+    return TIRA_OK;
+}
+
 template <int P>
 static void launch_mlp(bool stick, unsigned grid, hipStream_t st, const float4* ps, const float4* pn,
                        const float4* pu, const float* ct, const float4* mt, const uint4* wr, const MlpAffine& aff,
@@ -1181,13 +1364,26 @@ static void launch_mlp(bool stick, unsigned grid, hipStream_t st, const float4* 
     else k_vote3_mlp<P, false><<<grid, kThreads, 0, st>>>(ps, pn, pu, ct, mt, wr, aff, g, out, layout);
 }
 
+template <int P>
+static void launch_mlptab(bool stick, unsigned grid, size_t lds, hipStream_t st, const float4* ps, const float4* pn,
+                          const float* t, const float4* vt, int nrows, const VoteGeom& g, float* out, int layout) {
+    if (stick) k_vote3_mlptab<P, true, kBZ><<<grid, kThreads, lds, st>>>(ps, pn, t, vt, nrows, g, out, layout);
+    else k_vote3_mlptab<P, false, kBZ><<<grid, kThreads, lds, st>>>(ps, pn, t, vt, nrows, g, out, layout);
+}
+
+// Largest window of the tabulated-network vote: its LDS holds (w / 2 + 1)
+// distance rows of 2 kMlpTabM float4 (64 KB at w = 63).
+constexpr unsigned kMlpTabMaxW = 63;
+
 // Stick + learned plate vote on device buffers with the network `m` (trained
 // for its sigma1, sigma2 and power). Voters whose orientation bin's error
 // exceeds `tol` use the integrator. n_fallback (host, optional) receives the
-// number of such voters (synchronises).
+// number of such voters (synchronises). method: TIRA_MLP_TABLE (the network
+// tabulated over distance rows and x = 1 - (n.d)^2; windows up to
+// kMlpTabMaxW, larger ones evaluate per pair) or TIRA_MLP_PER_PAIR.
 int vote3_mlp_run(PlateMlp* m, const float* tensors, int in_layout, float* out, int out_layout, unsigned s0,
                   unsigned s1, unsigned s2, unsigned w, bool stick, float tol, unsigned long long* n_fallback,
-                  hipStream_t st) {
+                  hipStream_t st, int method) {
     const size_t nvox = (size_t)s0 * s1 * s2;
     if (nvox == 0) return TIRA_OK;
     const int P = (int)m->power;
@@ -1195,33 +1391,62 @@ int vote3_mlp_run(PlateMlp* m, const float* tensors, int in_layout, float* out, 
         set_error("the learned plate vote supports power 1..4 (network trained for %d)", P);
         return TIRA_ERR_UNSUPPORTED;
     }
-    const VoteGeom g = make_geom(s0, s1, s2, w, 1);
+    const bool table = method == TIRA_MLP_TABLE && w <= kMlpTabMaxW && m->vtab != nullptr;
+    const VoteGeom g = make_geom(s0, s1, s2, w, table ? kBZ : 1);
     const size_t ptotal = (size_t)g.P0 * g.P1 * g.P2;
-    float4 *ps = nullptr, *pn = nullptr, *pu = nullptr;
+    // payloads: stick, plate; per pair: plate basis vector; table: the
+    // flagged voters' plates
+    float4 *ps = nullptr, *pn = nullptr, *pu = nullptr, *pf = nullptr;
     if (stick) TH_TRY(ws_get(WS_PAY0, ptotal * 16, (void**)&ps));
     TH_TRY(ws_get(WS_PAY1, ptotal * 16, (void**)&pn));
-    TH_TRY(ws_get(WS_PAY2, ptotal * 16, (void**)&pu));
+    TH_TRY(ws_get(WS_PAY2, ptotal * 16, table ? (void**)&pf : (void**)&pu));
     unsigned long long* cnt = nullptr;
-    if (n_fallback) {
+    if (n_fallback || table) {
         TH_TRY(ws_get(WS_COUNTER, 8, (void**)&cnt));
         k_fill_zero<<<1, 64, 0, st>>>((float*)cnt, 2);
     }
     const unsigned pgrid = (unsigned)std::min<size_t>(ceil_div(ptotal, 256), 65536);
     {
         ProfScope prof("vote3_prep_mlp", st);
-        k_vote3_prep_mlp<<<pgrid, 256, 0, st>>>(tensors, in_layout, g, ps, pn, pu, m->bin_err, tol, cnt);
+        k_vote3_prep_mlp<<<pgrid, 256, 0, st>>>(tensors, in_layout, g, ps, pn, pu, pf,
+                                                table ? m->bin_err + kMlpBins : m->bin_err, tol, cnt);
     }
     TH_HIP(hipGetLastError());
     const TableKey key{0, (int)w, m->sigma1, m->sigma2, m->power, 0u};
     char* dtab = nullptr;
     float2* dcs = nullptr;
     TH_TRY(table_for(key, P, st, &dtab, &dcs));
-    float4* mt = nullptr;
-    TH_TRY(mlp_table_for((int)w, m->sigma1, m->sigma2, st, &mt));
     const unsigned grid = g.nbx * g.nby * g.nbz;
     const float* ct = (const float*)dtab;
-    const uint4* wr = (const uint4*)m->wreg;
-    {
+    if (table) {
+        float* mt = nullptr;
+        TH_TRY(mlptab_table_for(P, (int)w, m->sigma1, m->sigma2, m->power, st, &mt));
+        const size_t lds = (size_t)(w / 2 + 1) * 2 * kMlpTabM * sizeof(float4);
+        const float4* vt = m->vtab;
+        const int nr = m->vtab_rows;
+        {
+            ProfScope prof("vote3_mlptab", st);
+            switch (P) {
+                case 1: launch_mlptab<1>(stick, grid, lds, st, ps, pn, mt, vt, nr, g, out, out_layout); break;
+                case 2: launch_mlptab<2>(stick, grid, lds, st, ps, pn, mt, vt, nr, g, out, out_layout); break;
+                case 3: launch_mlptab<3>(stick, grid, lds, st, ps, pn, mt, vt, nr, g, out, out_layout); break;
+                default: launch_mlptab<4>(stick, grid, lds, st, ps, pn, mt, vt, nr, g, out, out_layout); break;
+            }
+        }
+        TH_HIP(hipGetLastError());
+        // integrator plates of the flagged voters, added to the output; the
+        // launch returns at once when the error check flagged no voter
+        ProfScope prof("vote3_mlp_fallback", st);
+        switch (P) {
+            case 1: k_vote3_closed<1, false, true, kBZ><<<grid, kThreads, 0, st>>>(nullptr, pf, ct, g, out, out_layout, cnt, 1); break;
+            case 2: k_vote3_closed<2, false, true, kBZ><<<grid, kThreads, 0, st>>>(nullptr, pf, ct, g, out, out_layout, cnt, 1); break;
+            case 3: k_vote3_closed<3, false, true, kBZ><<<grid, kThreads, 0, st>>>(nullptr, pf, ct, g, out, out_layout, cnt, 1); break;
+            default: k_vote3_closed<4, false, true, kBZ><<<grid, kThreads, 0, st>>>(nullptr, pf, ct, g, out, out_layout, cnt, 1); break;
+        }
+    } else {
+        float4* mt = nullptr;
+        TH_TRY(mlp_table_for((int)w, m->sigma1, m->sigma2, st, &mt));
+        const uint4* wr = (const uint4*)m->wreg;
         ProfScope prof("vote3_mlp", st);
         switch (P) {
             case 1: launch_mlp<1>(stick, grid, st, ps, pn, pu, ct, mt, wr, m->aff, g, out, out_layout); break;
@@ -1306,7 +1531,15 @@ int tira_tensorvote3_tk(float* output_field, const float* lambdas, const float* 
 int tira_tensorvote3_mlp(void* mlp, const float* input_field, int in_layout, float* output_field, int out_layout,
                          unsigned s0, unsigned s1, unsigned s2, unsigned w, int stick, float tol,
                          unsigned long long* n_fallback) {
+    return tira_tensorvote3_mlp_method(mlp, input_field, in_layout, output_field, out_layout, s0, s1, s2, w, stick,
+                                       tol, n_fallback, TIRA_MLP_TABLE);
+}
+
+int tira_tensorvote3_mlp_method(void* mlp, const float* input_field, int in_layout, float* output_field,
+                                int out_layout, unsigned s0, unsigned s1, unsigned s2, unsigned w, int stick,
+                                float tol, unsigned long long* n_fallback, int method) {
     TH_ARG(mlp && input_field && output_field, "NULL pointer");
+    TH_ARG(method == TIRA_MLP_TABLE || method == TIRA_MLP_PER_PAIR, "bad MLP evaluation method %d", method);
     PlateMlp* m = static_cast<PlateMlp*>(mlp);
     int dev = 0;
     TH_HIP(hipGetDevice(&dev));
@@ -1318,7 +1551,7 @@ int tira_tensorvote3_mlp(void* mlp, const float* input_field, int in_layout, flo
     TH_ARG(is_device(input_field) && is_device(output_field), "the learned plate vote takes device pointers");
     TH_TRY(check_vote_args(s0, s1, s2, w, m->sigma1, m->sigma2));
     return vote3_mlp_run(m, input_field, in_layout, output_field, out_layout, s0, s1, s2, w, stick != 0, tol,
-                         n_fallback, stream());
+                         n_fallback, stream(), method);
 }
 
 int tira_tensorvote3_async(const float* input_field, int in_layout, float* output_field, int out_layout,

@@ -6,7 +6,8 @@
 
 namespace th {
 
-constexpr int kSumStride = 8;  // floats per offset entry of the literal-sum table
+constexpr int kSumStride = 8;     // floats per offset entry of the literal-sum table
+constexpr int kMlpTabStride = 12; // floats per offset entry of the tabulated-network vote
 
 // Receiver / payload geometry of one 3D vote launch.
 struct VoteGeom {
