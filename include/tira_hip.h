@@ -154,6 +154,20 @@ int tira_tensorvote2(float* t_out, const float* t_in, float sigma1, float sigma2
 int tira_tensorvote2_prebuilt(float* t_out, const float* t_in, const void* host_nb, int nb_count, float sigma1,
                               float sigma2, int power, unsigned w, unsigned shape0, unsigned shape1, int stick,
                               int plate, unsigned samples);
+/* No reference counterpart. tira_tensorvote2_prebuilt for callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same kernels and results, with the
+ * voter payload (16 B per pixel) and the neighbour table (48 B per entry)
+ * kept in a 32 MiB device array owned by the library (one per GPU, held until
+ * the process ends) instead of allocated workspace. Device fields only; the
+ * table may be host or device memory (a host table is copied into the
+ * array). Rejects fields whose payload and table exceed 32 MiB
+ * (TIRA_ERR_INVALID_ARG, nothing launched). Blocking. Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (from several host
+ * threads or streams) race. */
+int tira_tensorvote2_prebuilt_noalloc(float* t_out, const float* t_in, const void* nb, int nb_count, float sigma1,
+                                      float sigma2, int power, unsigned w, unsigned s0, unsigned s1, int stick,
+                                      int plate, unsigned samples);
 /* tira::cuda::tensorvote_tk<T>(t_out, lambdas, thetas, sigma1, sigma2, power, w,
  * s0, s1, stick, plate, samples), tensorvote.h:2125-2179. lambdas / thetas
  * s0 x s1 x 2 (ascending eigenvalues, eigenvector angles). */

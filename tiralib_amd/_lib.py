@@ -95,6 +95,7 @@ SIGNATURES = {
     "tira_build_neighbors2d": [_i, _f, _f, _u, _u, _i, _i, _vp, _i, C.POINTER(_i)],
     "tira_tensorvote2": [_vp, _vp, _f, _f, _i, _u, _u, _u, _i, _i, _u, _i],
     "tira_tensorvote2_prebuilt": [_vp, _vp, _vp, _i, _f, _f, _i, _u, _u, _u, _i, _i, _u],
+    "tira_tensorvote2_prebuilt_noalloc": [_vp, _vp, _vp, _i, _f, _f, _i, _u, _u, _u, _i, _i, _u],
     "tira_tensorvote2_tk": [_vp, _vp, _vp, _f, _f, _u, _u, _u, _u, _i, _i, _u],
     "tira_tensorvote3": [_vp, _vp, _u, _u, _u, _f, _f, _u, _u, _i, _i, _i, _u, _i],
     "tira_tensorvote3_tk": [_vp, _vp, _vp, _vp, _u, _u, _u, _f, _f, _u, _u, _i, _i, _u],

@@ -15,6 +15,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 namespace th {
@@ -347,26 +348,22 @@ static int vote2_atv_entry(float* t_out, const float* t_in, const float* lam, co
 }
 
 // ---------------------------------------------------------------- host driver
-// plate: the reference's kernel adds the plate term when plate && samples > 0
-// (tensorvote.h:2093); the table's V entries are zero otherwise.
-static int vote2_run(const float* tin, const float* lam, const float* th, float* out, const Nb2* host_nb, int nbc,
-                     float s1, float s2, int power, unsigned s0, unsigned s1n, bool stick, bool plate,
-                     unsigned samples, hipStream_t st) {
+// The payload pass and the table-driven gather on device buffers: `pay` holds
+// n float4, `dnb` the nbc-entry neighbour table. plate: the reference's kernel
+// adds the plate term when plate && samples > 0 (tensorvote.h:2093); the
+// table's V entries are zero otherwise.
+static int vote2_launch(const float* tin, const float* lam, const float* th, float* out, float4* pay, const Nb2* dnb,
+                        int nbc, float s1, float s2, int power, unsigned s0, unsigned s1n, bool stick, bool plate,
+                        unsigned samples, hipStream_t st) {
     const size_t n = (size_t)s0 * s1n;
-    if (!n) return TIRA_OK;
     const unsigned p = power < 1 ? 1 : (unsigned)power;
     const float eta_s = stick_norm_2d(s1, s2, p);
-    float4* pay = nullptr;
-    TH_TRY(ws_get(WS_PAY0, n * 16, (void**)&pay));
     {
         ProfScope prof("vote2_prep", st);
         if (tin) k_vote2_prep<<<ceil_div(n, 256), 256, 0, st>>>((const float4*)tin, n, eta_s, pay);
         else k_vote2_prep_tk<<<ceil_div(n, 256), 256, 0, st>>>((const float2*)lam, (const float2*)th, n, eta_s, pay);
     }
     TH_HIP(hipGetLastError());
-    Nb2* dnb = nullptr;
-    static thread_local TableCache nb_cache;
-    TH_TRY(table_to_device(WS_NB2, host_nb, (size_t)(nbc > 0 ? nbc : 0) * sizeof(Nb2), nb_cache, st, (void**)&dnb));
     dim3 grid(ceil_div(s1n, 64), ceil_div(s0, 4));
     {
         ProfScope prof("vote2", st);
@@ -374,6 +371,46 @@ static int vote2_run(const float* tin, const float* lam, const float* th, float*
                                       (plate && samples > 0) ? 1 : 0, out);
     }
     TH_HIP(hipGetLastError());
+    return TIRA_OK;
+}
+
+static int vote2_run(const float* tin, const float* lam, const float* th, float* out, const Nb2* host_nb, int nbc,
+                     float s1, float s2, int power, unsigned s0, unsigned s1n, bool stick, bool plate,
+                     unsigned samples, hipStream_t st) {
+    const size_t n = (size_t)s0 * s1n;
+    if (!n) return TIRA_OK;
+    float4* pay = nullptr;
+    TH_TRY(ws_get(WS_PAY0, n * 16, (void**)&pay));
+    Nb2* dnb = nullptr;
+    static thread_local TableCache nb_cache;
+    TH_TRY(table_to_device(WS_NB2, host_nb, (size_t)(nbc > 0 ? nbc : 0) * sizeof(Nb2), nb_cache, st, (void**)&dnb));
+    return vote2_launch(tin, lam, th, out, pay, dnb, nbc, s1, s2, power, s0, s1n, stick, plate, samples, st);
+}
+
+// ---------------------------------------------------------------- allocation-free 2D vote
+// tira_tensorvote2_prebuilt_noalloc keeps the voter payload and the
+// neighbour table in this file-scope device array instead of library
+// workspace. Each device holds its own copy for the life of the process; its
+// device address is resolved once per device (hipGetSymbolAddress) and kept
+// in g_scratch: host code passes that address, never the array's name.
+constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kNoAllocMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch[kNoAllocMaxDevices] = {};
+
+static int scratch_address(void** out) {
+    int dev = 0;
+    TH_HIP(hipGetDevice(&dev));
+    TH_ARG(dev >= 0 && dev < kNoAllocMaxDevices, "device %d beyond the allocation-free vote's table", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        TH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        TH_ARG(p != nullptr, "no device address for the allocation-free vote's scratch array");
+        g_scratch[dev] = p;
+    }
+    *out = g_scratch[dev];
     return TIRA_OK;
 }
 
@@ -440,6 +477,39 @@ int tira_tensorvote2_prebuilt(float* t_out, const float* t_in, const void* host_
     TH_ARG(nb_count >= 0 && (nb_count == 0 || host_nb), "bad neighbour table");
     return vote2_entry(t_out, t_in, nullptr, nullptr, (const Nb2*)host_nb, nb_count, sigma1, sigma2, power, w, s0,
                        s1, stick, plate, samples);
+}
+
+// No reference counterpart: tira_tensorvote2_prebuilt without any device
+// allocation (scratch array above), for callers that cannot allocate.
+int tira_tensorvote2_prebuilt_noalloc(float* t_out, const float* t_in, const void* nb, int nb_count,
+                                       float sigma1, float sigma2, int power, unsigned w, unsigned s0, unsigned s1,
+                                       int stick, int plate, unsigned samples) {
+    TH_ARG(t_out && t_in, "NULL pointer");
+    TH_ARG(nb_count >= 0 && (nb_count == 0 || nb), "bad neighbour table");
+    TH_ARG(sigma1 >= 0.0f && sigma2 >= 0.0f, "sigma must be >= 0");
+    (void)w;
+    const size_t n = (size_t)s0 * s1;
+    const size_t pay_bytes = (n * 16 + 255) & ~(size_t)255;
+    const size_t tab_bytes = (size_t)nb_count * sizeof(Nb2);
+    TH_ARG(pay_bytes + tab_bytes <= kNoAllocScratchBytes,
+           "a %u x %u field with %d table entries needs %zu bytes of scratch; the allocation-free vote holds %zu "
+           "(use tira_tensorvote2_prebuilt)", s0, s1, nb_count, pay_bytes + tab_bytes, kNoAllocScratchBytes);
+    if (!n) return TIRA_OK;
+    TH_ARG(is_device(t_in) && is_device(t_out), "the allocation-free vote takes device fields (it stages nothing)");
+    void* base = nullptr;
+    TH_TRY(scratch_address(&base));
+    hipStream_t st = stream();
+    float4* pay = (float4*)base;
+    const Nb2* dnb = (const Nb2*)nb;
+    if (nb_count > 0 && !is_device(nb)) {
+        Nb2* tab = (Nb2*)((unsigned char*)base + pay_bytes);
+        TH_HIP(hipMemcpyAsync(tab, nb, tab_bytes, hipMemcpyHostToDevice, st));
+        dnb = tab;
+    }
+    TH_TRY(vote2_launch(t_in, nullptr, nullptr, t_out, pay, dnb, nb_count, sigma1, sigma2, power, s0, s1,
+                        stick != 0, plate != 0, samples, st));
+    TH_HIP(hipStreamSynchronize(st));
+    return TIRA_OK;
 }
 
 // tira::cuda::tensorvote_tk (2D, from eigenpairs), tensorvote.h:2125-2179
