@@ -16,6 +16,17 @@ and the full pipeline (structure tensor, Gaussian sigma 2, vote sigma 4,
 eig3 saliency) on a 192 x 256 x 256 vascular-tree volume, spectral then
 gather. Prints one line per
 stage.
+
+    python -m tiralib_amd.profile_run --config4 [--steps 3]
+
+runs instead the 1 GiB copy three times, the FP32 FMA loop once (the VALU
+counter reference) and the legs of the benchmark's
+256^3 curved-sheet workload (12
+sheets, 2% noise, seed 1004; sigma 4, w 25): the learned plate as the
+tabulated network and per pair (untrained weights, error check off: the
+kernels' work does not depend on the weights), the literal 20- and
+64-sample integrated plate, the closed form and the spectral vote, `steps`
+calls each (the per-pair network and the 64-sample sum once).
 """
 from __future__ import annotations
 
@@ -36,10 +47,13 @@ def main(argv=None):
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--skip-mlp", action="store_true")
     ap.add_argument("--skip-pipeline", action="store_true")
+    ap.add_argument("--config4", action="store_true")
     a = ap.parse_args(argv)
     from tiralib_amd import _lib, api, synth
     from tiralib_amd._lib import DeviceArray, call
     _lib.load()
+    if a.config4:
+        return config4(a.steps)
     nb = 1 << 30
     src, dst = DeviceArray(nb // 4), DeviceArray(nb // 4)
     for _ in range(3):
@@ -114,6 +128,51 @@ def main(argv=None):
                 api.pipeline3(pvol, smooth_sigma=2.0, vote_sigma=4.0, w=25, plate_mode=mode, saliency=sal)
             call("tira_hip_synchronize")
         print(f"pipeline: {a.steps} runs on 192 x 256 x 256", flush=True)
+
+
+def config4(steps):
+    from tiralib_amd import _lib, api, synth
+    from tiralib_amd._lib import DeviceArray, call
+    nb = 1 << 30
+    src, dst = DeviceArray(nb // 4), DeviceArray(nb // 4)
+    for _ in range(3):
+        call("tira_hip_copy_async", dst.ptr, src.ptr, nb)
+    call("tira_hip_synchronize")
+    src.free()
+    dst.free()
+    print("copy16: 3 x 1 GiB", flush=True)
+    # FP32 FMA loop (8 FMA per iteration and thread): the VALU counter reference
+    fblocks, fiters = 256 * 64, 4096
+    fout = DeviceArray(fblocks)
+    call("tira_hip_fma_loop_async", fout.ptr, fblocks, fiters)
+    call("tira_hip_synchronize")
+    fout.free()
+    print(f"fma loop: {fblocks} blocks x 256 threads x {fiters} iterations x 8 FMA", flush=True)
+    shape, sigma, w = (256, 256, 256), 4.0, 25
+    vol = synth.curved_sheets_volume_device(shape, 12, 0.02, seed=synth.SEED_CFG4)
+    out = DeviceArray((int(np.prod(shape)), 6))
+    net = api.PlateMLP(sigma, 0.0, 1, 64, seed=synth.SEED_CFG4)
+
+    def trad(mode, samples):
+        call("tira_tensorvote3_async", vol.ptr, _lib.LAYOUT_PACKED6, out.ptr, _lib.LAYOUT_PACKED6, *shape, sigma,
+             0.0, w, 1, 1, 1, samples, mode)
+
+    legs = [("tabulated learned plate", lambda: net.vote(vol, out, shape, w, stick=True, tol=1e30,
+                                                         in_layout=_lib.LAYOUT_PACKED6), steps),
+            ("integrated plate, 20 samples", lambda: trad(_lib.PLATE_SUM, 20), steps),
+            ("closed form", lambda: trad(_lib.PLATE_CLOSED, 20), steps),
+            ("spectral", lambda: trad(_lib.VOTE_SPECTRAL, 20), steps),
+            ("integrated plate, 64 samples", lambda: trad(_lib.PLATE_SUM, 64), 1),
+            ("learned plate per pair", lambda: net.vote(vol, out, shape, w, stick=True, tol=1e30,
+                                                        in_layout=_lib.LAYOUT_PACKED6, method=_lib.MLP_PER_PAIR), 1)]
+    for name, fn, reps in legs:
+        for _ in range(reps):
+            fn()
+        call("tira_hip_synchronize")
+        print(f"config 4: {reps} x {name}", flush=True)
+    net.close()
+    out.free()
+    vol.free()
 
 
 if __name__ == "__main__":
