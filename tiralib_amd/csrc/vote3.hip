@@ -400,14 +400,18 @@ __global__ __launch_bounds__(kThreads) void k_vote3_closed(const float4* __restr
 #pragma unroll
     for (int j = 0; j < BZ; j++) acc[j] = Acc{0, 0, 0, 0, 0, 0, 0};
 
+    // Offsets in the order (dv, voter plane, du): consecutive du steps read
+    // the same voter row shifted by one voxel, so a wave's cache lines are
+    // reused at once instead of after a whole z column of the window.
     for (int dv = -r; dv <= r; dv++) {
-        for (int du = -r; du <= r; du++) {
-            const size_t col = ((size_t)z0 * g.P1 + (size_t)(y + r + dv)) * g.P2 + (size_t)(x + r + du);
-            const int ebase = (dv + r) * w + (du + r);
-            for (int zz = 0; zz < BZ + 2 * r; zz++) {
+        const size_t rowb = ((size_t)z0 * g.P1 + (size_t)(y + r + dv)) * g.P2 + (size_t)(x + r);
+        for (int zz = 0; zz < BZ + 2 * r; zz++) {
+            const size_t zrow = rowb + (size_t)zz * plane;
+            for (int du = -r; du <= r; du++) {
                 float4 S, N;
-                if (STICK) S = pay_s[col + (size_t)zz * plane];
-                if (PLATE) N = pay_p[col + (size_t)zz * plane];
+                if (STICK) S = pay_s[zrow + du];
+                if (PLATE) N = pay_p[zrow + du];
+                const int ebase = (dv + r) * w + (du + r);
 #pragma unroll
                 for (int j = 0; j < BZ; j++) {
                     const int dw = zz - r - j;
@@ -449,14 +453,16 @@ __global__ __launch_bounds__(kThreads) void k_vote3_sum(const float4* __restrict
 #pragma unroll
     for (int j = 0; j < BZ; j++) acc[j] = Acc{0, 0, 0, 0, 0, 0, 0};
 
+    // Offsets in the order (dv, voter plane, du), as in k_vote3_closed.
     for (int dv = -r; dv <= r; dv++) {
-        for (int du = -r; du <= r; du++) {
-            const size_t col = ((size_t)z0 * g.P1 + (size_t)(y + r + dv)) * g.P2 + (size_t)(x + r + du);
-            const int ebase = (dv + r) * w + (du + r);
-            for (int zz = 0; zz < BZ + 2 * r; zz++) {
+        const size_t rowb = ((size_t)z0 * g.P1 + (size_t)(y + r + dv)) * g.P2 + (size_t)(x + r);
+        for (int zz = 0; zz < BZ + 2 * r; zz++) {
+            const size_t zrow = rowb + (size_t)zz * plane;
+            for (int du = -r; du <= r; du++) {
+                const int ebase = (dv + r) * w + (du + r);
                 float4 S, U, V;
-                if (STICK) S = pay_s[col + (size_t)zz * plane];
-                if (PLATE) { U = pay_u[col + (size_t)zz * plane]; V = pay_v[col + (size_t)zz * plane]; }
+                if (STICK) S = pay_s[zrow + du];
+                if (PLATE) { U = pay_u[zrow + du]; V = pay_v[zrow + du]; }
 #pragma unroll
                 for (int j = 0; j < BZ; j++) {
                     const int dw = zz - r - j;
@@ -960,12 +966,13 @@ __global__ __launch_bounds__(kThreads) void k_vote3_atv(const float4* __restrict
 #pragma unroll
     for (int j = 0; j < BZ; j++) acc[j] = Acc{0, 0, 0, 0, 0, 0, 0};
 
+    // Offsets in the order (dv, voter plane, du), as in k_vote3_closed.
     for (int dv = -r; dv <= r; dv++) {
-        for (int du = -r; du <= r; du++) {
-            const size_t col = ((size_t)z0 * g.P1 + (size_t)(y + r + dv)) * g.P2 + (size_t)(x + r + du);
-            const int ebase = (dv + r) * w + (du + r);
-            for (int zz = 0; zz < BZ + 2 * r; zz++) {
-                const size_t pi = col + (size_t)zz * plane;
+        const size_t rowb = ((size_t)z0 * g.P1 + (size_t)(y + r + dv)) * g.P2 + (size_t)(x + r);
+        for (int zz = 0; zz < BZ + 2 * r; zz++) {
+            for (int du = -r; du <= r; du++) {
+                const int ebase = (dv + r) * w + (du + r);
+                const size_t pi = rowb + (size_t)zz * plane + du;
                 const float4 A = pay_a[pi], B = pay_b[pi];
                 const float s3 = pay_c[pi];
 #pragma unroll
@@ -1154,7 +1161,8 @@ static int table_for(const TableKey& key, int P, hipStream_t st, char** dtab, fl
     return TIRA_OK;
 }
 
-constexpr int kBZ = 4;
+constexpr int kBZ = 4;     // receivers per lane along z: closed, literal-sum and ATV gathers
+constexpr int kBZMlp = 8;  // the tabulated-network gather (its LDS rows are staged per offset column)
 
 template <int P>
 static void launch_closed(bool stick, bool plate, unsigned grid, hipStream_t st, const float4* ps, const float4* pp,
@@ -1367,8 +1375,8 @@ static void launch_mlp(bool stick, unsigned grid, hipStream_t st, const float4* 
 template <int P>
 static void launch_mlptab(bool stick, unsigned grid, size_t lds, hipStream_t st, const float4* ps, const float4* pn,
                           const float* t, const float4* vt, int nrows, const VoteGeom& g, float* out, int layout) {
-    if (stick) k_vote3_mlptab<P, true, kBZ><<<grid, kThreads, lds, st>>>(ps, pn, t, vt, nrows, g, out, layout);
-    else k_vote3_mlptab<P, false, kBZ><<<grid, kThreads, lds, st>>>(ps, pn, t, vt, nrows, g, out, layout);
+    if (stick) k_vote3_mlptab<P, true, kBZMlp><<<grid, kThreads, lds, st>>>(ps, pn, t, vt, nrows, g, out, layout);
+    else k_vote3_mlptab<P, false, kBZMlp><<<grid, kThreads, lds, st>>>(ps, pn, t, vt, nrows, g, out, layout);
 }
 
 // Largest window of the tabulated-network vote: its LDS holds (w / 2 + 1)
@@ -1392,7 +1400,7 @@ int vote3_mlp_run(PlateMlp* m, const float* tensors, int in_layout, float* out, 
         return TIRA_ERR_UNSUPPORTED;
     }
     const bool table = method == TIRA_MLP_TABLE && w <= kMlpTabMaxW && m->vtab != nullptr;
-    const VoteGeom g = make_geom(s0, s1, s2, w, table ? kBZ : 1);
+    const VoteGeom g = make_geom(s0, s1, s2, w, table ? kBZMlp : 1);
     const size_t ptotal = (size_t)g.P0 * g.P1 * g.P2;
     // payloads: stick, plate; per pair: plate basis vector; table: the
     // flagged voters' plates
@@ -1438,10 +1446,10 @@ int vote3_mlp_run(PlateMlp* m, const float* tensors, int in_layout, float* out, 
         // launch returns at once when the error check flagged no voter
         ProfScope prof("vote3_mlp_fallback", st);
         switch (P) {
-            case 1: k_vote3_closed<1, false, true, kBZ><<<grid, kThreads, 0, st>>>(nullptr, pf, ct, g, out, out_layout, cnt, 1); break;
-            case 2: k_vote3_closed<2, false, true, kBZ><<<grid, kThreads, 0, st>>>(nullptr, pf, ct, g, out, out_layout, cnt, 1); break;
-            case 3: k_vote3_closed<3, false, true, kBZ><<<grid, kThreads, 0, st>>>(nullptr, pf, ct, g, out, out_layout, cnt, 1); break;
-            default: k_vote3_closed<4, false, true, kBZ><<<grid, kThreads, 0, st>>>(nullptr, pf, ct, g, out, out_layout, cnt, 1); break;
+            case 1: k_vote3_closed<1, false, true, kBZMlp><<<grid, kThreads, 0, st>>>(nullptr, pf, ct, g, out, out_layout, cnt, 1); break;
+            case 2: k_vote3_closed<2, false, true, kBZMlp><<<grid, kThreads, 0, st>>>(nullptr, pf, ct, g, out, out_layout, cnt, 1); break;
+            case 3: k_vote3_closed<3, false, true, kBZMlp><<<grid, kThreads, 0, st>>>(nullptr, pf, ct, g, out, out_layout, cnt, 1); break;
+            default: k_vote3_closed<4, false, true, kBZMlp><<<grid, kThreads, 0, st>>>(nullptr, pf, ct, g, out, out_layout, cnt, 1); break;
         }
     } else {
         float4* mt = nullptr;
