@@ -52,6 +52,8 @@ enum Slot {
     WS_NSLOTS
 };
 int ws_get(int slot, size_t bytes, void** p);
+// Bytes held by `slot` for this thread on the current device (0 if none).
+size_t ws_capacity(int slot);
 // Changes whenever any workspace buffer of this thread is allocated or released.
 unsigned long long ws_generation();
 

@@ -111,6 +111,14 @@ int ws_get(int slot, size_t bytes, void** p) {
     return TIRA_OK;
 }
 
+size_t ws_capacity(int slot) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices || slot < 0 || slot >= WS_NSLOTS ||
+        !g_ws[dev])
+        return 0;
+    return g_ws[dev]->cap[slot];
+}
+
 int stage_in(const void* src, size_t bytes, int slot, const void** dev) {
     if (!src) { *dev = nullptr; return TIRA_OK; }
     if (is_device(src)) { *dev = src; return TIRA_OK; }

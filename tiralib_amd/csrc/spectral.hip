@@ -1443,11 +1443,15 @@ bool spectral_preferred(unsigned s0, unsigned s1, unsigned s2, unsigned w, unsig
     const size_t need = nz * (size_t)L0 * L1 * L2 * 8 + 6 * nf * h0 * h1 * h2 * 4 + 6 * nf * W * h1 * h2 * 8 +
                         (size_t)s0 * s1 * s2 * 32;
     size_t free_b = 0, total_b = 0;
-    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+    int dev = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || hipGetDevice(&dev) != hipSuccess) {
         (void)hipGetLastError();
         return false;
     }
-    return need <= total_b / 10 * 8;
+    // memory this thread's spectral vote already holds is reused, not allocated again
+    size_t held = ws_capacity(WS_SPEC0) + ws_capacity(WS_PAY0) + ws_capacity(WS_PAY1);
+    if (g_spec.buf && g_spec.device == dev) held += g_spec.bytes;
+    return need <= total_b / 10 * 8 && need <= (free_b + held) / 10 * 9;
 }
 
 int vote3_spectral_run(const float* tensors, int in_layout, const float* lam, const float* ev2, const float* ev0,
@@ -1514,14 +1518,25 @@ int vote3_spectral_run(const float* tensors, int in_layout, const float* lam, co
     const bool fused = tensors != nullptr;
     float4 *ps = nullptr, *pp = nullptr;
     float2* Z = nullptr;
-    if (stick && !fused) TH_TRY(ws_get(WS_PAY0, nvox * 16, (void**)&ps));
-    if (plate && !fused) TH_TRY(ws_get(WS_PAY1, nvox * 16, (void**)&pp));
-    TH_TRY(ws_get(WS_SPEC0, (size_t)g.nz * g.cs * 8, (void**)&Z));
+    // Out of device memory for the spectral buffers: clear the error and let
+    // the caller take the direct gather, which needs far less.
+    auto alloc = [&](int rc) {
+        if (rc == TIRA_ERR_OUT_OF_MEMORY) (void)hipGetLastError();
+        return rc;
+    };
+    int rc = TIRA_OK;
+    if (stick && !fused && (rc = alloc(ws_get(WS_PAY0, nvox * 16, (void**)&ps))) != TIRA_OK)
+        return rc == TIRA_ERR_OUT_OF_MEMORY ? kSpectralNotApplicable : rc;
+    if (plate && !fused && (rc = alloc(ws_get(WS_PAY1, nvox * 16, (void**)&pp))) != TIRA_OK)
+        return rc == TIRA_ERR_OUT_OF_MEMORY ? kSpectralNotApplicable : rc;
+    if ((rc = alloc(ws_get(WS_SPEC0, (size_t)g.nz * g.cs * 8, (void**)&Z))) != TIRA_OK)
+        return rc == TIRA_ERR_OUT_OF_MEMORY ? kSpectralNotApplicable : rc;
     // the spectra are those of the stick part whatever the stick / plate selection
     const SpecKey key{g.L0, g.L1, g.L2, (int)w, sigma1, sigma2, 1, 1, atv ? 1 : 0};
     const float2 *tw0, *tw1, *tw2;
     const float* kh;
-    TH_TRY(spectra_for(key, g, st, &tw0, &tw1, &tw2, &kh));
+    if ((rc = alloc(spectra_for(key, g, st, &tw0, &tw1, &tw2, &kh))) != TIRA_OK)
+        return rc == TIRA_ERR_OUT_OF_MEMORY ? kSpectralNotApplicable : rc;
 
     if (!fused) {
         ProfScope prof("vote3_prep", st);
