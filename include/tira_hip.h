@@ -423,6 +423,15 @@ int tira_stick_saliency2_async(const float* field4, size_t n, float* saliency);
 int tira_otsu_threshold_async(const float* values, size_t n, int stride, float* threshold);
 /* mask[i] = values[i * stride] > *threshold (reconstruct.py:87), device pointers. */
 int tira_threshold_async(const float* values, size_t n, int stride, const float* threshold, unsigned char* mask);
+/* _fill_enclosed(boundary, close_radius) of saliency_threshold(fill=True)
+ * (reconstruct.py:92-116): a 0/1 byte mask of h x w (w fastest; nonzero = 1),
+ * closed with a (2r+1) x (2r+1) square when close_radius r > 0 (scipy's
+ * binary_closing, pixels outside the image count as 0), then holes filled
+ * (binary_fill_holes: background not 4-connected to the outside through
+ * background becomes 1). out receives 0/1 bytes. Host or device pointers;
+ * blocking. *sweeps (may be NULL) receives the number of flood sweeps. */
+int tira_fill_holes2(const unsigned char* mask, unsigned h, unsigned w, unsigned close_radius, unsigned char* out,
+                     unsigned* sweeps);
 /* lee(in, out, x, y, z) / computeThinImage, thinning/lee_thinning.h:600-706:
  * Lee et al. 1994 3D thinning of an int volume (x fastest; nonzero = object),
  * bit-exact, output 0/1. Host or device pointers; blocking. *iterations (may

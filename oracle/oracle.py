@@ -409,6 +409,50 @@ def otsu_threshold(values):
     return lib().oracle_otsu_threshold_f32(v, v.size)
 
 
+def fill_enclosed(boundary, close_radius):
+    """_fill_enclosed (reconstruct.py:98-116) exactly as the reference calls it:
+    scipy.ndimage.binary_closing with a (2r+1)^2 square (r > 0), then
+    binary_fill_holes (scipy 1.15.3, the image's version)."""
+    from scipy.ndimage import binary_closing, binary_fill_holes
+    b = np.asarray(boundary).astype(bool)
+    if close_radius and close_radius > 0:
+        size = 2 * int(close_radius) + 1
+        b = binary_closing(b, structure=np.ones((size, size), dtype=bool))
+    return binary_fill_holes(b)
+
+
+def fill_enclosed_bruteforce(boundary, close_radius):
+    """The same from the definitions, by loops (small masks only): dilation and
+    erosion with the square, pixels outside the image counting as 0; then a
+    breadth-first flood of the background from the border (4-connected); the
+    result is every pixel the flood did not reach."""
+    b = np.asarray(boundary).astype(bool)
+    H, W = b.shape
+    r = int(close_radius) if close_radius and close_radius > 0 else 0
+    if r:
+        def morph(m, dil):
+            o = np.zeros_like(m)
+            for y in range(H):
+                for x in range(W):
+                    vals = [m[y + j, x + i] if 0 <= y + j < H and 0 <= x + i < W else False
+                            for j in range(-r, r + 1) for i in range(-r, r + 1)]
+                    o[y, x] = any(vals) if dil else all(vals)
+            return o
+        b = morph(morph(b, True), False)
+    reach = np.zeros_like(b)
+    queue = [(y, x) for y in range(H) for x in range(W)
+             if (y in (0, H - 1) or x in (0, W - 1)) and not b[y, x]]
+    for y, x in queue:
+        reach[y, x] = True
+    while queue:
+        y, x = queue.pop()
+        for j, i in ((y - 1, x), (y + 1, x), (y, x - 1), (y, x + 1)):
+            if 0 <= j < H and 0 <= i < W and not b[j, i] and not reach[j, i]:
+                reach[j, i] = True
+                queue.append((j, i))
+    return ~reach
+
+
 def lee_thin3(vol):
     """computeThinImage (lee_thinning.h:600-680) of a (z, y, x) volume; nonzero = object.
     Returns (uint8 0/1 volume, outer iterations)."""

@@ -140,8 +140,73 @@ def test_saliency_threshold2(hip, oracle):
     mask = reconstruct.saliency_threshold(T.astype(np.float32))
     np.testing.assert_array_equal(mask, s > oracle.otsu_threshold(s))
     np.testing.assert_array_equal(reconstruct.saliency_threshold(T.astype(np.float32), threshold=0.7), s > 0.7)
-    with pytest.raises(NotImplementedError):
-        reconstruct.saliency_threshold(T, fill=True)
+    for r in (0, 1, 3):
+        got = reconstruct.saliency_threshold(T.astype(np.float32), fill=True, close_radius=r)
+        np.testing.assert_array_equal(got, oracle.fill_enclosed(s > oracle.otsu_threshold(s), r))
+
+
+def _spiral(n):
+    """A square spiral wall of side n: a one-pixel corridor opens on the left
+    edge at row 1 and winds to the centre, so a flood from the border crosses
+    tile edges many times."""
+    m = np.zeros((n, n), bool)
+    steps = ((0, 1), (1, 0), (0, -1), (-1, 0))
+    y = x = d = 0
+    m[0, 0] = True
+    turns = 0
+    while turns < 2:
+        dy, dx = steps[d]
+        y1, x1, y2, x2 = y + dy, x + dx, y + 2 * dy, x + 2 * dx
+        free2 = not (0 <= y2 < n and 0 <= x2 < n) or not m[y2, x2]
+        if 0 <= y1 < n and 0 <= x1 < n and not m[y1, x1] and free2:
+            y, x, turns = y1, x1, 0
+            m[y, x] = True
+        else:
+            d, turns = (d + 1) % 4, turns + 1
+    return m
+
+
+@pytest.mark.parametrize("shape,density,r", [
+    ((1, 1), 0.5, 0), ((1, 70), 0.5, 1), ((70, 1), 0.3, 0), ((37, 29), 0.4, 0), ((37, 29), 0.2, 2),
+    ((100, 257), 0.45, 0), ((257, 100), 0.12, 1), ((512, 384), 0.55, 0), ((512, 384), 0.08, 3),
+    ((1023, 65), 0.3, 5), ((64, 64), 0.0, 0), ((64, 64), 1.0, 0), ((64, 64), 1.0, 2)])
+def test_fill_enclosed_equals_scipy(hip, oracle, shape, density, r):
+    """_fill_enclosed (reconstruct.py:98-116) against scipy's binary_closing +
+    binary_fill_holes, bit-exact, on random masks of ragged shapes."""
+    print(f"fill shape={shape} density={density} r={r}", flush=True)
+    rng = np.random.default_rng(hash((shape, r)) % (1 << 32))
+    b = rng.random(shape) < density
+    got, sweeps = api.fill_holes2(b, r, return_sweeps=True)
+    np.testing.assert_array_equal(got, oracle.fill_enclosed(b, r))
+    assert sweeps >= 1
+
+
+def test_fill_enclosed_spiral(hip, oracle):
+    """A corridor winding from the border to the centre: the flood crosses tile
+    edges hundreds of times, so hundreds of sweeps run. Shut at its mouth, the
+    whole corridor is filled."""
+    m = _spiral(301)
+    got, sweeps = api.fill_holes2(m, 0, return_sweeps=True)
+    print(f"open spiral: sweeps={sweeps}", flush=True)
+    np.testing.assert_array_equal(got, oracle.fill_enclosed(m, 0))
+    assert sweeps > 100
+    m[1, 0] = True
+    ref = oracle.fill_enclosed(m, 0)
+    assert ref.all()
+    np.testing.assert_array_equal(api.fill_holes2(m, 0), ref)
+
+
+def test_fill_enclosed_device_pointers(hip, oracle):
+    rng = np.random.default_rng(5)
+    b = (rng.random((200, 333)) < 0.4).astype(np.uint8) * 3  # nonzero bytes count as 1
+    d_in, d_out = DeviceArray.from_host(b), DeviceArray(b.shape, np.uint8)
+    try:
+        call("tira_fill_holes2", d_in.ptr, 200, 333, 1, d_out.ptr, None)
+        got = d_out.to_host()
+    finally:
+        d_in.free(); d_out.free()
+    assert set(np.unique(got)) <= {0, 1}
+    np.testing.assert_array_equal(got.astype(bool), oracle.fill_enclosed(b != 0, 1))
 
 
 def test_eig3_magnitude_saliency(hip):

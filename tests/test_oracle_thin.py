@@ -210,3 +210,37 @@ def test_otsu_degenerate(oracle):
     h_np = np.histogram(v, bins=256, range=(0.0, float(vmax)))[0]
     assert h_np.sum() == v.size
     assert oracle.otsu_threshold(v) == _otsu_by_within_variance(v)
+
+
+@pytest.mark.parametrize("seed,r", [(0, 0), (1, 1), (2, 2), (3, 0), (4, 1)])
+def test_fill_enclosed_matches_definition(oracle, seed, r):
+    """The scipy route of saliency_threshold(fill=True) (reconstruct.py:98-116)
+    against a flood of the background from the border written from the
+    definitions, on small random masks."""
+    rng = np.random.default_rng(seed)
+    b = rng.random((13 + seed, 17)) < (0.35 + 0.05 * seed)
+    np.testing.assert_array_equal(oracle.fill_enclosed(b, r), oracle.fill_enclosed_bruteforce(b, r))
+
+
+def test_fill_enclosed_known_shapes(oracle):
+    ring = np.zeros((9, 9), bool)
+    ring[2:7, 2:7] = True
+    ring[3:6, 3:6] = False
+    full = np.zeros((9, 9), bool)
+    full[2:7, 2:7] = True
+    np.testing.assert_array_equal(oracle.fill_enclosed(ring, 0), full)
+    # a one-pixel gap in the ring lets the flood in; closing with r = 1 shuts it
+    gap = ring.copy()
+    gap[2, 4] = False
+    np.testing.assert_array_equal(oracle.fill_enclosed(gap, 0), gap)
+    np.testing.assert_array_equal(oracle.fill_enclosed(gap, 1), full)
+    # background is 4-connected: a pixel whose only background neighbours are
+    # diagonal is enclosed
+    diag = np.zeros((5, 5), bool)
+    diag[1, 2] = diag[2, 1] = diag[2, 3] = diag[3, 2] = True
+    out = diag.copy()
+    out[2, 2] = True
+    np.testing.assert_array_equal(oracle.fill_enclosed(diag, 0), out)
+    # closing with border_value 0 clears objects touching the image edge
+    edge = np.ones((6, 6), bool)
+    assert not oracle.fill_enclosed(edge, 1)[0].any()

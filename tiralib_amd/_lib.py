@@ -113,6 +113,7 @@ SIGNATURES = {
     "tira_lee_thin3_u8": [_vp, _u, _u, _u, _vp],
     "tira_thin3_classify_async": [_u, _u, _vp],
     "tira_skeleton3": [_vp, _i, _u, _u, _u, _vp, _vp, _vp, _vp],
+    "tira_fill_holes2": [_vp, _u, _u, _u, _vp, _vp],
     "tira_tensorvote3_atv": [_vp, _vp, _vp, _vp, _u, _u, _u, _f, _u],
     "tira_tensorvote3_atv_async": [_vp, _i, _vp, _i, _u, _u, _u, _f, _u],
     "tira_gaussian_filter3d": [_vp, _u, _u, _u, _f, _f, _f, _vp, C.POINTER(_u), C.POINTER(_u), C.POINTER(_u)],

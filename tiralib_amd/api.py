@@ -560,11 +560,27 @@ def otsu_threshold(values, stride=1, n=None):
             d.free()
 
 
-def saliency_threshold2(T, threshold="otsu"):
-    """reconstruct.saliency_threshold(T, fill=False) on the GPU: bool mask (H, W)."""
+def saliency_threshold2(T, threshold="otsu", fill=False, close_radius=0):
+    """reconstruct.saliency_threshold(T, fill, threshold, close_radius) on the GPU: bool mask (H, W)."""
     s = stick_saliency2(T)
     thr = otsu_threshold(s) if threshold == "otsu" else float(threshold)
-    return s > thr
+    binary = s > thr
+    return fill_holes2(binary, close_radius) if fill else binary
+
+
+def fill_holes2(boundary, close_radius=0, return_sweeps=False):
+    """_fill_enclosed(boundary, close_radius) (reconstruct.py:98-116) on the GPU:
+    closing with a (2r+1)^2 square when close_radius > 0, then hole filling.
+    bool (H, W) in and out."""
+    b = np.ascontiguousarray(np.asarray(boundary).astype(bool), dtype=np.uint8)
+    if b.ndim != 2:
+        raise ValueError(f"boundary must be 2D, got shape {b.shape}")
+    r = int(close_radius) if close_radius and close_radius > 0 else 0
+    out = np.empty_like(b)
+    sweeps = C.c_uint()
+    call("tira_fill_holes2", ptr(b), b.shape[0], b.shape[1], r, ptr(out), C.byref(sweeps))
+    out = out.astype(bool)
+    return (out, int(sweeps.value)) if return_sweeps else out
 
 
 def lee_thin3(volume):
