@@ -154,6 +154,25 @@ int tira_tensorvote2(float* t_out, const float* t_in, float sigma1, float sigma2
 int tira_tensorvote2_prebuilt(float* t_out, const float* t_in, const void* host_nb, int nb_count, float sigma1,
                               float sigma2, int power, unsigned w, unsigned shape0, unsigned shape1, int stick,
                               int plate, unsigned samples);
+/* Search-loop session: tira::cuda::tensorvote_prebuilt (tensorvote.h:2236-2305,
+ * bound as tensorvote_tk_prebuilt at tensorvote_tk.cu:239-253) with the field
+ * bound once, for optimize_tk's repeated small calls (optimize.py:256-288).
+ * create copies the s0 x s1 x 4 field (host or device) into the session;
+ * vote votes it with a host Neighbor2D table (nb, nb_count) or, with nb ==
+ * NULL, with the table build_neighbors2d makes from the parameters (as
+ * tira_tensorvote2). Each session keeps the device copies of its last 32
+ * tables (keyed by parameters or by the table's bytes) and runs each vote as
+ * one replay of an instantiated graph; results are identical to
+ * tira_tensorvote2_prebuilt / tira_tensorvote2. t_out host or device; blocking.
+ * A session belongs to the device current at create; calls on one session
+ * must not overlap. */
+int tira_vote2_session_create(void** session, const float* t_in, unsigned s0, unsigned s1);
+int tira_vote2_session_vote(void* session, float* t_out, const void* nb, int nb_count, float sigma1, float sigma2,
+                            int power, unsigned w, int stick, int plate, unsigned samples);
+/* Graph replays, table cache hits and table uploads of a session so far. */
+int tira_vote2_session_stats(void* session, unsigned long long* replays, unsigned long long* table_hits,
+                             unsigned long long* table_builds);
+int tira_vote2_session_destroy(void* session);
 /* No reference counterpart. tira_tensorvote2_prebuilt for callers that cannot
  * allocate device memory (code under stream capture, tight serving loops,
  * harnesses that forbid hipMalloc): the same kernels and results, with the

@@ -116,3 +116,82 @@ def test_vote2_table_kept_between_calls(hip):
     np.testing.assert_array_equal(b, b2)
     np.testing.assert_array_equal(t1, api.tensorvote(f, 2.0, 0.0, 1, 13, True, True, 20, is_atv=True))
     np.testing.assert_array_equal(t2, api.tensorvote(f, 3.0, 0.0, 1, 19, True, True, 20, is_atv=True))
+
+
+def _search_params(k, seed):
+    """Parameter sets as a differential-evolution population draws them over
+    optimize_tk's bounds (optimize.py:258-259), sigma1 capped for test time."""
+    rng = np.random.default_rng(seed)
+    return [(float(rng.uniform(1.0, 6.0)), float(rng.uniform(0.0, 4.0)), int(round(rng.uniform(1, 6))))
+            for _ in range(k)]
+
+
+def test_vote2_session_equals_module_calls(hip, oracle):
+    """Session votes (one graph replay each, cached tables) against the module
+    functions on the same field, bit for bit, with other library work and
+    waits between replays; repeated parameter sets hit the table cache."""
+    f = synth.bezier_field(seed=synth.SEED_CFG1)
+    other = _fields64(3)["mixed"]
+    params = _search_params(12, 42)
+    params += params[:4]  # repeats: served from the cache
+    with tvk.Session(f) as s:
+        for i, (s1, s2, p) in enumerate(params):
+            w = 2 * int(3 * max(s1, s2)) + 1
+            nb = oracle.build_neighbors2d(w, s1, s2, p, 20)
+            got = s.tensorvote_tk_prebuilt(nb.tobytes(), nb.size, w, s1, s2, p, plate=True, samples=20)
+            ref = tvk.tensorvote_tk_prebuilt(f, nb.tobytes(), nb.size, w, s1, s2, p, plate=True, samples=20)
+            np.testing.assert_array_equal(got, ref, err_msg=f"replay {i} ({s1}, {s2}, {p})")
+            got2 = s.tensorvote_tk(s1, s2, p, stick=(i % 3 != 0), plate=(i % 3 != 1))
+            ref2 = tvk.tensorvote_tk(f, s1, s2, p, stick=(i % 3 != 0), plate=(i % 3 != 1))
+            np.testing.assert_array_equal(got2, ref2, err_msg=f"replay {i} built table")
+            api.tensorvote(other, 2.0, 0.0, 1, 0, True, True, 20)  # eager work between replays
+        replays, hits, builds = s.stats()
+    assert replays == 2 * len(params)
+    assert hits >= 8 and builds <= 2 * 12
+    # the session's own result against the oracle on one parameter set
+    s1, s2, p = params[0]
+    w = 2 * int(3 * max(s1, s2)) + 1
+    with tvk.Session(f) as s:
+        got = s.tensorvote_tk(s1, s2, p)
+    assert rel_frob(got, oracle.tensorvote2(f, s1, s2, p, w, True, True, 20, nthreads=NT)) <= 1e-4
+
+
+def test_vote2_session_device_output_and_eviction(hip, oracle):
+    """Device output through the C ABI, more distinct tables than the cache
+    holds (the first is rebuilt after eviction), and odd field shapes."""
+    from tiralib_amd._lib import DeviceArray, call
+    import ctypes as C
+    f = _fields64(4)["mixed"][:37, :53].copy()
+    h = C.c_void_p()
+    call("tira_vote2_session_create", C.byref(h), f.ctypes.data, 37, 53)
+    out = DeviceArray((37, 53, 4))
+    try:
+        params = [(1.0 + 0.1 * k, 0.5, 1 + k % 3) for k in range(40)] + [(1.0, 0.5, 1)]
+        for k, (s1, s2, p) in enumerate(params):
+            w = 2 * int(3 * max(s1, s2)) + 1
+            call("tira_vote2_session_vote", h, out.ptr, None, 0, s1, s2, p, w, 1, 1, 20)
+            ref = tvk.tensorvote_tk(f, s1, s2, p, w=w)
+            np.testing.assert_array_equal(out.to_host(), ref, err_msg=f"call {k}")
+            if k % 10 == 0:  # a host output in between rebuilds the graph both ways
+                host = np.empty((37, 53, 4), np.float32)
+                call("tira_vote2_session_vote", h, host.ctypes.data, None, 0, s1, s2, p, w, 1, 1, 20)
+                np.testing.assert_array_equal(host, ref)
+        r, hits, builds = (C.c_ulonglong(), C.c_ulonglong(), C.c_ulonglong())
+        call("tira_vote2_session_stats", h, C.byref(r), C.byref(hits), C.byref(builds))
+        assert builds.value == 41 and hits.value == 5
+    finally:
+        out.free()
+        call("tira_vote2_session_destroy", h)
+
+
+def test_vote2_session_errors(hip):
+    f = _fields64(5)["stick"]
+    s = tvk.Session(f)
+    nb = np.zeros(3, tvk.NEIGHBOR2D_DTYPE)
+    with pytest.raises(RuntimeError, match="nb_buf byte size"):
+        s.tensorvote_tk_prebuilt(nb.tobytes(), 4, 3, 1.0)
+    with pytest.raises(RuntimeError, match="sigma1 must be > 0"):
+        s.tensorvote_tk(0.0)
+    s.close()
+    with pytest.raises(RuntimeError, match="closed"):
+        s.tensorvote_tk(1.0)

@@ -414,6 +414,173 @@ static int scratch_address(void** out) {
     return TIRA_OK;
 }
 
+// ---------------------------------------------------------------- search-loop session
+// A parameter search (differential evolution over sigma1, sigma2, power on
+// one field) votes the same field thousands of times with small changes. The
+// session keeps the field on the device, keeps the device copies of the last
+// kSessionTables neighbour tables (by parameters, or by the bytes of a
+// caller's table), and launches each vote as one replay of an instantiated
+// graph (payload pass, gather, copy-out) whose kernel arguments are updated in
+// place: one launch and one wait per call instead of an upload of the field,
+// a table copy, two kernel launches and a download.
+constexpr int kSessionTables = 32;
+
+struct SessionTable {
+    // key: parameters when built here, or the caller's bytes (size + FNV-1a)
+    bool built = false;
+    int w = 0, power = 0, stick = 0, plate = 0;
+    unsigned samples = 0;
+    float s1 = 0.0f, s2 = 0.0f;
+    unsigned long long hash = 0;
+    std::vector<unsigned char> bytes;
+    Nb2* dev = nullptr;
+    int count = 0;
+    unsigned long long used = 0;
+};
+
+struct Vote2Session {
+    int device = -1;
+    unsigned s0 = 0, s1 = 0;
+    size_t n = 0;
+    float4* field = nullptr;
+    float4* pay = nullptr;
+    float4* out = nullptr;      // device result when the caller's output is host memory
+    float4* pinned = nullptr;   // page-locked staging of a host result
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraphNode_t n_prep = nullptr, n_vote = nullptr;
+    int graph_host_out = -1;
+    unsigned long long clock = 0, replays = 0, table_hits = 0, table_builds = 0;
+    SessionTable tabs[kSessionTables];
+};
+
+static unsigned long long fnv1a(const void* p, size_t n) {
+    const unsigned char* b = (const unsigned char*)p;
+    unsigned long long h = 1469598103934665603ull;
+    for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * 1099511628211ull;
+    return h;
+}
+
+static void session_free(Vote2Session* s) {
+    if (!s) return;
+    if (s->exec) (void)hipGraphExecDestroy(s->exec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    for (auto& t : s->tabs)
+        if (t.dev) (void)hipFree(t.dev);
+    if (s->field) (void)hipFree(s->field);
+    if (s->pay) (void)hipFree(s->pay);
+    if (s->out) (void)hipFree(s->out);
+    if (s->pinned) (void)hipHostFree(s->pinned);
+    delete s;
+}
+
+// The device table for this call: a cached entry, or a new one (built from the
+// parameters when `nb` is NULL) in the least recently used slot.
+static int session_table(Vote2Session& s, const Nb2* nb, int nbc, int w, float s1, float s2, unsigned power,
+                         unsigned samples, bool stick, bool plate, const Nb2** dev, int* count) {
+    const bool built = nb == nullptr;
+    const size_t nbytes = built ? 0 : (size_t)nbc * sizeof(Nb2);
+    const unsigned long long h = built ? 0 : fnv1a(nb, nbytes);
+    s.clock++;
+    SessionTable* lru = &s.tabs[0];
+    for (auto& t : s.tabs) {
+        const bool hit = t.used && t.built == built &&
+                         (built ? (t.w == w && t.s1 == s1 && t.s2 == s2 && t.power == (int)power &&
+                                   t.samples == samples && t.stick == (int)stick && t.plate == (int)plate)
+                                : (t.hash == h && t.bytes.size() == nbytes &&
+                                   std::memcmp(t.bytes.data(), nb, nbytes) == 0));
+        if (hit) {
+            t.used = s.clock;
+            s.table_hits++;
+            *dev = t.dev;
+            *count = t.count;
+            return TIRA_OK;
+        }
+        if (t.used < lru->used) lru = &t;
+    }
+    std::vector<Nb2> host;
+    if (built) {
+        if (stick || plate) build_neighbors2d(w, s1, s2, power, samples, stick, plate, host);
+        nb = host.data();
+        nbc = (int)host.size();
+    }
+    SessionTable& t = *lru;
+    if (t.dev) {
+        // the graph's last replay may still read the slot being replaced
+        TH_HIP(hipStreamSynchronize(stream()));
+        TH_HIP(hipFree(t.dev));
+        t.dev = nullptr;
+    }
+    t.used = 0;
+    TH_HIP(hipMalloc((void**)&t.dev, (size_t)(nbc > 0 ? nbc : 1) * sizeof(Nb2)));
+    if (nbc > 0) TH_HIP(hipMemcpy(t.dev, nb, (size_t)nbc * sizeof(Nb2), hipMemcpyHostToDevice));
+    t.built = built;
+    t.w = w; t.s1 = s1; t.s2 = s2; t.power = (int)power; t.samples = samples;
+    t.stick = stick; t.plate = plate;
+    t.hash = h;
+    if (built) t.bytes.clear();
+    else t.bytes.assign((const unsigned char*)nb, (const unsigned char*)nb + nbytes);
+    t.count = nbc;
+    t.used = s.clock;
+    s.table_builds++;
+    *dev = t.dev;
+    *count = nbc;
+    return TIRA_OK;
+}
+
+// Kernel-node parameters of one call: the argument values live in `a`, which
+// must outlive the graph update that reads them.
+struct SessionArgs {
+    const float4* field; size_t n; float eta_s; float4* pay;
+    const Nb2* nb; int nbc, s0, s1, power, stick, plate; float* out;
+    void* prep[4];
+    void* vote[9];
+    void bind() {
+        prep[0] = &field; prep[1] = &n; prep[2] = &eta_s; prep[3] = &pay;
+        vote[0] = &pay; vote[1] = &nb; vote[2] = &nbc; vote[3] = &s0; vote[4] = &s1;
+        vote[5] = &power; vote[6] = &stick; vote[7] = &plate; vote[8] = &out;
+    }
+};
+
+static hipKernelNodeParams prep_node(const Vote2Session& s, SessionArgs& a) {
+    hipKernelNodeParams p{};
+    p.func = reinterpret_cast<void*>(k_vote2_prep);
+    p.gridDim = dim3(ceil_div(s.n, 256));
+    p.blockDim = dim3(256);
+    p.sharedMemBytes = 0;
+    p.kernelParams = a.prep;
+    p.extra = nullptr;
+    return p;
+}
+
+static hipKernelNodeParams vote_node(const Vote2Session& s, SessionArgs& a) {
+    hipKernelNodeParams p{};
+    p.func = reinterpret_cast<void*>(k_vote2);
+    p.gridDim = dim3(ceil_div(s.s1, 64), ceil_div(s.s0, 4));
+    p.blockDim = dim3(256);
+    p.sharedMemBytes = 0;
+    p.kernelParams = a.vote;
+    p.extra = nullptr;
+    return p;
+}
+
+static int session_build_graph(Vote2Session& s, SessionArgs& a, bool host_out) {
+    if (s.exec) { TH_HIP(hipGraphExecDestroy(s.exec)); s.exec = nullptr; }
+    if (s.graph) { TH_HIP(hipGraphDestroy(s.graph)); s.graph = nullptr; }
+    TH_HIP(hipGraphCreate(&s.graph, 0));
+    const hipKernelNodeParams pp = prep_node(s, a), pv = vote_node(s, a);
+    TH_HIP(hipGraphAddKernelNode(&s.n_prep, s.graph, nullptr, 0, &pp));
+    TH_HIP(hipGraphAddKernelNode(&s.n_vote, s.graph, &s.n_prep, 1, &pv));
+    if (host_out) {
+        hipGraphNode_t n_copy = nullptr;
+        TH_HIP(hipGraphAddMemcpyNode1D(&n_copy, s.graph, &s.n_vote, 1, s.pinned, s.out, s.n * 16,
+                                       hipMemcpyDeviceToHost));
+    }
+    TH_HIP(hipGraphInstantiate(&s.exec, s.graph, nullptr, nullptr, 0));
+    s.graph_host_out = host_out ? 1 : 0;
+    return TIRA_OK;
+}
+
 }  // namespace th
 
 using namespace th;
@@ -530,6 +697,105 @@ int tira_tensorvote2_tk(float* t_out, const float* lambdas, const float* thetas,
 int tira_tensorvote2_atv(float* t_out, const float* lambdas, const float* thetas, float sigma, unsigned w,
                          unsigned s0, unsigned s1) {
     return vote2_atv_entry(t_out, nullptr, lambdas, thetas, sigma, w, s0, s1);
+}
+
+// ---------------------------------------------------------------- search-loop session
+// tensorvote_tk_prebuilt (tensorvote_tk.cu:239-253, tensorvote.h:2236-2305)
+// with the field bound once, for optimize_tk's repeated calls
+// (optimize.py:256-288).
+int tira_vote2_session_create(void** session, const float* t_in, unsigned s0, unsigned s1) {
+    TH_ARG(session && t_in, "NULL pointer");
+    TH_ARG(s0 > 0 && s1 > 0, "empty field");
+    *session = nullptr;
+    Vote2Session* s = new Vote2Session();
+    s->s0 = s0;
+    s->s1 = s1;
+    s->n = (size_t)s0 * s1;
+    const size_t bytes = s->n * 16;
+    int st = TIRA_OK;
+    do {
+        if (hipGetDevice(&s->device) != hipSuccess ||
+            hipMalloc((void**)&s->field, bytes) != hipSuccess || hipMalloc((void**)&s->pay, bytes) != hipSuccess) {
+            set_error("device allocation for the vote session failed (%zu bytes per buffer)", bytes);
+            st = TIRA_ERR_OUT_OF_MEMORY;
+            break;
+        }
+        if (hipMemcpy(s->field, t_in, bytes, hipMemcpyDefault) != hipSuccess) {
+            set_error("copying the field into the vote session failed");
+            st = TIRA_ERR_HIP;
+        }
+    } while (0);
+    if (st != TIRA_OK) { session_free(s); (void)hipGetLastError(); return st; }
+    *session = s;
+    return TIRA_OK;
+}
+
+// One vote of the bound field: with `nb` (nb_count 48-byte Neighbor2D entries,
+// host memory) like tensorvote_tk_prebuilt, or with nb == NULL from the
+// parameters like tensorvote_tk (table built by build_neighbors2d). t_out:
+// s0 x s1 x 4 floats, host or device. Blocking.
+int tira_vote2_session_vote(void* session, float* t_out, const void* nb, int nb_count, float sigma1, float sigma2,
+                            int power, unsigned w, int stick, int plate, unsigned samples) {
+    TH_ARG(session && t_out, "NULL pointer");
+    TH_ARG(sigma1 >= 0.0f && sigma2 >= 0.0f, "sigma must be >= 0");
+    TH_ARG(nb_count >= 0 && (nb_count == 0 || nb), "bad neighbour table");
+    Vote2Session& s = *(Vote2Session*)session;
+    int dev = 0;
+    TH_HIP(hipGetDevice(&dev));
+    TH_ARG(dev == s.device, "the session lives on device %d, the current device is %d", s.device, dev);
+    TH_ARG(nb || w >= 1, "w must be >= 1");
+    TH_ARG(!nb || !is_device(nb), "the session takes a host neighbour table");
+    const unsigned p = power < 1 ? 1 : (unsigned)power;
+    const bool use_plate = plate && samples > 0;
+    const Nb2* dnb = nullptr;
+    int nbc = 0;
+    TH_TRY(session_table(s, (const Nb2*)nb, nb_count, (int)w, sigma1, sigma2, p, samples, stick != 0, use_plate,
+                         &dnb, &nbc));
+    const bool host_out = !is_device(t_out);
+    if (host_out && !s.out) {
+        TH_HIP(hipMalloc((void**)&s.out, s.n * 16));
+        TH_HIP(hipHostMalloc((void**)&s.pinned, s.n * 16, hipHostMallocDefault));
+    }
+    SessionArgs a;
+    a.field = s.field; a.n = s.n; a.eta_s = stick_norm_2d(sigma1, sigma2, p); a.pay = s.pay;
+    a.nb = dnb; a.nbc = nbc; a.s0 = (int)s.s0; a.s1 = (int)s.s1; a.power = (int)p;
+    a.stick = stick ? 1 : 0; a.plate = use_plate ? 1 : 0;
+    a.out = host_out ? (float*)s.out : t_out;
+    a.bind();
+    if (s.graph_host_out != (host_out ? 1 : 0)) {
+        TH_TRY(session_build_graph(s, a, host_out));
+    } else {
+        const hipKernelNodeParams pp = prep_node(s, a), pv = vote_node(s, a);
+        TH_HIP(hipGraphExecKernelNodeSetParams(s.exec, s.n_prep, &pp));
+        TH_HIP(hipGraphExecKernelNodeSetParams(s.exec, s.n_vote, &pv));
+    }
+    hipStream_t st = stream();
+    {
+        ProfScope prof("vote2_session", st);
+        TH_HIP(hipGraphLaunch(s.exec, st));
+    }
+    TH_HIP(hipStreamSynchronize(st));
+    s.replays++;
+    if (host_out) std::memcpy(t_out, s.pinned, s.n * 16);
+    return TIRA_OK;
+}
+
+// Counters of a session: graph replays, table cache hits and table uploads.
+int tira_vote2_session_stats(void* session, unsigned long long* replays, unsigned long long* table_hits,
+                             unsigned long long* table_builds) {
+    TH_ARG(session, "NULL session");
+    const Vote2Session& s = *(const Vote2Session*)session;
+    if (replays) *replays = s.replays;
+    if (table_hits) *table_hits = s.table_hits;
+    if (table_builds) *table_builds = s.table_builds;
+    return TIRA_OK;
+}
+
+int tira_vote2_session_destroy(void* session) {
+    if (!session) return TIRA_OK;
+    TH_HIP(hipStreamSynchronize(stream()));
+    session_free((Vote2Session*)session);
+    return TIRA_OK;
 }
 
 }  // extern "C"
