@@ -154,16 +154,25 @@ int tira_tensorvote2(float* t_out, const float* t_in, float sigma1, float sigma2
 int tira_tensorvote2_prebuilt(float* t_out, const float* t_in, const void* host_nb, int nb_count, float sigma1,
                               float sigma2, int power, unsigned w, unsigned shape0, unsigned shape1, int stick,
                               int plate, unsigned samples);
+/* cpu::build_neighbors2d (tensorvote.h:1223-1280) on the device: the w * w
+ * Neighbor2D entries (dv outer, du inner) into dev_out, non-blocking. The same
+ * operations as tira_build_neighbors2d, FP64 cos/sin/exp rounded to float,
+ * no contraction; c1 and c2 take the correctly rounded float exponential
+ * where the host takes glibc's expf. */
+int tira_build_neighbors2d_async(int w, float sigma1, float sigma2, unsigned power, unsigned samples, int stick,
+                                 int plate, void* dev_out);
 /* Search-loop session: tira::cuda::tensorvote_prebuilt (tensorvote.h:2236-2305,
  * bound as tensorvote_tk_prebuilt at tensorvote_tk.cu:239-253) with the field
  * bound once, for optimize_tk's repeated small calls (optimize.py:256-288).
  * create copies the s0 x s1 x 4 field (host or device) into the session;
  * vote votes it with a host Neighbor2D table (nb, nb_count) or, with nb ==
- * NULL, with the table build_neighbors2d makes from the parameters (as
- * tira_tensorvote2). Each session keeps the device copies of its last 32
- * tables (keyed by parameters or by the table's bytes) and runs each vote as
- * one replay of an instantiated graph; results are identical to
- * tira_tensorvote2_prebuilt / tira_tensorvote2. t_out host or device; blocking.
+ * NULL, with the table built from the parameters on the device
+ * (tira_build_neighbors2d_async). Each session keeps the device copies of its
+ * last 32 tables (keyed by parameters or by the table's bytes) and runs each
+ * vote as one replay of an instantiated graph. Results are identical to
+ * tira_tensorvote2_prebuilt with the same table; with a device-built table
+ * they are within a few FP32 ulps of tira_tensorvote2 (whose host table can
+ * differ in c1, c2 by one ulp). t_out host or device; blocking.
  * A session belongs to the device current at create; calls on one session
  * must not overlap. */
 int tira_vote2_session_create(void** session, const float* t_in, unsigned s0, unsigned s1);

@@ -128,8 +128,10 @@ def _search_params(k, seed):
 
 def test_vote2_session_equals_module_calls(hip, oracle):
     """Session votes (one graph replay each, cached tables) against the module
-    functions on the same field, bit for bit, with other library work and
-    waits between replays; repeated parameter sets hit the table cache."""
+    functions on the same field, with other library work and waits between
+    replays: bit for bit with the caller's table, within 1e-6 of the largest
+    component with the table built on the device; repeated parameter sets hit
+    the table cache."""
     f = synth.bezier_field(seed=synth.SEED_CFG1)
     other = _fields64(3)["mixed"]
     params = _search_params(12, 42)
@@ -143,7 +145,8 @@ def test_vote2_session_equals_module_calls(hip, oracle):
             np.testing.assert_array_equal(got, ref, err_msg=f"replay {i} ({s1}, {s2}, {p})")
             got2 = s.tensorvote_tk(s1, s2, p, stick=(i % 3 != 0), plate=(i % 3 != 1))
             ref2 = tvk.tensorvote_tk(f, s1, s2, p, stick=(i % 3 != 0), plate=(i % 3 != 1))
-            np.testing.assert_array_equal(got2, ref2, err_msg=f"replay {i} built table")
+            # table built on the device: c1, c2 may sit one ulp from the host's
+            assert np.abs(got2 - ref2).max() <= 1e-6 * np.abs(ref2).max(), f"replay {i} built table"
             api.tensorvote(other, 2.0, 0.0, 1, 0, True, True, 20)  # eager work between replays
         replays, hits, builds = s.stats()
     assert replays == 2 * len(params)
@@ -171,11 +174,12 @@ def test_vote2_session_device_output_and_eviction(hip, oracle):
             w = 2 * int(3 * max(s1, s2)) + 1
             call("tira_vote2_session_vote", h, out.ptr, None, 0, s1, s2, p, w, 1, 1, 20)
             ref = tvk.tensorvote_tk(f, s1, s2, p, w=w)
-            np.testing.assert_array_equal(out.to_host(), ref, err_msg=f"call {k}")
+            got = out.to_host()
+            assert np.abs(got - ref).max() <= 1e-6 * np.abs(ref).max(), f"call {k}"
             if k % 10 == 0:  # a host output in between rebuilds the graph both ways
                 host = np.empty((37, 53, 4), np.float32)
                 call("tira_vote2_session_vote", h, host.ctypes.data, None, 0, s1, s2, p, w, 1, 1, 20)
-                np.testing.assert_array_equal(host, ref)
+                np.testing.assert_array_equal(host, got)
         r, hits, builds = (C.c_ulonglong(), C.c_ulonglong(), C.c_ulonglong())
         call("tira_vote2_session_stats", h, C.byref(r), C.byref(hits), C.byref(builds))
         assert builds.value == 41 and hits.value == 5
@@ -195,3 +199,30 @@ def test_vote2_session_errors(hip):
     s.close()
     with pytest.raises(RuntimeError, match="closed"):
         s.tensorvote_tk(1.0)
+
+
+def test_build_neighbors2d_device_equals_host(hip):
+    """The device table build against the host build (the reference's
+    arithmetic): bit-identical entries, c1 and c2 within one ulp (glibc's expf
+    against the correctly rounded exponential), over a search population."""
+    from tiralib_amd._lib import DeviceArray, call
+    params = _search_params(40, 7) + [(3.0, 0.0, 1), (5.0, 2.0, 3), (0.5, 0.0, 1), (30.0, 20.0, 30)]
+    ulps = 0
+    for s1, s2, p in params:
+        for stick, plate, N in ((1, 1, 20), (1, 0, 20), (0, 1, 7)):
+            w = 2 * int(3 * max(s1, s2)) + 1
+            host = api.build_neighbors2d(w, s1, s2, p, N, stick, plate)
+            d = DeviceArray((w * w * 12,))
+            try:
+                call("tira_build_neighbors2d_async", w, s1, s2, p, N, stick, plate, d.ptr)
+                call("tira_hip_synchronize")
+                dev = d.to_host().view(tvk.NEIGHBOR2D_DTYPE).reshape(-1)
+            finally:
+                d.free()
+            for k in ("du", "dv", "dx", "dy", "Ra", "Rb", "Rc", "Va", "Vb", "Vc"):
+                np.testing.assert_array_equal(dev[k], host[k], err_msg=f"{k} at {(s1, s2, p, stick, plate)}")
+            for k in ("c1", "c2"):
+                diff = np.abs(dev[k].view(np.int32).astype(np.int64) - host[k].view(np.int32).astype(np.int64))
+                assert diff.max() <= 1, (k, s1, s2, p)
+                ulps += int((diff > 0).sum())
+    print(f"c1/c2 entries one ulp apart: {ulps}")

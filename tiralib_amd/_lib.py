@@ -96,6 +96,7 @@ SIGNATURES = {
     "tira_tensorvote2": [_vp, _vp, _f, _f, _i, _u, _u, _u, _i, _i, _u, _i],
     "tira_tensorvote2_prebuilt": [_vp, _vp, _vp, _i, _f, _f, _i, _u, _u, _u, _i, _i, _u],
     "tira_tensorvote2_prebuilt_noalloc": [_vp, _vp, _vp, _i, _f, _f, _i, _u, _u, _u, _i, _i, _u],
+    "tira_build_neighbors2d_async": [_i, _f, _f, _u, _u, _i, _i, _vp],
     "tira_vote2_session_create": [C.POINTER(_vp), _vp, _u, _u],
     "tira_vote2_session_vote": [_vp, _vp, _vp, _i, _f, _f, _i, _u, _i, _i, _u],
     "tira_vote2_session_stats": [_vp, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)],

@@ -26,27 +26,43 @@ struct Nb2 {  // tira::Neighbor2D, 48 bytes (tensorvote.h:776-786)
 };
 static_assert(sizeof(Nb2) == 48, "Neighbor2D must stay 48 bytes");
 
-// ---------------------------------------------------------------- host table
+// ---------------------------------------------------------------- neighbour table
 // Same arithmetic as the reference (tensorvote.h:34-40, 803-914, 1047-1071):
 // the unqualified cos/sin/exp of the reference's float code resolve to the
 // double overloads under the host compiler, so they are evaluated in double
-// and rounded here too.
-static float atten(float dist, float sigma) {
-    if (sigma == 0.0f) return (dist > 1e-12f) ? 0.0f : 1.0f;
-    return (float)std::exp((double)(-(dist * dist) / (sigma * sigma)));
+// and rounded here too. The functions serve the host build and the device
+// build (k_build_nb2) alike, with contraction off so that both round every
+// operation as the reference's host code does; the only other difference is
+// the float exponential of c1, c2 (glibc's expf on the host, the double
+// exponential rounded to float on the device: equal wherever expf rounds
+// correctly).
+__host__ __device__ inline float exp_f32(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return (float)exp((double)x);
+#else
+    return std::exp(x);
+#endif
 }
 
-float stick_norm_2d(float s1, float s2, unsigned power) {
+__host__ __device__ inline float atten(float dist, float sigma) {
+#pragma clang fp contract(off)
+    if (sigma == 0.0f) return (dist > 1e-12f) ? 0.0f : 1.0f;
+    return (float)exp((double)(-(dist * dist) / (sigma * sigma)));
+}
+
+__host__ __device__ float stick_norm_2d(float s1, float s2, unsigned power) {
+#pragma clang fp contract(off)
     if (power < 1) power = 1;
     float ratio = 1.0f;
     for (unsigned k = 1; k <= power; ++k) ratio *= (float)(2 * k) / (float)(2 * k - 1);
     return ratio / ((float)M_PI * (s1 * s1 + s2 * s2));
 }
 
-static void stickvote_2d(float rx, float ry, float s1, float s2, unsigned power, float theta, float& Va, float& Vb,
-                         float& Vc) {
-    const float qx = (float)std::cos((double)theta), qy = (float)std::sin((double)theta);
-    const float dist = std::sqrt(rx * rx + ry * ry);
+__host__ __device__ inline void stickvote_2d(float rx, float ry, float s1, float s2, unsigned power, float theta,
+                                             float& Va, float& Vb, float& Vc) {
+#pragma clang fp contract(off)
+    const float qx = (float)cos((double)theta), qy = (float)sin((double)theta);
+    const float dist = sqrtf(rx * rx + ry * ry);
     float dx = 0.0f, dy = 0.0f;
     if (dist != 0.0f) { dx = rx / dist; dy = ry / dist; }
     const float qTd = qx * dx + qy * dy;
@@ -68,8 +84,9 @@ static void stickvote_2d(float rx, float ry, float s1, float s2, unsigned power,
     Vc = Rqy * Rqy * factor;
 }
 
-static void plate_numerical_2d(float rx, float ry, float s1, float s2, unsigned power, float& a, float& b, float& c,
-                               unsigned n) {
+__host__ __device__ inline void plate_numerical_2d(float rx, float ry, float s1, float s2, unsigned power, float& a,
+                                                   float& b, float& c, unsigned n) {
+#pragma clang fp contract(off)
     a = b = c = 0.0f;
     if (n == 0) return;
     const float dth = (float)(M_PI / (double)(float)n);
@@ -82,35 +99,47 @@ static void plate_numerical_2d(float rx, float ry, float s1, float s2, unsigned 
     a *= wb; b *= wb; c *= wb;
 }
 
+// Entry k of the w x w table (dv outer, du inner; tensorvote.h:1223-1280).
+__host__ __device__ inline Nb2 neighbor2d(int k, int w, float s1, float s2, unsigned power, unsigned samples,
+                                          bool stick, bool plate) {
+#pragma clang fp contract(off)
+    const int hw = w / 2;
+    const float is1 = (s1 > 0.0f) ? 1.0f / (s1 * s1) : 0.0f;
+    const float is2 = (s2 > 0.0f) ? 1.0f / (s2 * s2) : 0.0f;
+    Nb2 n{};
+    const int dv = k / w - hw, du = k % w - hw;
+    n.du = du; n.dv = dv;
+    const float l2 = (float)(du * du + dv * dv);
+    if (stick) {
+        if (l2 != 0.0f) {
+            const float il = 1.0f / sqrtf(l2);
+            n.dx = (float)du * il;
+            n.dy = (float)dv * il;
+        }
+        n.c1 = (s1 == 0.0f) ? (l2 > 1e-12f ? 0.0f : 1.0f) : exp_f32(-l2 * is1);
+        n.c2 = (s2 == 0.0f) ? (l2 > 1e-12f ? 0.0f : 1.0f) : exp_f32(-l2 * is2);
+        n.Ra = 1.0f - 2.0f * n.dx * n.dx;
+        n.Rb = 0.0f - 2.0f * n.dx * n.dy;
+        n.Rc = 1.0f - 2.0f * n.dy * n.dy;
+    }
+    if (plate && samples > 0) plate_numerical_2d((float)du, (float)dv, s1, s2, power, n.Va, n.Vb, n.Vc, samples);
+    return n;
+}
+
 int build_neighbors2d(int w, float s1, float s2, unsigned power, unsigned samples, bool stick, bool plate,
                       std::vector<Nb2>& nb) {
     nb.clear();
     if (w <= 0) return 0;
-    const int hw = w / 2;
-    const float is1 = (s1 > 0.0f) ? 1.0f / (s1 * s1) : 0.0f;
-    const float is2 = (s2 > 0.0f) ? 1.0f / (s2 * s2) : 0.0f;
     nb.reserve((size_t)w * w);
-    for (int dv = -hw; dv <= hw; dv++)
-        for (int du = -hw; du <= hw; du++) {
-            Nb2 n{};
-            n.du = du; n.dv = dv;
-            const float l2 = (float)(du * du + dv * dv);
-            if (stick) {
-                if (l2 != 0.0f) {
-                    const float il = 1.0f / std::sqrt(l2);
-                    n.dx = (float)du * il;
-                    n.dy = (float)dv * il;
-                }
-                n.c1 = (s1 == 0.0f) ? (l2 > 1e-12f ? 0.0f : 1.0f) : std::exp(-l2 * is1);
-                n.c2 = (s2 == 0.0f) ? (l2 > 1e-12f ? 0.0f : 1.0f) : std::exp(-l2 * is2);
-                n.Ra = 1.0f - 2.0f * n.dx * n.dx;
-                n.Rb = 0.0f - 2.0f * n.dx * n.dy;
-                n.Rc = 1.0f - 2.0f * n.dy * n.dy;
-            }
-            if (plate && samples > 0) plate_numerical_2d((float)du, (float)dv, s1, s2, power, n.Va, n.Vb, n.Vc, samples);
-            nb.push_back(n);
-        }
+    for (int k = 0; k < w * w; k++) nb.push_back(neighbor2d(k, w, s1, s2, power, samples, stick, plate));
     return (int)nb.size();
+}
+
+__global__ __launch_bounds__(256) void k_build_nb2(int w, float s1, float s2, unsigned power, unsigned samples,
+                                                   int stick, int plate, Nb2* __restrict__ out) {
+    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= w * w) return;
+    out[k] = neighbor2d(k, w, s1, s2, power, samples, stick != 0, plate != 0);
 }
 
 // Device copy of a host table in workspace `slot`, kept while its bytes and
@@ -498,12 +527,7 @@ static int session_table(Vote2Session& s, const Nb2* nb, int nbc, int w, float s
         }
         if (t.used < lru->used) lru = &t;
     }
-    std::vector<Nb2> host;
-    if (built) {
-        if (stick || plate) build_neighbors2d(w, s1, s2, power, samples, stick, plate, host);
-        nb = host.data();
-        nbc = (int)host.size();
-    }
+    if (built) nbc = (stick || plate) && w > 0 ? w * w : 0;
     SessionTable& t = *lru;
     if (t.dev) {
         // the graph's last replay may still read the slot being replaced
@@ -513,7 +537,14 @@ static int session_table(Vote2Session& s, const Nb2* nb, int nbc, int w, float s
     }
     t.used = 0;
     TH_HIP(hipMalloc((void**)&t.dev, (size_t)(nbc > 0 ? nbc : 1) * sizeof(Nb2)));
-    if (nbc > 0) TH_HIP(hipMemcpy(t.dev, nb, (size_t)nbc * sizeof(Nb2), hipMemcpyHostToDevice));
+    if (built && nbc > 0) {
+        // built on the device, ahead of the vote on the same stream
+        k_build_nb2<<<ceil_div((size_t)nbc, 256), 256, 0, stream()>>>(w, s1, s2, power, samples, stick ? 1 : 0,
+                                                                      plate ? 1 : 0, t.dev);
+        TH_HIP(hipGetLastError());
+    } else if (nbc > 0) {
+        TH_HIP(hipMemcpy(t.dev, nb, (size_t)nbc * sizeof(Nb2), hipMemcpyHostToDevice));
+    }
     t.built = built;
     t.w = w; t.s1 = s1; t.s2 = s2; t.power = (int)power; t.samples = samples;
     t.stick = stick; t.plate = plate;
@@ -597,6 +628,20 @@ int tira_build_neighbors2d(int w, float sigma1, float sigma2, unsigned power, un
         TH_ARG(capacity >= k, "capacity %d < %d entries", capacity, k);
         std::memcpy(out, nb.data(), (size_t)k * sizeof(Nb2));
     }
+    return TIRA_OK;
+}
+
+// cpu::build_neighbors2d (tensorvote.h:1223-1280) on the device: all w * w
+// entries into dev_out (device memory), non-blocking.
+int tira_build_neighbors2d_async(int w, float sigma1, float sigma2, unsigned power, unsigned samples, int stick,
+                                 int plate, void* dev_out) {
+    TH_ARG(dev_out, "NULL pointer");
+    TH_ARG(w >= 1 && w <= 46339, "w must be 1..46339");
+    TH_ARG(sigma1 >= 0.0f && sigma2 >= 0.0f, "sigma must be >= 0");
+    const size_t n = (size_t)w * w;
+    k_build_nb2<<<ceil_div(n, 256), 256, 0, stream()>>>(w, sigma1, sigma2, power < 1 ? 1 : power, samples,
+                                                         stick ? 1 : 0, plate ? 1 : 0, (Nb2*)dev_out);
+    TH_HIP(hipGetLastError());
     return TIRA_OK;
 }
 
@@ -732,7 +777,8 @@ int tira_vote2_session_create(void** session, const float* t_in, unsigned s0, un
 
 // One vote of the bound field: with `nb` (nb_count 48-byte Neighbor2D entries,
 // host memory) like tensorvote_tk_prebuilt, or with nb == NULL from the
-// parameters like tensorvote_tk (table built by build_neighbors2d). t_out:
+// parameters like tensorvote_tk (table built on the device by the same
+// arithmetic as build_neighbors2d). t_out:
 // s0 x s1 x 4 floats, host or device. Blocking.
 int tira_vote2_session_vote(void* session, float* t_out, const void* nb, int nb_count, float sigma1, float sigma2,
                             int power, unsigned w, int stick, int plate, unsigned samples) {
