@@ -647,6 +647,56 @@ def bench_vote2d(steps=20):
             "note": "launch- and synchronisation-bound: 2.28e7 pairs are ~4 us of FP32 VALU at peak"}
 
 
+def bench_search_loop(population=45, seed=42):
+    """The parameter search of optimize_tk (optimize.py:256-288) on the config-1
+    field: one differential-evolution population (45 members drawn over its
+    bounds sigma1 in [1, 30], sigma2 in [0, 20], power in [1, 30], seed 42),
+    each member one plate + stick vote with N = 20. Per evaluation: the module
+    call (host table build, field upload, vote, download, as each of the
+    reference's calls does) against the session (field bound once, table built
+    on the device, one graph replay), and the session again on the same
+    members (tables cached). Device time of the session's graph from the
+    library's events."""
+    from tiralib_amd import _lib, synth, tensorvote_tk as tvk
+    from tiralib_amd._lib import call
+    f = synth.bezier_field(seed=synth.SEED_CFG1)
+    H, W = f.shape[:2]
+    rng = np.random.default_rng(seed)
+    pop = [(float(rng.uniform(1.0, 30.0)), float(rng.uniform(0.0, 20.0)), max(1, int(round(rng.uniform(1, 30)))))
+           for _ in range(population)]
+    wins = [2 * int(3 * max(a, b)) + 1 for a, b, _ in pop]
+    pairs = sum(in_bounds_pairs((H, W), w) for w in wins)
+    tvk.tensorvote_tk(f, *pop[0], plate=True, samples=20)  # warm
+    t = time.perf_counter()
+    for s1, s2, p in pop:
+        tvk.tensorvote_tk(f, s1, s2, p, plate=True, samples=20)
+    module_ms = (time.perf_counter() - t) / population * 1e3
+    with tvk.Session(f) as ses:
+        ses.tensorvote_tk(1.0, 0.0, 1)  # warm: graph instantiation
+        call("tira_hip_profile_begin")
+        t = time.perf_counter()
+        for s1, s2, p in pop:
+            ses.tensorvote_tk(s1, s2, p, plate=True, samples=20)
+        ses_ms = (time.perf_counter() - t) / population * 1e3
+        call("tira_hip_profile_end")
+        graph_ms = _lib.profile_get("vote2_session")[0]
+        t = time.perf_counter()
+        for s1, s2, p in pop:
+            ses.tensorvote_tk(s1, s2, p, plate=True, samples=20)
+        cached_ms = (time.perf_counter() - t) / population * 1e3
+        replays, hits, builds = ses.stats()
+    return {"workload": "optimize_tk's search loop: 45-member DE population over sigma1 [1,30], sigma2 [0,20], "
+                        "power [1,30] (seed 42), plate + stick, N=20, on the 256x256 config-1 field",
+            "pairs_per_evaluation_mean": int(pairs / population),
+            "module_ms_per_evaluation": round(module_ms, 3),
+            "session_ms_per_evaluation": round(ses_ms, 3),
+            "session_cached_tables_ms_per_evaluation": round(cached_ms, 3),
+            "session_graph_device_ms_mean": round(graph_ms, 3),
+            "speedup_session_vs_module": round(module_ms / ses_ms, 2),
+            "session_gpairs_s": round(pairs / population / (ses_ms * 1e-3) / 1e9, 1),
+            "session_stats": {"replays": replays, "table_hits": hits, "tables_built": builds}}
+
+
 def eig_errors(got, ref6, significant=1e-2):
     """Eigenvalue errors of `got` (n x 3, ascending) against LAPACK FP64
     eigenvalues of the FP64 packed-6 tensors ref6. frobenius_rel: the
@@ -1040,6 +1090,7 @@ def main(argv=None):
         out["config2_vote_eig3"] = bench_config2(100, 3, a.plate_mode, a.no_atv, cpu_s, log)
     if not a.no_vote2d:
         out["vote2d"] = bench_vote2d()
+        out["search_loop"] = bench_search_loop()
         log("2D vote done")
     out["calibration"] = calibrate()
     if not a.no_batch:

@@ -168,7 +168,7 @@ int tira_build_neighbors2d_async(int w, float sigma1, float sigma2, unsigned pow
  * vote votes it with a host Neighbor2D table (nb, nb_count) or, with nb ==
  * NULL, with the table built from the parameters on the device
  * (tira_build_neighbors2d_async). Each session keeps the device copies of its
- * last 32 tables (keyed by parameters or by the table's bytes) and runs each
+ * last 64 tables (keyed by parameters or by the table's bytes) and runs each
  * vote as one replay of an instantiated graph. Results are identical to
  * tira_tensorvote2_prebuilt with the same table; with a device-built table
  * they are within a few FP32 ulps of tira_tensorvote2 (whose host table can

@@ -169,7 +169,7 @@ def test_vote2_session_device_output_and_eviction(hip, oracle):
     call("tira_vote2_session_create", C.byref(h), f.ctypes.data, 37, 53)
     out = DeviceArray((37, 53, 4))
     try:
-        params = [(1.0 + 0.1 * k, 0.5, 1 + k % 3) for k in range(40)] + [(1.0, 0.5, 1)]
+        params = [(1.0 + 0.1 * k, 0.5, 1 + k % 3) for k in range(70)] + [(1.0, 0.5, 1)]
         for k, (s1, s2, p) in enumerate(params):
             w = 2 * int(3 * max(s1, s2)) + 1
             call("tira_vote2_session_vote", h, out.ptr, None, 0, s1, s2, p, w, 1, 1, 20)
@@ -182,7 +182,7 @@ def test_vote2_session_device_output_and_eviction(hip, oracle):
                 np.testing.assert_array_equal(host, got)
         r, hits, builds = (C.c_ulonglong(), C.c_ulonglong(), C.c_ulonglong())
         call("tira_vote2_session_stats", h, C.byref(r), C.byref(hits), C.byref(builds))
-        assert builds.value == 41 and hits.value == 5
+        assert builds.value == 71 and hits.value == 8  # 64 tables kept: the first is rebuilt
     finally:
         out.free()
         call("tira_vote2_session_destroy", h)

@@ -206,14 +206,20 @@ __global__ __launch_bounds__(256) void k_vote2_prep_tk(const float2* __restrict_
 }
 
 // Table-driven gather: receiver (x0, x1) sums over the table entries
-// (tk_stick_window 938-1005 + tk_plate_window 1100-1115).
+// (tk_stick_window 938-1005 + tk_plate_window 1100-1115). A small field does
+// not fill the chip with one thread per receiver (256 x 256 pixels are 1024
+// waves, one per SIMD), so the table is split over blockIdx.z into chunks of
+// `per` entries: chunk z writes its partial sums to part[z * n + pixel] and
+// k_vote2_reduce adds the chunks in order (deterministic). With part == NULL
+// (one chunk) the sums go straight to out.
 __global__ __launch_bounds__(256) void k_vote2(const float4* __restrict__ pay, const Nb2* __restrict__ nb, int nbc,
-                                               int s0, int s1, int power, int stick, int plate,
-                                               float* __restrict__ out) {
+                                               int per, int s0, int s1, int power, int stick, int plate,
+                                               float* __restrict__ out, float4* __restrict__ part) {
     const int x1 = blockIdx.x * 64 + (threadIdx.x & 63);
     const int x0 = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int k0 = (int)blockIdx.z * per, k1 = min(nbc, k0 + per);
     float a = 0.0f, b = 0.0f, c = 0.0f;
-    for (int k = 0; k < nbc; k++) {
+    for (int k = k0; k < k1; k++) {
         const Nb2 e = nb[k];
         const int r0 = x0 + e.dv, r1 = x1 + e.du;
         if ((unsigned)r0 >= (unsigned)s0 || (unsigned)r1 >= (unsigned)s1) continue;
@@ -239,10 +245,34 @@ __global__ __launch_bounds__(256) void k_vote2(const float4* __restrict__ pay, c
         }
     }
     if (x0 < s0 && x1 < s1) {
-        float4* o = reinterpret_cast<float4*>(out) + (size_t)x0 * s1 + x1;
-        *o = make_float4(a, b, b, c);
+        const size_t p = (size_t)x0 * s1 + x1;
+        if (part) part[(size_t)blockIdx.z * ((size_t)s0 * s1) + p] = make_float4(a, b, c, 0.0f);
+        else reinterpret_cast<float4*>(out)[p] = make_float4(a, b, b, c);
     }
 }
+
+__global__ __launch_bounds__(256) void k_vote2_reduce(const float4* __restrict__ part, int chunks, size_t n,
+                                                      float* __restrict__ out) {
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    float4 v = part[p];
+    for (int z = 1; z < chunks; z++) {
+        const float4 q = part[(size_t)z * n + p];
+        v.x += q.x; v.y += q.y; v.z += q.z;
+    }
+    reinterpret_cast<float4*>(out)[p] = make_float4(v.x, v.y, v.y, v.z);
+}
+
+// Table chunks for an s0 x s1 field: enough blocks for about 8 waves per
+// SIMD (2048 blocks of 4 waves), at least 64 entries per chunk, at most 32.
+constexpr int kVote2TargetBlocks = 2048;
+static int vote2_chunks(unsigned s0, unsigned s1, int nbc) {
+    const size_t blocks = (size_t)ceil_div(s1, 64) * ceil_div(s0, 4);
+    if (nbc < 128 || blocks >= (size_t)kVote2TargetBlocks) return 1;
+    const int want = (int)ceil_div(kVote2TargetBlocks, blocks);
+    return std::max(1, std::min(std::min(want, nbc / 64), 32));
+}
+static int vote2_per(int nbc, int chunks) { return chunks > 1 ? (nbc + chunks - 1) / chunks : nbc; }
 
 // ---------------------------------------------------------------- analytical tensor voting (ATV)
 // tira::atv 2D (tensorvote.h:159-253): c R H R with R = I - 2 v v^T and
@@ -383,7 +413,7 @@ static int vote2_atv_entry(float* t_out, const float* t_in, const float* lam, co
 // table's V entries are zero otherwise.
 static int vote2_launch(const float* tin, const float* lam, const float* th, float* out, float4* pay, const Nb2* dnb,
                         int nbc, float s1, float s2, int power, unsigned s0, unsigned s1n, bool stick, bool plate,
-                        unsigned samples, hipStream_t st) {
+                        unsigned samples, float4* part, hipStream_t st) {
     const size_t n = (size_t)s0 * s1n;
     const unsigned p = power < 1 ? 1 : (unsigned)power;
     const float eta_s = stick_norm_2d(s1, s2, p);
@@ -393,14 +423,23 @@ static int vote2_launch(const float* tin, const float* lam, const float* th, flo
         else k_vote2_prep_tk<<<ceil_div(n, 256), 256, 0, st>>>((const float2*)lam, (const float2*)th, n, eta_s, pay);
     }
     TH_HIP(hipGetLastError());
-    dim3 grid(ceil_div(s1n, 64), ceil_div(s0, 4));
+    const int chunks = vote2_chunks(s0, s1n, nbc);
+    dim3 grid(ceil_div(s1n, 64), ceil_div(s0, 4), chunks);
     {
         ProfScope prof("vote2", st);
-        k_vote2<<<grid, 256, 0, st>>>(pay, dnb, nbc, (int)s0, (int)s1n, (int)p, stick ? 1 : 0,
-                                      (plate && samples > 0) ? 1 : 0, out);
+        k_vote2<<<grid, 256, 0, st>>>(pay, dnb, nbc, vote2_per(nbc, chunks), (int)s0, (int)s1n, (int)p,
+                                      stick ? 1 : 0, (plate && samples > 0) ? 1 : 0, out,
+                                      chunks > 1 ? part : nullptr);
+        if (chunks > 1) k_vote2_reduce<<<ceil_div(n, 256), 256, 0, st>>>(part, chunks, n, out);
     }
     TH_HIP(hipGetLastError());
     return TIRA_OK;
+}
+
+// Bytes of partial sums vote2_launch needs for this field and table.
+static size_t vote2_part_bytes(unsigned s0, unsigned s1, int nbc) {
+    const int chunks = vote2_chunks(s0, s1, nbc);
+    return chunks > 1 ? (size_t)chunks * s0 * s1 * 16 : 0;
 }
 
 static int vote2_run(const float* tin, const float* lam, const float* th, float* out, const Nb2* host_nb, int nbc,
@@ -413,7 +452,10 @@ static int vote2_run(const float* tin, const float* lam, const float* th, float*
     Nb2* dnb = nullptr;
     static thread_local TableCache nb_cache;
     TH_TRY(table_to_device(WS_NB2, host_nb, (size_t)(nbc > 0 ? nbc : 0) * sizeof(Nb2), nb_cache, st, (void**)&dnb));
-    return vote2_launch(tin, lam, th, out, pay, dnb, nbc, s1, s2, power, s0, s1n, stick, plate, samples, st);
+    float4* part = nullptr;
+    const size_t pb = vote2_part_bytes(s0, s1n, nbc);
+    if (pb) TH_TRY(ws_get(WS_PAY1, pb, (void**)&part));
+    return vote2_launch(tin, lam, th, out, pay, dnb, nbc, s1, s2, power, s0, s1n, stick, plate, samples, part, st);
 }
 
 // ---------------------------------------------------------------- allocation-free 2D vote
@@ -452,15 +494,15 @@ static int scratch_address(void** out) {
 // graph (payload pass, gather, copy-out) whose kernel arguments are updated in
 // place: one launch and one wait per call instead of an upload of the field,
 // a table copy, two kernel launches and a download.
-constexpr int kSessionTables = 32;
+constexpr int kSessionTables = 64;
 
 struct SessionTable {
-    // key: parameters when built here, or the caller's bytes (size + FNV-1a)
+    // key: parameters when built here, or the caller's bytes (tables of other
+    // parameters differ within the first entries, so memcmp settles a miss early)
     bool built = false;
     int w = 0, power = 0, stick = 0, plate = 0;
     unsigned samples = 0;
     float s1 = 0.0f, s2 = 0.0f;
-    unsigned long long hash = 0;
     std::vector<unsigned char> bytes;
     Nb2* dev = nullptr;
     int count = 0;
@@ -475,20 +517,15 @@ struct Vote2Session {
     float4* pay = nullptr;
     float4* out = nullptr;      // device result when the caller's output is host memory
     float4* pinned = nullptr;   // page-locked staging of a host result
+    float4* part = nullptr;     // partial sums of the table chunks
+    size_t part_cap = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    hipGraphNode_t n_prep = nullptr, n_vote = nullptr;
+    hipGraphNode_t n_prep = nullptr, n_vote = nullptr, n_reduce = nullptr;
     int graph_host_out = -1;
     unsigned long long clock = 0, replays = 0, table_hits = 0, table_builds = 0;
     SessionTable tabs[kSessionTables];
 };
-
-static unsigned long long fnv1a(const void* p, size_t n) {
-    const unsigned char* b = (const unsigned char*)p;
-    unsigned long long h = 1469598103934665603ull;
-    for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * 1099511628211ull;
-    return h;
-}
 
 static void session_free(Vote2Session* s) {
     if (!s) return;
@@ -499,6 +536,7 @@ static void session_free(Vote2Session* s) {
     if (s->field) (void)hipFree(s->field);
     if (s->pay) (void)hipFree(s->pay);
     if (s->out) (void)hipFree(s->out);
+    if (s->part) (void)hipFree(s->part);
     if (s->pinned) (void)hipHostFree(s->pinned);
     delete s;
 }
@@ -509,15 +547,13 @@ static int session_table(Vote2Session& s, const Nb2* nb, int nbc, int w, float s
                          unsigned samples, bool stick, bool plate, const Nb2** dev, int* count) {
     const bool built = nb == nullptr;
     const size_t nbytes = built ? 0 : (size_t)nbc * sizeof(Nb2);
-    const unsigned long long h = built ? 0 : fnv1a(nb, nbytes);
     s.clock++;
     SessionTable* lru = &s.tabs[0];
     for (auto& t : s.tabs) {
         const bool hit = t.used && t.built == built &&
                          (built ? (t.w == w && t.s1 == s1 && t.s2 == s2 && t.power == (int)power &&
                                    t.samples == samples && t.stick == (int)stick && t.plate == (int)plate)
-                                : (t.hash == h && t.bytes.size() == nbytes &&
-                                   std::memcmp(t.bytes.data(), nb, nbytes) == 0));
+                                : (t.bytes.size() == nbytes && std::memcmp(t.bytes.data(), nb, nbytes) == 0));
         if (hit) {
             t.used = s.clock;
             s.table_hits++;
@@ -548,7 +584,6 @@ static int session_table(Vote2Session& s, const Nb2* nb, int nbc, int w, float s
     t.built = built;
     t.w = w; t.s1 = s1; t.s2 = s2; t.power = (int)power; t.samples = samples;
     t.stick = stick; t.plate = plate;
-    t.hash = h;
     if (built) t.bytes.clear();
     else t.bytes.assign((const unsigned char*)nb, (const unsigned char*)nb + nbytes);
     t.count = nbc;
@@ -560,51 +595,55 @@ static int session_table(Vote2Session& s, const Nb2* nb, int nbc, int w, float s
 }
 
 // Kernel-node parameters of one call: the argument values live in `a`, which
-// must outlive the graph update that reads them.
+// must outlive the graph update that reads them. The session always takes the
+// chunked form (partial sums, then the ordered reduction), also with one chunk,
+// so that the graph's shape does not depend on the table.
 struct SessionArgs {
     const float4* field; size_t n; float eta_s; float4* pay;
-    const Nb2* nb; int nbc, s0, s1, power, stick, plate; float* out;
+    const Nb2* nb; int nbc, per, s0, s1, power, stick, plate; float* out; float4* part;
+    int chunks;
     void* prep[4];
-    void* vote[9];
+    void* vote[11];
+    void* reduce[4];
     void bind() {
         prep[0] = &field; prep[1] = &n; prep[2] = &eta_s; prep[3] = &pay;
-        vote[0] = &pay; vote[1] = &nb; vote[2] = &nbc; vote[3] = &s0; vote[4] = &s1;
-        vote[5] = &power; vote[6] = &stick; vote[7] = &plate; vote[8] = &out;
+        vote[0] = &pay; vote[1] = &nb; vote[2] = &nbc; vote[3] = &per; vote[4] = &s0; vote[5] = &s1;
+        vote[6] = &power; vote[7] = &stick; vote[8] = &plate; vote[9] = &out; vote[10] = &part;
+        reduce[0] = &part; reduce[1] = &chunks; reduce[2] = &n; reduce[3] = &out;
     }
 };
 
-static hipKernelNodeParams prep_node(const Vote2Session& s, SessionArgs& a) {
+static hipKernelNodeParams kernel_node(void* func, dim3 grid, void** args) {
     hipKernelNodeParams p{};
-    p.func = reinterpret_cast<void*>(k_vote2_prep);
-    p.gridDim = dim3(ceil_div(s.n, 256));
+    p.func = func;
+    p.gridDim = grid;
     p.blockDim = dim3(256);
     p.sharedMemBytes = 0;
-    p.kernelParams = a.prep;
+    p.kernelParams = args;
     p.extra = nullptr;
     return p;
 }
 
-static hipKernelNodeParams vote_node(const Vote2Session& s, SessionArgs& a) {
-    hipKernelNodeParams p{};
-    p.func = reinterpret_cast<void*>(k_vote2);
-    p.gridDim = dim3(ceil_div(s.s1, 64), ceil_div(s.s0, 4));
-    p.blockDim = dim3(256);
-    p.sharedMemBytes = 0;
-    p.kernelParams = a.vote;
-    p.extra = nullptr;
-    return p;
-}
+struct SessionNodes {
+    hipKernelNodeParams prep, vote, reduce;
+    SessionNodes(const Vote2Session& s, SessionArgs& a)
+        : prep(kernel_node(reinterpret_cast<void*>(k_vote2_prep), dim3(ceil_div(s.n, 256)), a.prep)),
+          vote(kernel_node(reinterpret_cast<void*>(k_vote2), dim3(ceil_div(s.s1, 64), ceil_div(s.s0, 4), a.chunks),
+                           a.vote)),
+          reduce(kernel_node(reinterpret_cast<void*>(k_vote2_reduce), dim3(ceil_div(s.n, 256)), a.reduce)) {}
+};
 
 static int session_build_graph(Vote2Session& s, SessionArgs& a, bool host_out) {
     if (s.exec) { TH_HIP(hipGraphExecDestroy(s.exec)); s.exec = nullptr; }
     if (s.graph) { TH_HIP(hipGraphDestroy(s.graph)); s.graph = nullptr; }
     TH_HIP(hipGraphCreate(&s.graph, 0));
-    const hipKernelNodeParams pp = prep_node(s, a), pv = vote_node(s, a);
-    TH_HIP(hipGraphAddKernelNode(&s.n_prep, s.graph, nullptr, 0, &pp));
-    TH_HIP(hipGraphAddKernelNode(&s.n_vote, s.graph, &s.n_prep, 1, &pv));
+    const SessionNodes nodes(s, a);
+    TH_HIP(hipGraphAddKernelNode(&s.n_prep, s.graph, nullptr, 0, &nodes.prep));
+    TH_HIP(hipGraphAddKernelNode(&s.n_vote, s.graph, &s.n_prep, 1, &nodes.vote));
+    TH_HIP(hipGraphAddKernelNode(&s.n_reduce, s.graph, &s.n_vote, 1, &nodes.reduce));
     if (host_out) {
         hipGraphNode_t n_copy = nullptr;
-        TH_HIP(hipGraphAddMemcpyNode1D(&n_copy, s.graph, &s.n_vote, 1, s.pinned, s.out, s.n * 16,
+        TH_HIP(hipGraphAddMemcpyNode1D(&n_copy, s.graph, &s.n_reduce, 1, s.pinned, s.out, s.n * 16,
                                        hipMemcpyDeviceToHost));
     }
     TH_HIP(hipGraphInstantiate(&s.exec, s.graph, nullptr, nullptr, 0));
@@ -701,11 +740,14 @@ int tira_tensorvote2_prebuilt_noalloc(float* t_out, const float* t_in, const voi
     TH_ARG(sigma1 >= 0.0f && sigma2 >= 0.0f, "sigma must be >= 0");
     (void)w;
     const size_t n = (size_t)s0 * s1;
+    // scratch layout: payload | table | partial sums of the table chunks
     const size_t pay_bytes = (n * 16 + 255) & ~(size_t)255;
     const size_t tab_bytes = (size_t)nb_count * sizeof(Nb2);
-    TH_ARG(pay_bytes + tab_bytes <= kNoAllocScratchBytes,
+    const size_t part_off = pay_bytes + ((tab_bytes + 255) & ~(size_t)255);
+    const size_t need = part_off + vote2_part_bytes(s0, s1, nb_count);
+    TH_ARG(need <= kNoAllocScratchBytes,
            "a %u x %u field with %d table entries needs %zu bytes of scratch; the allocation-free vote holds %zu "
-           "(use tira_tensorvote2_prebuilt)", s0, s1, nb_count, pay_bytes + tab_bytes, kNoAllocScratchBytes);
+           "(use tira_tensorvote2_prebuilt)", s0, s1, nb_count, need, kNoAllocScratchBytes);
     if (!n) return TIRA_OK;
     TH_ARG(is_device(t_in) && is_device(t_out), "the allocation-free vote takes device fields (it stages nothing)");
     void* base = nullptr;
@@ -718,8 +760,9 @@ int tira_tensorvote2_prebuilt_noalloc(float* t_out, const float* t_in, const voi
         TH_HIP(hipMemcpyAsync(tab, nb, tab_bytes, hipMemcpyHostToDevice, st));
         dnb = tab;
     }
+    float4* part = (float4*)((unsigned char*)base + part_off);
     TH_TRY(vote2_launch(t_in, nullptr, nullptr, t_out, pay, dnb, nb_count, sigma1, sigma2, power, s0, s1,
-                        stick != 0, plate != 0, samples, st));
+                        stick != 0, plate != 0, samples, part, st));
     TH_HIP(hipStreamSynchronize(st));
     return TIRA_OK;
 }
@@ -802,18 +845,30 @@ int tira_vote2_session_vote(void* session, float* t_out, const void* nb, int nb_
         TH_HIP(hipMalloc((void**)&s.out, s.n * 16));
         TH_HIP(hipHostMalloc((void**)&s.pinned, s.n * 16, hipHostMallocDefault));
     }
+    const int chunks = vote2_chunks(s.s0, s.s1, nbc);
+    const size_t part_bytes = (size_t)chunks * s.n * 16;
+    if (part_bytes > s.part_cap) {
+        if (s.part) TH_HIP(hipFree(s.part));  // the previous call has finished (blocking)
+        s.part = nullptr;
+        s.part_cap = 0;
+        TH_HIP(hipMalloc((void**)&s.part, part_bytes));
+        s.part_cap = part_bytes;
+    }
     SessionArgs a;
     a.field = s.field; a.n = s.n; a.eta_s = stick_norm_2d(sigma1, sigma2, p); a.pay = s.pay;
-    a.nb = dnb; a.nbc = nbc; a.s0 = (int)s.s0; a.s1 = (int)s.s1; a.power = (int)p;
-    a.stick = stick ? 1 : 0; a.plate = use_plate ? 1 : 0;
+    a.nb = dnb; a.nbc = nbc; a.per = vote2_per(nbc, chunks); a.s0 = (int)s.s0; a.s1 = (int)s.s1;
+    a.power = (int)p; a.stick = stick ? 1 : 0; a.plate = use_plate ? 1 : 0;
     a.out = host_out ? (float*)s.out : t_out;
+    a.part = s.part;
+    a.chunks = chunks;
     a.bind();
     if (s.graph_host_out != (host_out ? 1 : 0)) {
         TH_TRY(session_build_graph(s, a, host_out));
     } else {
-        const hipKernelNodeParams pp = prep_node(s, a), pv = vote_node(s, a);
-        TH_HIP(hipGraphExecKernelNodeSetParams(s.exec, s.n_prep, &pp));
-        TH_HIP(hipGraphExecKernelNodeSetParams(s.exec, s.n_vote, &pv));
+        const SessionNodes nodes(s, a);
+        TH_HIP(hipGraphExecKernelNodeSetParams(s.exec, s.n_prep, &nodes.prep));
+        TH_HIP(hipGraphExecKernelNodeSetParams(s.exec, s.n_vote, &nodes.vote));
+        TH_HIP(hipGraphExecKernelNodeSetParams(s.exec, s.n_reduce, &nodes.reduce));
     }
     hipStream_t st = stream();
     {
