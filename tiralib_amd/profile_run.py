@@ -4,10 +4,10 @@
     python -m tiralib_amd.profile_run [--steps 5]
 
 Runs a 1 GiB plain copy (the counter calibration reference: 1 GiB read,
-1 GiB written) three times, then `steps` iterations of: the 3D vote workload
-of bench.py (128^3 tubes + sheets, sigma 2, w 13, N 20) with the spectral
-vote and with the direct gather, and the analytical vote (ATV) of the same
-volume; the spectral vote of the 256^3 curved-sheet volume (sigma 4, w 25); the batched eig3 +
+1 GiB written) three times, then `steps` iterations of: the 3D vote + eig3
+step on 128^3 tubes + sheets (sigma 2, w 13, N 20) with the spectral vote
+(eigenvalues written by its inverse x pass) and with the direct gather, and
+the analytical vote (ATV) of the same volume; the spectral vote of the 256^3 curved-sheet volume (sigma 4, w 25); the batched eig3 +
 Gaussian launch and the one-launch eig3 of its 48 volumes; the learned plate
 vote on a 96^3 curved-sheet volume (sigma 4, w 25; untrained weights with
 the error check off: the kernel's work does not depend on the weights, and
@@ -64,14 +64,14 @@ def main(argv=None):
     field = DeviceArray.from_host(synth.tubes_sheets_volume(shape, seed=synth.SEED_CFG2))
     n = int(np.prod(shape))
     vote, ev = DeviceArray((n, 6)), DeviceArray((n, 3))
-    for mode in (_lib.PLATE_AUTO, _lib.PLATE_CLOSED):  # the spectral vote, then the direct gather
+    # the benchmark's step: the spectral vote whose inverse x pass also writes
+    # the eigenvalues (one call), then the direct gather followed by eig3
+    for mode in (_lib.PLATE_AUTO, _lib.PLATE_CLOSED):
         for _ in range(a.steps):
-            call("tira_tensorvote3_async", field.ptr, _lib.LAYOUT_MAT9, vote.ptr, _lib.LAYOUT_PACKED6, *shape, 2.0,
-                 0.0, 13, 1, 1, 1, 20, mode)
-            call("tira_eig3_async", vote.ptr, n, _lib.LAYOUT_PACKED6, ev.ptr, None, None, _lib.EIG_SCALE_INVARIANT,
-                 None)
+            call("tira_tensorvote3_eig_async", field.ptr, _lib.LAYOUT_MAT9, vote.ptr, _lib.LAYOUT_PACKED6, *shape,
+                 2.0, 0.0, 13, 1, 1, 1, 20, mode, ev.ptr, _lib.EIG_REFINE_FP64 | _lib.EIG_SCALE_INVARIANT, None)
         call("tira_hip_synchronize")
-    print(f"vote3: {a.steps} steps of 128^3 sigma 2 w 13 N 20, spectral then gather", flush=True)
+    print(f"vote3: {a.steps} steps of 128^3 sigma 2 w 13 N 20 (vote + eig3), spectral then gather", flush=True)
     if not a.skip_mlp:
         sshape = (256, 256, 256)
         svol = synth.curved_sheets_volume_device(sshape, 12, 0.02, seed=synth.SEED_CFG4)
