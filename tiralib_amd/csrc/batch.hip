@@ -3,8 +3,12 @@
 // eigenvalues of every input tensor, captured in a hipGraph.
 //
 // Work items are (volume, 16 x 16 tile, z-chunk) triples, ordered largest
-// volume first, and handed out by an atomic counter to a persistent grid of
-// blocks (blocks/CU x CUs). Each item runs the fused Gaussian tile; because
+// volume first, and handed out to a persistent grid of blocks (blocks/CU x
+// CUs). The item range is cut into one contiguous part per XCD, each with its
+// own atomic counter: block b runs on XCD b % 8 and takes the items of its
+// XCD's part in order, so neighbouring tiles are in flight on one XCD together
+// and share their halo columns through its L2; a block whose part is used up
+// takes items from the other parts. Each item runs the fused Gaussian tile; because
 // the tile stages every input plane of its columns in LDS anyway, the same
 // item also eigen-decomposes those voxels from LDS, so each input tensor is
 // read from HBM once for both results (24 B in, 24 B smoothed + 12 B
@@ -39,6 +43,12 @@ struct BatchVolume {
 constexpr int kBatchR = 5;  // 11 taps; other radii dispatch below
 constexpr int kMaxVolumes = 4096;
 
+constexpr unsigned kBatchParts = 8;  // one item part (and counter) per XCD
+
+__device__ __forceinline__ unsigned batch_part_begin(unsigned total, unsigned q) {
+    return (unsigned)(((unsigned long long)total * q) / kBatchParts);
+}
+
 template <int R>
 __global__ __launch_bounds__(512) void k_batch_gauss_eig(const BatchVolume* __restrict__ vols, int nvol,
                                                          unsigned total, unsigned* __restrict__ counter,
@@ -48,12 +58,28 @@ __global__ __launch_bounds__(512) void k_batch_gauss_eig(const BatchVolume* __re
     __shared__ __attribute__((aligned(16))) float s_raw[6 * PW * RS];
     __shared__ __attribute__((aligned(16))) float s_xf[6 * PW * kGT];
     __shared__ unsigned s_item;
+    const unsigned home = blockIdx.x % kBatchParts;
+    unsigned spent = 0;  // parts this block found used up (thread 0 only)
     while (true) {
-        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
+        if (threadIdx.x == 0) {
+            // the first part, from the home XCD's on, that still has an item;
+            // each counter passes its part's end at most once per block, so the
+            // search ends and every block leaves once all parts are used up
+            unsigned item = total;
+            for (unsigned t = 0; t < kBatchParts; t++) {
+                const unsigned q = (home + t) % kBatchParts;
+                if (spent & (1u << q)) continue;
+                const unsigned lo = batch_part_begin(total, q), hi = batch_part_begin(total, q + 1);
+                const unsigned c = atomicAdd(&counter[q], 1u);
+                if (c < hi - lo) { item = lo + c; break; }
+                spent |= 1u << q;
+            }
+            s_item = item;
+        }
         __syncthreads();
         const unsigned item = s_item;
         __syncthreads();  // every lane has read s_item before it is overwritten
-        if (item >= total) break;  // uniform: every block drains the counter and exits
+        if (item >= total) break;  // uniform: every block drains the counters and exits
         int v = 0;
         while (v + 1 < nvol && vols[v + 1].first_item <= item) v++;
         const BatchVolume& bv = vols[v];
@@ -61,7 +87,9 @@ __global__ __launch_bounds__(512) void k_batch_gauss_eig(const BatchVolume* __re
     }
 }
 
-__global__ void k_reset_counter(unsigned* c) { *c = 0u; }
+__global__ void k_reset_counter(unsigned* c) {
+    if (threadIdx.x < kBatchParts) c[threadIdx.x] = 0u;
+}
 
 struct BatchPlan {
     int zchunk = 32;
@@ -132,7 +160,7 @@ static int build_items(Batch& b, int zchunk, hipStream_t st) {
 
 static int launch_batch(Batch& b, hipStream_t st) {
     const unsigned grid = (unsigned)(b.ncu * b.plan.blocks_per_cu);
-    k_reset_counter<<<1, 1, 0, st>>>(b.d_counter);
+    k_reset_counter<<<1, 64, 0, st>>>(b.d_counter);
     switch (b.R) {
         case 3: k_batch_gauss_eig<3><<<grid, 512, 0, st>>>(b.d_vols, b.nvol, b.total, b.d_counter, b.d_taps); break;
         case 4: k_batch_gauss_eig<4><<<grid, 512, 0, st>>>(b.d_vols, b.nvol, b.total, b.d_counter, b.d_taps); break;

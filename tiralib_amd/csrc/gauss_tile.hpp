@@ -199,7 +199,9 @@ __global__ __launch_bounds__(GaussThreads<C>::N) void k_gauss3(const float* __re
     constexpr int RS = (PW + 3 + 3) & ~3;
     __shared__ __attribute__((aligned(16))) float s_raw[C * PW * RS];
     __shared__ __attribute__((aligned(16))) float s_xf[C * PW * kGT];
-    gauss3_tile<R, C, false>(in, out, taps, g, blockIdx.x, nullptr, s_raw, s_xf);
+    // XCD-major tile order: neighbouring tiles (x fastest) run on one XCD at
+    // the same time, so the halo columns they share come from its L2
+    gauss3_tile<R, C, false>(in, out, taps, g, xcd_remap(blockIdx.x, gridDim.x), nullptr, s_raw, s_xf);
 }
 
 }  // namespace th

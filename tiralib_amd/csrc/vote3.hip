@@ -375,6 +375,16 @@ __device__ __forceinline__ void tile_coords(const VoteGeom& g, int bz_tile, unsi
     (void)bz_tile;
 }
 
+// The same with z fastest: the tiles stacked along z, which share most of
+// their voter planes, run on one XCD at the same time.
+__device__ __forceinline__ void tile_coords_zfast(const VoteGeom& g, unsigned& bx, unsigned& by, unsigned& bz) {
+    const unsigned nb = g.nbx * g.nby * g.nbz;
+    const unsigned b = xcd_remap(blockIdx.x, nb);
+    bz = b % g.nbz;
+    bx = (b / g.nbz) % g.nbx;
+    by = b / (g.nbz * g.nbx);
+}
+
 // Closed-form stick + plate gather. Payloads are padded volumes of
 // (P0, P1, P2) voxels; receiver (z, y, x) reads voter (z+dw+r, y+dv+r, x+du+r).
 // gate (optional): the launch does nothing while *gate == 0; accumulate:
@@ -389,7 +399,7 @@ __global__ __launch_bounds__(kThreads) void k_vote3_closed(const float4* __restr
     constexpr int STRIDE = (4 * P + 13 + 3) & ~3;
     if (gate && *gate == 0ull) return;
     unsigned bx, by, bz;
-    tile_coords(g, BZ, bx, by, bz);
+    tile_coords_zfast(g, bx, by, bz);
     const int x = (int)bx * kTileX + (int)(threadIdx.x % kTileX);
     const int y = (int)by * kTileY + (int)(threadIdx.x / kTileX);
     const int z0 = (int)bz * BZ;
@@ -856,7 +866,7 @@ __global__ __launch_bounds__(kThreads) void k_vote3_mlptab(const float4* __restr
     constexpr int CELLS2 = 2 * kMlpTabM;  // float4 per row
     extern __shared__ float4 s_rows[];
     unsigned bx, by, bz;
-    tile_coords(g, BZ, bx, by, bz);
+    tile_coords_zfast(g, bx, by, bz);
     const int x = (int)bx * kTileX + (int)(threadIdx.x % kTileX);
     const int y = (int)by * kTileY + (int)(threadIdx.x / kTileX);
     const int z0 = (int)bz * BZ;
