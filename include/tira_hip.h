@@ -278,6 +278,10 @@ int tira_fft3_async(float* data, unsigned nchan, unsigned L0, unsigned L1, unsig
 /* FFT length the spectral vote uses for an axis of n voxels and window w
  * (smallest 7-smooth L >= n + w/2), 0 when above 1024. */
 unsigned tira_spectral_length(unsigned n, unsigned w);
+/* The calling thread's cache of kernel spectra: entries held (the last 8
+ * parameter sets at most, within 1/8 of device memory, least recently used
+ * evicted first), their bytes, and the spectra built so far. */
+int tira_spectral_cache_info(unsigned* entries, unsigned long long* bytes, unsigned long long* builds);
 
 /* ------------------------------------------------------------------ Gaussian filter */
 /* tira::cuda::gaussian_filter3d<float>(src, width, height, depth, sigma_w, sigma_h,

@@ -163,6 +163,42 @@ def test_spectral_repeat_and_cache(hip):
     np.testing.assert_array_equal(a, d)
 
 
+def _cache_info():
+    import ctypes as C
+    n, b, k = C.c_uint(), C.c_ulonglong(), C.c_ulonglong()
+    call("tira_spectral_cache_info", C.byref(n), C.byref(b), C.byref(k))
+    return n.value, b.value, k.value
+
+
+def test_spectral_cache_keeps_several_parameter_sets(hip):
+    """Alternating between parameter sets and shapes builds each set's kernel
+    spectra once (the cache keeps the last 8), with results bit-identical to
+    the first call of each set; a ninth set evicts the least recently used."""
+    shapes = [(16, 18, 20), (20, 16, 18)]
+    fields = {sh: _random_field(sh, seed=11) for sh in shapes}
+    sets = [(shapes[0], 1.5, 9), (shapes[0], 2.0, 9), (shapes[1], 1.5, 11)]
+    _, _, k0 = _cache_info()
+    first = {}
+    for rep in range(3):
+        for sh, sig, w in sets:
+            v = _vote(fields[sh], sh, sig, 0.0, w, _lib.VOTE_SPECTRAL)
+            if rep == 0:
+                first[(sh, sig, w)] = v
+            else:
+                np.testing.assert_array_equal(v, first[(sh, sig, w)])
+    n, nbytes, k1 = _cache_info()
+    assert k1 - k0 <= len(sets)  # built at most once each (earlier tests may have left one cached)
+    assert n >= len(sets) and nbytes > 0
+    # eight more parameter sets: the first three are evicted, the cache stays at 8
+    for i in range(8):
+        _vote(fields[shapes[0]], shapes[0], 1.0 + 0.1 * i, 0.0, 7, _lib.VOTE_SPECTRAL)
+    n2, _, k2 = _cache_info()
+    assert n2 == 8 and k2 - k1 == 8
+    v = _vote(fields[shapes[0]], shapes[0], 1.5, 0.0, 9, _lib.VOTE_SPECTRAL)
+    np.testing.assert_array_equal(v, first[(shapes[0], 1.5, 9)])
+    assert _cache_info()[2] == k2 + 1  # rebuilt after its eviction
+
+
 @pytest.mark.parametrize("power,samples", [(2, 20), (1, 2)])
 def test_spectral_falls_back_to_gather(hip, power, samples):
     shape = (10, 11, 12)

@@ -112,6 +112,7 @@ SIGNATURES = {
     "tira_tensorvote3_atv_method_async": [_vp, _i, _vp, _i, _u, _u, _u, _f, _u, _i],
     "tira_fft3_async": [_vp, _u, _u, _u, _u, _i],
     "tira_spectral_length": [_u, _u],
+    "tira_spectral_cache_info": [C.POINTER(_u), C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)],
     "tira_otsu_threshold_async": [_vp, _sz, _i, _vp],
     "tira_threshold_async": [_vp, _sz, _i, _vp, _vp],
     "tira_lee_thin3": [_vp, _vp, _u, _u, _u, _vp],

@@ -1238,16 +1238,60 @@ struct SpecKey {
     }
 };
 
-struct SpecCache {
+// Kernel spectra of the last few parameter sets of this host thread (a
+// caller alternating between shapes or scales rebuilds nothing), evicted
+// least recently used beyond kSpecEntries or 1/8 of the device's memory.
+struct SpecEntry {
     SpecKey key{0, 0, 0, 0, 0, 0, 0, 0, 0};
     int device = -1;
     void* buf = nullptr;  // twiddles (3 axes) then the octant spectra
     size_t bytes = 0;
+    unsigned long long used = 0;
+};
+constexpr size_t kSpecEntries = 8;
+struct SpecCache {
+    std::vector<SpecEntry> e;
+    unsigned long long clock = 0;
+    unsigned long long builds = 0;
     ~SpecCache() {
-        if (buf) (void)hipFree(buf);
+        int cur = 0;
+        if (hipGetDevice(&cur) != hipSuccess) return;
+        for (auto& x : e)
+            if (x.buf && hipSetDevice(x.device) == hipSuccess) (void)hipFree(x.buf);
+        (void)hipSetDevice(cur);
     }
 };
 static thread_local SpecCache g_spec;
+
+// Frees cache entry i (its device made current for the free).
+static void spec_evict(size_t i) {
+    SpecEntry& x = g_spec.e[i];
+    if (x.buf) {
+        int cur = 0;
+        (void)hipGetDevice(&cur);
+        if (x.device != cur) (void)hipSetDevice(x.device);
+        (void)hipFree(x.buf);
+        if (x.device != cur) (void)hipSetDevice(cur);
+    }
+    g_spec.e.erase(g_spec.e.begin() + (long)i);
+}
+
+static size_t spec_lru(int dev_only) {
+    size_t best = g_spec.e.size();
+    for (size_t i = 0; i < g_spec.e.size(); i++)
+        if ((dev_only < 0 || g_spec.e[i].device == dev_only) &&
+            (best == g_spec.e.size() || g_spec.e[i].used < g_spec.e[best].used))
+            best = i;
+    return best;
+}
+
+// Bytes the cache holds on device `dev`.
+static size_t spec_held(int dev) {
+    size_t b = 0;
+    for (const auto& x : g_spec.e)
+        if (x.device == dev) b += x.bytes;
+    return b;
+}
 
 static void twiddles(int L, float2* t) {
     for (int j = 0; j < L; j++) {
@@ -1258,42 +1302,62 @@ static void twiddles(int L, float2* t) {
 
 static size_t tw_bytes(int L) { return ((size_t)L * 8 + 255) & ~size_t(255); }
 
-// Kernel spectra for `key` (built and uploaded when the key changes).
+// Kernel spectra for `key` (built and uploaded unless cached).
 static int spectra_for(const SpecKey& key, const SpecGeom& g, hipStream_t st, const float2** tw0,
                        const float2** tw1, const float2** tw2, const float** kh) {
     int dev = 0;
     TH_HIP(hipGetDevice(&dev));
     const size_t tb = tw_bytes(key.L0) + tw_bytes(key.L1) + tw_bytes(key.L2);
     const size_t kbytes = (size_t)6 * g.nf * g.ocs * 4;
-    auto set_ptrs = [&]() {
-        char* b = (char*)g_spec.buf;
+    auto set_ptrs = [&](const SpecEntry& x) {
+        char* b = (char*)x.buf;
         *tw0 = (const float2*)b;
         *tw1 = (const float2*)(b + tw_bytes(key.L0));
         *tw2 = (const float2*)(b + tw_bytes(key.L0) + tw_bytes(key.L1));
         *kh = (const float*)(b + tb);
     };
-    if (g_spec.buf && g_spec.key == key && g_spec.device == dev) {
-        set_ptrs();
-        return TIRA_OK;
-    }
+    g_spec.clock++;
+    for (auto& x : g_spec.e)
+        if (x.buf && x.device == dev && x.key == key) {
+            x.used = g_spec.clock;
+            set_ptrs(x);
+            return TIRA_OK;
+        }
+    // evicted spectra may still be read by work queued on the stream
     TH_HIP(hipStreamSynchronize(st));
-    if (g_spec.buf) {
-        int cur = dev;
-        if (g_spec.device != dev) TH_HIP(hipSetDevice(g_spec.device));
-        (void)hipFree(g_spec.buf);
-        if (g_spec.device != dev) TH_HIP(hipSetDevice(cur));
-        g_spec.buf = nullptr;
+    size_t total_b = 0, free_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) { (void)hipGetLastError(); total_b = 0; }
+    const size_t need = tb + kbytes, budget = total_b / 8;
+    while (!g_spec.e.empty() &&
+           (g_spec.e.size() >= kSpecEntries || (spec_held(dev) > 0 && spec_held(dev) + need > budget))) {
+        size_t i = spec_lru(dev);
+        if (i == g_spec.e.size()) i = spec_lru(-1);
+        spec_evict(i);
     }
-    if (hipMalloc(&g_spec.buf, tb + kbytes) != hipSuccess) {
-        g_spec.buf = nullptr;
+    void* buf = nullptr;
+    while (hipMalloc(&buf, need) != hipSuccess) {
         (void)hipGetLastError();
-        set_error("spectral vote: out of device memory for the kernel spectra (%zu bytes)", tb + kbytes);
-        return TIRA_ERR_OUT_OF_MEMORY;
+        buf = nullptr;
+        if (g_spec.e.empty()) {
+            set_error("spectral vote: out of device memory for the kernel spectra (%zu bytes)", need);
+            return TIRA_ERR_OUT_OF_MEMORY;
+        }
+        spec_evict(spec_lru(-1));  // make room: the older spectra go first
     }
-    g_spec.bytes = tb + kbytes;
-    g_spec.device = dev;
-    g_spec.key = SpecKey{0, 0, 0, 0, 0, 0, 0, 0, 0};
-    set_ptrs();
+    g_spec.e.push_back(SpecEntry{});
+    SpecEntry& ent = g_spec.e.back();
+    ent.buf = buf;
+    ent.bytes = need;
+    ent.device = dev;
+    ent.used = g_spec.clock;
+    set_ptrs(ent);
+    // an entry whose build fails is dropped again
+    struct Drop {
+        bool armed = true;
+        ~Drop() {
+            if (armed) spec_evict(g_spec.e.size() - 1);
+        }
+    } drop;
 
     // twiddles
     {
@@ -1301,7 +1365,7 @@ static int spectra_for(const SpecKey& key, const SpecGeom& g, hipStream_t st, co
         twiddles(key.L0, t.data());
         twiddles(key.L1, t.data() + tw_bytes(key.L0) / 8);
         twiddles(key.L2, t.data() + (tw_bytes(key.L0) + tw_bytes(key.L1)) / 8);
-        TH_HIP(hipMemcpy(g_spec.buf, t.data(), tb, hipMemcpyHostToDevice));
+        TH_HIP(hipMemcpy(ent.buf, t.data(), tb, hipMemcpyHostToDevice));
     }
     // offset kernels, parities and trig tables
     std::vector<double> K;
@@ -1376,7 +1440,9 @@ static int spectra_for(const SpecKey& key, const SpecGeom& g, hipStream_t st, co
         set_error("spectral vote: building the kernel spectra failed");
         return TIRA_ERR_HIP;
     }
-    g_spec.key = key;
+    g_spec.e.back().key = key;
+    g_spec.builds++;
+    drop.armed = false;
     return TIRA_OK;
 }
 
@@ -1450,7 +1516,7 @@ bool spectral_preferred(unsigned s0, unsigned s1, unsigned s2, unsigned w, unsig
     }
     // memory this thread's spectral vote already holds is reused, not allocated again
     size_t held = ws_capacity(WS_SPEC0) + ws_capacity(WS_PAY0) + ws_capacity(WS_PAY1);
-    if (g_spec.buf && g_spec.device == dev) held += g_spec.bytes;
+    held += spec_held(dev);
     return need <= total_b / 10 * 8 && need <= (free_b + held) / 10 * 9;
 }
 
@@ -1638,6 +1704,15 @@ int tira_spectral_offset_kernels_atv(unsigned w, float sigma, double* out, unsig
 }
 
 unsigned tira_spectral_length(unsigned n, unsigned w) { return spectral_length(n, w); }
+
+int tira_spectral_cache_info(unsigned* entries, unsigned long long* bytes, unsigned long long* builds) {
+    unsigned long long b = 0;
+    for (const auto& x : g_spec.e) b += x.bytes;
+    if (entries) *entries = (unsigned)g_spec.e.size();
+    if (bytes) *bytes = b;
+    if (builds) *builds = g_spec.builds;
+    return TIRA_OK;
+}
 
 int tira_fft3_async(float* data, unsigned nchan, unsigned L0, unsigned L1, unsigned L2, int inverse) {
     TH_ARG(data, "NULL pointer");
