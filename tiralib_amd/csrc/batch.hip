@@ -50,11 +50,11 @@ __device__ __forceinline__ unsigned batch_part_begin(unsigned total, unsigned q)
 }
 
 template <int R>
-__global__ __launch_bounds__(512) void k_batch_gauss_eig(const BatchVolume* __restrict__ vols, int nvol,
+__global__ __launch_bounds__(512, R <= 5 ? 6 : 1) void k_batch_gauss_eig(const BatchVolume* __restrict__ vols, int nvol,
                                                          unsigned total, unsigned* __restrict__ counter,
                                                          const float* __restrict__ taps) {
     constexpr int PW = kGT + 2 * R;
-    constexpr int RS = (PW + 3 + 3) & ~3;
+    constexpr int RS = gauss_row_stride(PW);
     __shared__ __attribute__((aligned(16))) float s_raw[6 * PW * RS];
     __shared__ __attribute__((aligned(16))) float s_xf[6 * PW * kGT];
     __shared__ unsigned s_item;

@@ -9,6 +9,10 @@ namespace th {
 
 constexpr int kGT = 16;  // tile edge (x and y)
 
+// LDS row of a staged patch of edge pw: room for the x-pass's 16-byte reads
+// past the edge, rounded up to 16 floats past a multiple of 32.
+__host__ __device__ constexpr int gauss_row_stride(int pw) { return ((pw + 6 + 15) / 32) * 32 + 16; }
+
 struct Gauss3Geom {
     int s0, s1, s2;     // input shape (z, y, x)
     int o0, o1, o2;     // output shape
@@ -38,12 +42,18 @@ __device__ __forceinline__ void tile_eig3(const float* s_raw, int pos, int cs, f
 
 // taps: 3 x (2R+1) floats (x, y, z), each centred and zero-padded.
 //
-// LDS: the staged plane is component-major with rows padded to a multiple of
-// 4 floats, so the x-pass reads 16-byte vectors and slides a register window
-// over 4 outputs at a time; the x-filtered plane is component-major too.
-// Every thread's staged elements sit at the same in-plane offsets for every
-// z-plane, so their global offsets and LDS slots are computed once, and the
-// next plane is prefetched into registers while the current one is filtered.
+// LDS: the staged plane is component-major, so the x-pass reads 16-byte
+// vectors and slides a register window over 4 outputs at a time; the
+// x-filtered plane is component-major too. Rows are padded to 16 floats past
+// a multiple of 32, so the 16-byte window reads of neighbouring rows fall on
+// disjoint banks. Each thread stages whole voxels (all components, read with
+// 8-byte loads for tensors) and writes one component at a time, so the lanes
+// of a wave write consecutive x of one component: no bank conflicts (one lane
+// per element of a component-interleaved voxel put the six components of a
+// voxel on one bank). A thread's staged voxels sit at the same in-plane
+// offsets for every z-plane, so their global offsets and LDS slots are
+// computed once, and the next plane is prefetched into registers while the
+// current one is filtered.
 // 6-component fields use 512 threads: the two 256-thread halves split the
 // components (3 each) for the y- and z-passes, which keeps the register ring
 // at (2R+1) x 3 floats per thread.
@@ -57,9 +67,9 @@ __device__ __forceinline__ void gauss3_tile(const float* __restrict__ in, float*
     constexpr int NT = GaussThreads<C>::N;
     constexpr int CP = C / GaussThreads<C>::CG;         // components per thread in the y/z passes
     constexpr int PW = kGT + 2 * R;                     // staged patch edge
-    constexpr int RS = (PW + 3 + 3) & ~3;               // padded LDS row (room for 4-wide reads past the edge)
-    constexpr int NRAW = PW * PW * C;                   // staged floats per plane
-    constexpr int NS = (NRAW + NT - 1) / NT;            // staged floats per thread
+    constexpr int RS = gauss_row_stride(PW);            // padded LDS row
+    constexpr int NV = PW * PW;                         // staged voxels per plane
+    constexpr int NS = (NV + NT - 1) / NT;              // staged voxels per thread
     constexpr int NSEG = kGT / 4;                       // 4-output segments per x-filtered row
     constexpr int NXT = C * PW * NSEG;                  // x-pass tasks per plane
     constexpr int NX = (NXT + NT - 1) / NT;
@@ -78,30 +88,44 @@ __device__ __forceinline__ void gauss3_tile(const float* __restrict__ in, float*
     const int ix0 = g.c2 + ox0 - R, iy0 = g.c1 + oy0 - R, iz0 = g.c0 + oz0 - R;
     const size_t plane = (size_t)g.s1 * g.s2 * C;
 
-    // staged elements of this thread: global element order [yy][xx][c]
-    // (coalesced), LDS slot [c][yy][xx]; goff < 0 marks voxels outside the volume
+    // staged voxels of this thread: voxel (yy, xx) of the patch, LDS slot
+    // [c][yy][xx] = lslot + c * PW * RS; goff < 0 marks voxels outside the volume
     int goff[NS], lslot[NS];
 #pragma unroll
     for (int s = 0; s < NS; s++) {
-        const int e = tid + NT * s;
+        const int v = tid + NT * s;
         goff[s] = -1;
         lslot[s] = -1;
-        if (e < NRAW) {
-            const int c = e % C, v = e / C;
+        if (v < NV) {
             const int xx = v % PW, yy = v / PW;
             const int x = ix0 + xx, y = iy0 + yy;
-            lslot[s] = (c * PW + yy) * RS + xx;
-            if (x >= 0 && x < g.s2 && y >= 0 && y < g.s1) goff[s] = (y * g.s2 + x) * C + c;
+            lslot[s] = yy * RS + xx;
+            if (x >= 0 && x < g.s2 && y >= 0 && y < g.s1) goff[s] = (y * g.s2 + x) * C;
         }
     }
 
-    float pre[NS];
+    float pre[NS][C];
     auto prefetch = [&](int i) {
         const int iz = iz0 + i;
         const bool zin = i < nplanes && iz >= 0 && iz < g.s0;
         const float* pl = in + (zin ? (size_t)iz * plane : 0);
 #pragma unroll
-        for (int s = 0; s < NS; s++) pre[s] = (zin && goff[s] >= 0) ? pl[goff[s]] : 0.0f;
+        for (int s = 0; s < NS; s++) {
+            const bool ok = zin && goff[s] >= 0;
+            if (C % 2 == 0) {
+                // a voxel's components are 8-byte aligned (the field is)
+                const float2* v2 = reinterpret_cast<const float2*>(pl + (ok ? goff[s] : 0));
+#pragma unroll
+                for (int c = 0; c < C; c += 2) {
+                    const float2 q = ok ? v2[c / 2] : make_float2(0.0f, 0.0f);
+                    pre[s][c] = q.x;
+                    pre[s][c + 1] = q.y;
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < C; c++) pre[s][c] = ok ? pl[goff[s] + c] : 0.0f;
+            }
+        }
     };
 
     float acc[K][CP];
@@ -121,7 +145,10 @@ __device__ __forceinline__ void gauss3_tile(const float* __restrict__ in, float*
             if (i < nplanes) {  // uniform
 #pragma unroll
                 for (int s = 0; s < NS; s++)
-                    if (lslot[s] >= 0) s_raw[lslot[s]] = pre[s];
+                    if (lslot[s] >= 0) {
+#pragma unroll
+                        for (int c = 0; c < C; c++) s_raw[lslot[s] + c * PW * RS] = pre[s][c];
+                    }
                 __syncthreads();
                 prefetch(i + 1);  // in flight while this plane is filtered
                 // x-pass: task = (component, row, 4 consecutive outputs)
@@ -192,11 +219,13 @@ __device__ __forceinline__ void gauss3_tile(const float* __restrict__ in, float*
     }
 }
 
+// Tensor fields up to R = 5 fit 80 VGPRs: six waves per SIMD, three blocks
+// per CU (4% faster at R = 5); R = 6 would spill at that bound (30% slower).
 template <int R, int C>
-__global__ __launch_bounds__(GaussThreads<C>::N) void k_gauss3(const float* __restrict__ in, float* __restrict__ out,
+__global__ __launch_bounds__(GaussThreads<C>::N, (C == 6 && R <= 5) ? 6 : 1) void k_gauss3(const float* __restrict__ in, float* __restrict__ out,
                                                                const float* __restrict__ taps, Gauss3Geom g) {
     constexpr int PW = kGT + 2 * R;
-    constexpr int RS = (PW + 3 + 3) & ~3;
+    constexpr int RS = gauss_row_stride(PW);
     __shared__ __attribute__((aligned(16))) float s_raw[C * PW * RS];
     __shared__ __attribute__((aligned(16))) float s_xf[C * PW * kGT];
     // XCD-major tile order: neighbouring tiles (x fastest) run on one XCD at
