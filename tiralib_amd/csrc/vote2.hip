@@ -334,12 +334,17 @@ __global__ __launch_bounds__(256) void k_vote2_atv_prep_eig(const float2* __rest
     pay[i] = make_float4(cs, sn, l.y - l.x, l.x);
 }
 
+// The table is split over blockIdx.z as in k_vote2: chunk z covers entries
+// [z per, (z + 1) per) and, with several chunks, writes its partial sums
+// (xx, 2 xy, yy, isotropic) to part[z * n + pixel] for k_vote2_atv_reduce.
 __global__ __launch_bounds__(256) void k_vote2_atv(const float4* __restrict__ pay, const Atv2* __restrict__ tab,
-                                                   int nt, int s0, int s1, float* __restrict__ out) {
+                                                   int nt, int per, int s0, int s1, float* __restrict__ out,
+                                                   float4* __restrict__ part) {
     const int x1 = blockIdx.x * 64 + (threadIdx.x & 63);
     const int x0 = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int k0 = (int)blockIdx.z * per, k1 = min(nt, k0 + per);
     float xx = 0.0f, xy2 = 0.0f, yy = 0.0f, ii = 0.0f;
-    for (int k = 0; k < nt; k++) {
+    for (int k = k0; k < k1; k++) {
         const Atv2 e = tab[k];
         const int r0 = x0 + e.dv, r1 = x1 + e.du;
         if ((unsigned)r0 >= (unsigned)s0 || (unsigned)r1 >= (unsigned)s1) continue;
@@ -356,9 +361,27 @@ __global__ __launch_bounds__(256) void k_vote2_atv(const float4* __restrict__ pa
         ii = fmaf(P.w, e.ck4, ii);
     }
     if (x0 < s0 && x1 < s1) {
-        const float b = 0.5f * xy2;
-        reinterpret_cast<float4*>(out)[(size_t)x0 * s1 + x1] = make_float4(xx + ii, b, b, yy + ii);
+        const size_t p = (size_t)x0 * s1 + x1;
+        if (part) {
+            part[(size_t)blockIdx.z * ((size_t)s0 * s1) + p] = make_float4(xx, xy2, yy, ii);
+        } else {
+            const float b = 0.5f * xy2;
+            reinterpret_cast<float4*>(out)[p] = make_float4(xx + ii, b, b, yy + ii);
+        }
     }
+}
+
+__global__ __launch_bounds__(256) void k_vote2_atv_reduce(const float4* __restrict__ part, int chunks, size_t n,
+                                                          float* __restrict__ out) {
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    float4 v = part[p];
+    for (int z = 1; z < chunks; z++) {
+        const float4 q = part[(size_t)z * n + p];
+        v.x += q.x; v.y += q.y; v.z += q.z; v.w += q.w;
+    }
+    const float b = 0.5f * v.y;
+    reinterpret_cast<float4*>(out)[p] = make_float4(v.x + v.w, b, b, v.z + v.w);
 }
 
 static int vote2_atv_run(const float* tin, const float* lam, const float* th, float* out, float sigma, unsigned w,
@@ -378,10 +401,14 @@ static int vote2_atv_run(const float* tin, const float* lam, const float* th, fl
     Atv2* dtab = nullptr;
     static thread_local TableCache atv_cache;
     TH_TRY(table_to_device(WS_TABLE2, tab.data(), tab.size() * sizeof(Atv2), atv_cache, st, (void**)&dtab));
-    dim3 grid(ceil_div(s1, 64), ceil_div(s0, 4));
+    const int nt = (int)tab.size(), chunks = vote2_chunks(s0, s1, nt);
+    float4* part = nullptr;
+    if (chunks > 1) TH_TRY(ws_get(WS_PAY1, (size_t)chunks * n * 16, (void**)&part));
+    dim3 grid(ceil_div(s1, 64), ceil_div(s0, 4), chunks);
     {
         ProfScope prof("vote2_atv", st);
-        k_vote2_atv<<<grid, 256, 0, st>>>(pay, dtab, (int)tab.size(), (int)s0, (int)s1, out);
+        k_vote2_atv<<<grid, 256, 0, st>>>(pay, dtab, nt, vote2_per(nt, chunks), (int)s0, (int)s1, out, part);
+        if (chunks > 1) k_vote2_atv_reduce<<<ceil_div(n, 256), 256, 0, st>>>(part, chunks, n, out);
     }
     TH_HIP(hipGetLastError());
     return TIRA_OK;
