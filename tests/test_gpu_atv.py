@@ -43,7 +43,7 @@ def _check3(gpu, ref, ref64=None, tol=TOL_VOTE):
     assert e <= tol, msg
 
 
-@pytest.mark.parametrize("H", [16, 32, 64])
+@pytest.mark.parametrize("H", [16, 32, 64, 128])
 def test_atv2_mixed_field_module(hip, oracle, H):
     """tensorvote_atv.tensorvote_atv(field, sigma=5) as cuda_wrappers.atv_vote2_cuda calls it."""
     f = _mixed_field(H, H)

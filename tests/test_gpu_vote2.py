@@ -226,3 +226,38 @@ def test_build_neighbors2d_device_equals_host(hip):
                 assert diff.max() <= 1, (k, s1, s2, p)
                 ulps += int((diff > 0).sum())
     print(f"c1/c2 entries one ulp apart: {ulps}")
+
+
+def _rng42_fields(H, W):
+    """The two fields of the reference's CUDA-vs-Python script
+    (test_cuda_vs_python.py:48-85): mixed stick/ball and pure stick,
+    default_rng(42), orientations and eigenvalues uniform."""
+    out = {}
+    for kind in ("mixed", "stick"):
+        rng = np.random.default_rng(42)
+        ang = rng.uniform(0, np.pi, (H, W))
+        l1 = rng.uniform(0.5, 1.0, (H, W))
+        l0 = rng.uniform(0.0, 0.5, (H, W)) if kind == "mixed" else np.zeros((H, W))
+        vx, vy = np.cos(ang), np.sin(ang)
+        T = np.zeros((H, W, 4))
+        T[..., 0] = l0 + (l1 - l0) * vx * vx
+        T[..., 1] = T[..., 2] = (l1 - l0) * vx * vy
+        T[..., 3] = l0 + (l1 - l0) * vy * vy
+        out[kind] = T.astype(np.float32)
+    return out
+
+
+@pytest.mark.parametrize("H", [16, 32, 64, 128])
+def test_vote2_reference_script_sweep(hip, oracle, H):
+    """Sections 2 and 3 of test_cuda_vs_python.py (sizes 16..128, sigma1 5,
+    sigma2 2, p 1, N 20; stick-only on the pure-stick field, stick + plate on
+    the mixed one) through the pybind-module mirror, against the oracle at
+    1e-4 relative Frobenius (the script's own bar is 1%)."""
+    f = _rng42_fields(H, H)
+    w = 2 * int(3 * 5.0) + 1
+    for kind, plate in (("stick", False), ("mixed", True)):
+        gpu = tvk.tensorvote_tk(f[kind], 5.0, 2.0, 1, 0, True, plate, 20)
+        ref = oracle.tensorvote2(f[kind], 5.0, 2.0, 1, w, True, plate, 20, nthreads=NT)
+        e = rel_frob(gpu, ref)
+        print(f"{kind} {H}x{H}: rel Frobenius {e:.2e}")
+        assert e <= 1e-4
