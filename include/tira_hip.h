@@ -311,14 +311,21 @@ int tira_gaussian_filter3_tensor_async(const float* in, float* out, unsigned s0,
  * in[i] (the same odd kernel of ntaps taps, 7..13, on all three axes) and
  * evals[i] = ascending eigenvalues of every input tensor. shapes: nvol x 3
  * (s0, s1, s2), all pointers device memory. autotune != 0 times the candidate
- * z-chunks / occupancies on this batch once per process and batch shape. The
- * launch is captured in a hipGraph at creation; run() replays it on the
- * library stream (non-blocking). Buffers must stay valid until destroy(). */
+ * tile shapes (16 x 8, 16 x 16, 16 x 32 outputs), z-chunks (the tile's depth,
+ * hence its z-halo share) and occupancies on this batch once per process and
+ * batch shape; 0 takes 16 x 16 x 32 at two blocks per CU. The launch is
+ * captured in a hipGraph at creation; run() replays it on the library stream
+ * (non-blocking). Buffers must stay valid until destroy(). */
 int tira_batch_gauss_eig3_create(void** handle, int nvol, const float* const* in, float* const* out,
                                  float* const* evals, const unsigned* shapes, const float* taps, unsigned ntaps,
                                  int autotune);
 int tira_batch_gauss_eig3_run(void* handle);
 int tira_batch_gauss_eig3_plan(void* handle, int* zchunk, int* blocks_per_cu, float* tuned_ms, unsigned* items);
+/* The plan's tile shape: outputs per tile along x and y. */
+int tira_batch_gauss_eig3_tile(void* handle, int* tile_x, int* tile_y);
+/* Replaces the plan (tile height 8, 16 or 32; z-chunk; blocks per CU, at
+ * most 2 for 32-row tiles) and captures the launch again. Blocking. */
+int tira_batch_gauss_eig3_set_plan(void* handle, int tile_y, int zchunk, int blocks_per_cu);
 int tira_batch_gauss_eig3_destroy(void* handle);
 
 /* ------------------------------------------------------------------ synthetic inputs */

@@ -129,6 +129,8 @@ SIGNATURES = {
     "tira_batch_gauss_eig3_create": [C.POINTER(_vp), _i, _vp, _vp, _vp, _vp, _vp, _u, _i],
     "tira_batch_gauss_eig3_run": [_vp],
     "tira_batch_gauss_eig3_plan": [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_f), C.POINTER(_u)],
+    "tira_batch_gauss_eig3_tile": [_vp, C.POINTER(_i), C.POINTER(_i)],
+    "tira_batch_gauss_eig3_set_plan": [_vp, _i, _i, _i],
     "tira_batch_gauss_eig3_destroy": [_vp],
     "tira_synth_eig_tensors_async": [_vp, _sz, C.c_ulonglong, _f, _f, _f, _f, _f, _vp],
     "tira_synth_sheets_async": [_vp, _u, _u, _u, _i, _vp, _f, _f, C.c_ulonglong],

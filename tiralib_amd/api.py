@@ -357,7 +357,14 @@ class BatchGaussEig3:
     def plan(self):
         zc, occ, ms, items = C.c_int(), C.c_int(), C.c_float(), C.c_uint()
         call("tira_batch_gauss_eig3_plan", self.handle, C.byref(zc), C.byref(occ), C.byref(ms), C.byref(items))
-        return {"zchunk": zc.value, "blocks_per_cu": occ.value, "tuned_ms": ms.value, "items": items.value}
+        tx, ty = C.c_int(), C.c_int()
+        call("tira_batch_gauss_eig3_tile", self.handle, C.byref(tx), C.byref(ty))
+        return {"tile": [tx.value, ty.value], "zchunk": zc.value, "blocks_per_cu": occ.value, "tuned_ms": ms.value,
+                "items": items.value}
+
+    def set_plan(self, tile_y, zchunk, blocks_per_cu):
+        """Fix the launch plan (tile height 8/16/32, z-chunk, blocks per CU) and re-capture."""
+        call("tira_batch_gauss_eig3_set_plan", self.handle, int(tile_y), int(zchunk), int(blocks_per_cu))
 
     def close(self):
         if self.handle:

@@ -49,14 +49,21 @@ __device__ __forceinline__ unsigned batch_part_begin(unsigned total, unsigned q)
     return (unsigned)(((unsigned long long)total * q) / kBatchParts);
 }
 
-template <int R>
-__global__ __launch_bounds__(512, R <= 5 ? 6 : 1) void k_batch_gauss_eig(const BatchVolume* __restrict__ vols, int nvol,
-                                                         unsigned total, unsigned* __restrict__ counter,
-                                                         const float* __restrict__ taps) {
+// Waves per SIMD the batch kernel is bounded to: six for 16-row tiles up to
+// R = 5 (80 VGPRs, no spill), none otherwise.
+template <int R, int TY> struct BatchBound { static constexpr int W = (TY == 16 && R <= 5) ? 6 : 1; };
+
+// TY: tile height (8, 16 or 32 rows of 16 outputs); LDS
+// gauss_tile_lds_floats<R, 6, TY>() floats, sized at launch.
+template <int R, int TY>
+__global__ __launch_bounds__((GaussThreads<6, TY>::N), (BatchBound<R, TY>::W)) void k_batch_gauss_eig(
+    const BatchVolume* __restrict__ vols, int nvol, unsigned total, unsigned* __restrict__ counter,
+    const float* __restrict__ taps) {
     constexpr int PW = kGT + 2 * R;
-    constexpr int RS = gauss_row_stride(PW);
-    __shared__ __attribute__((aligned(16))) float s_raw[6 * PW * RS];
-    __shared__ __attribute__((aligned(16))) float s_xf[6 * PW * kGT];
+    constexpr int PH = TY + 2 * R;
+    extern __shared__ __attribute__((aligned(16))) float s_tile[];
+    float* s_raw = s_tile;
+    float* s_xf = s_tile + 6 * PH * gauss_row_stride(PW);
     __shared__ unsigned s_item;
     const unsigned home = blockIdx.x % kBatchParts;
     unsigned spent = 0;  // parts this block found used up (thread 0 only)
@@ -83,7 +90,7 @@ __global__ __launch_bounds__(512, R <= 5 ? 6 : 1) void k_batch_gauss_eig(const B
         int v = 0;
         while (v + 1 < nvol && vols[v + 1].first_item <= item) v++;
         const BatchVolume& bv = vols[v];
-        gauss3_tile<R, 6, true>(bv.in, bv.out, taps, bv.g, item - bv.first_item, bv.evals, s_raw, s_xf);
+        gauss3_tile<R, 6, true, TY>(bv.in, bv.out, taps, bv.g, item - bv.first_item, bv.evals, s_raw, s_xf);
     }
 }
 
@@ -94,6 +101,7 @@ __global__ void k_reset_counter(unsigned* c) {
 struct BatchPlan {
     int zchunk = 32;
     int blocks_per_cu = 2;
+    int tile_y = 16;  // tile height (x width kGT)
     float tuned_ms = 0.0f;
 };
 
@@ -125,7 +133,7 @@ static std::string plan_key(const Batch& b) {
 }
 
 // Builds the item table for a z-chunk (largest volumes first) and uploads it.
-static int build_items(Batch& b, int zchunk, hipStream_t st) {
+static int build_items(Batch& b, int zchunk, int tile_y, hipStream_t st) {
     std::vector<int> order(b.nvol);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
@@ -146,7 +154,7 @@ static int build_items(Batch& b, int zchunk, hipStream_t st) {
         g.c0 = g.c1 = g.c2 = 0;
         g.zchunk = zchunk;
         g.nbx = ceil_div(g.o2, kGT);
-        g.nby = ceil_div(g.o1, kGT);
+        g.nby = ceil_div(g.o1, tile_y);
         g.nbz = ceil_div(g.o0, zchunk);
         v.first_item = first;
         v.nitems = g.nbx * g.nby * g.nbz;
@@ -158,14 +166,48 @@ static int build_items(Batch& b, int zchunk, hipStream_t st) {
     return TIRA_OK;
 }
 
-static int launch_batch(Batch& b, hipStream_t st) {
+// Tiles above 64 KB of LDS (32 rows at R = 6) need the kernel's dynamic-LDS
+// limit raised, once per process, before any launch (and before a capture).
+template <int R, int TY>
+static int batch_lds_attr() {
+    const size_t lds = (size_t)gauss_tile_lds_floats<R, 6, TY>() * 4;
+    static bool set = false;
+    if (lds > 65536 && !set) {
+        TH_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_batch_gauss_eig<R, TY>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        set = true;
+    }
+    return TIRA_OK;
+}
+
+template <int R, int TY>
+static int launch_batch_rt(Batch& b, unsigned grid, hipStream_t st, bool prepare_only) {
+    TH_TRY((batch_lds_attr<R, TY>()));
+    if (prepare_only) return TIRA_OK;
+    const size_t lds = (size_t)gauss_tile_lds_floats<R, 6, TY>() * 4;
+    k_batch_gauss_eig<R, TY><<<grid, GaussThreads<6, TY>::N, lds, st>>>(b.d_vols, b.nvol, b.total, b.d_counter,
+                                                                       b.d_taps);
+    return TIRA_OK;
+}
+
+template <int R>
+static int launch_batch_r(Batch& b, unsigned grid, hipStream_t st, bool prepare_only) {
+    switch (b.plan.tile_y) {
+        case 8: return launch_batch_rt<R, 8>(b, grid, st, prepare_only);
+        case 32: return launch_batch_rt<R, 32>(b, grid, st, prepare_only);
+        default: return launch_batch_rt<R, 16>(b, grid, st, prepare_only);
+    }
+}
+
+// prepare_only: set the kernel attributes the plan needs and launch nothing.
+static int launch_batch(Batch& b, hipStream_t st, bool prepare_only = false) {
     const unsigned grid = (unsigned)(b.ncu * b.plan.blocks_per_cu);
-    k_reset_counter<<<1, 64, 0, st>>>(b.d_counter);
+    if (!prepare_only) k_reset_counter<<<1, 64, 0, st>>>(b.d_counter);
     switch (b.R) {
-        case 3: k_batch_gauss_eig<3><<<grid, 512, 0, st>>>(b.d_vols, b.nvol, b.total, b.d_counter, b.d_taps); break;
-        case 4: k_batch_gauss_eig<4><<<grid, 512, 0, st>>>(b.d_vols, b.nvol, b.total, b.d_counter, b.d_taps); break;
-        case 5: k_batch_gauss_eig<5><<<grid, 512, 0, st>>>(b.d_vols, b.nvol, b.total, b.d_counter, b.d_taps); break;
-        case 6: k_batch_gauss_eig<6><<<grid, 512, 0, st>>>(b.d_vols, b.nvol, b.total, b.d_counter, b.d_taps); break;
+        case 3: TH_TRY(launch_batch_r<3>(b, grid, st, prepare_only)); break;
+        case 4: TH_TRY(launch_batch_r<4>(b, grid, st, prepare_only)); break;
+        case 5: TH_TRY(launch_batch_r<5>(b, grid, st, prepare_only)); break;
+        case 6: TH_TRY(launch_batch_r<6>(b, grid, st, prepare_only)); break;
         default: set_error("batched filter radius %d outside 3..6", b.R); return TIRA_ERR_INVALID_ARG;
     }
     TH_HIP(hipGetLastError());
@@ -179,7 +221,7 @@ static int autotune(Batch& b, hipStream_t st) {
         auto it = g_plan_cache.find(key);
         if (it != g_plan_cache.end()) {
             b.plan = it->second;
-            return build_items(b, b.plan.zchunk, st);
+            return build_items(b, b.plan.zchunk, b.plan.tile_y, st);
         }
     }
     hipEvent_t e0, e1;
@@ -187,11 +229,18 @@ static int autotune(Batch& b, hipStream_t st) {
     TH_HIP(hipEventCreate(&e1));
     BatchPlan best;
     best.tuned_ms = 1e30f;
+    // tile height x z-chunk (the tile's shape and halo share) x resident
+    // blocks (256-, 512- and 1024-thread blocks for 8-, 16- and 32-row tiles)
+    const int tys[] = {8, 16, 32};
     const int zcs[] = {8, 16, 32, 64};
-    const int occ[] = {1, 2, 3, 4};
+    const int occ_by_ty[3][4] = {{2, 4, 6, 8}, {1, 2, 3, 4}, {1, 2, 0, 0}};
+    for (int ti = 0; ti < 3; ti++)
     for (int zc : zcs) {
-        TH_TRY(build_items(b, zc, st));
-        for (int o : occ) {
+        const int ty = tys[ti];
+        TH_TRY(build_items(b, zc, ty, st));
+        for (int o : occ_by_ty[ti]) {
+            if (o == 0) continue;
+            b.plan.tile_y = ty;
             b.plan.zchunk = zc;
             b.plan.blocks_per_cu = o;
             TH_TRY(launch_batch(b, st));  // warm
@@ -207,7 +256,7 @@ static int autotune(Batch& b, hipStream_t st) {
     TH_HIP(hipEventDestroy(e0));
     TH_HIP(hipEventDestroy(e1));
     b.plan = best;
-    TH_TRY(build_items(b, best.zchunk, st));
+    TH_TRY(build_items(b, best.zchunk, best.tile_y, st));
     std::lock_guard<std::mutex> lk(g_plan_mu);
     g_plan_cache[key] = best;
     return TIRA_OK;
@@ -216,6 +265,7 @@ static int autotune(Batch& b, hipStream_t st) {
 static int capture_graph(Batch& b) {
     if (b.exec) { TH_HIP(hipGraphExecDestroy(b.exec)); b.exec = nullptr; }
     if (b.graph) { TH_HIP(hipGraphDestroy(b.graph)); b.graph = nullptr; }
+    TH_TRY(launch_batch(b, b.cap_stream, true));  // kernel attributes outside the capture
     TH_HIP(hipStreamBeginCapture(b.cap_stream, hipStreamCaptureModeThreadLocal));
     const int st = launch_batch(b, b.cap_stream);
     hipGraph_t g = nullptr;
@@ -279,7 +329,7 @@ int tira_batch_gauss_eig3_create(void** handle, int nvol, const float* const* in
         if (hipMemcpy(b->d_taps, t.data(), t.size() * 4, hipMemcpyHostToDevice) != hipSuccess) { st_ = TIRA_ERR_HIP; break; }
         if (hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking) != hipSuccess) { st_ = TIRA_ERR_HIP; break; }
         if (autotune_plan) st_ = autotune(*b, b->cap_stream);
-        else st_ = build_items(*b, b->plan.zchunk, b->cap_stream);
+        else st_ = build_items(*b, b->plan.zchunk, b->plan.tile_y, b->cap_stream);
         if (st_ != TIRA_OK) break;
         st_ = capture_graph(*b);
     } while (0);
@@ -293,6 +343,30 @@ int tira_batch_gauss_eig3_run(void* handle) {
     Batch* b = (Batch*)handle;
     ProfScope prof("batch_gauss_eig3", stream());
     TH_HIP(hipGraphLaunch(b->exec, stream()));
+    return TIRA_OK;
+}
+
+int tira_batch_gauss_eig3_set_plan(void* handle, int tile_y, int zchunk, int blocks_per_cu) {
+    TH_ARG(handle, "NULL handle");
+    TH_ARG(tile_y == 8 || tile_y == 16 || tile_y == 32, "tile height must be 8, 16 or 32 (got %d)", tile_y);
+    TH_ARG(zchunk >= 1 && zchunk <= 4096, "z-chunk must be 1..4096 (got %d)", zchunk);
+    TH_ARG(blocks_per_cu >= 1 && blocks_per_cu <= (tile_y == 32 ? 2 : 8), "blocks per CU out of range (got %d)",
+           blocks_per_cu);
+    Batch* b = (Batch*)handle;
+    TH_HIP(hipStreamSynchronize(stream()));  // a replay on the library stream may still read the items
+    b->plan.tile_y = tile_y;
+    b->plan.zchunk = zchunk;
+    b->plan.blocks_per_cu = blocks_per_cu;
+    b->plan.tuned_ms = 0.0f;
+    TH_TRY(build_items(*b, zchunk, tile_y, b->cap_stream));
+    return capture_graph(*b);
+}
+
+int tira_batch_gauss_eig3_tile(void* handle, int* tile_x, int* tile_y) {
+    TH_ARG(handle, "NULL handle");
+    const Batch* b = (const Batch*)handle;
+    if (tile_x) *tile_x = kGT;
+    if (tile_y) *tile_y = b->plan.tile_y;
     return TIRA_OK;
 }
 

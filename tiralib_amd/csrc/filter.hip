@@ -73,20 +73,22 @@ template <int C>
 static int launch_gauss3_R(int R, const float* in, float* out, const float* taps, const Gauss3Geom& g,
                            hipStream_t st) {
     const unsigned grid = g.nbx * g.nby * g.nbz;
+#define TH_G3(RR, CC) k_gauss3<RR, CC><<<grid, GaussThreads<CC>::N, gauss_tile_lds_floats<RR, CC, kGT>() * 4, st>>>(in, out, taps, g)
     switch (R) {
-        case 1: k_gauss3<1, C><<<grid, GaussThreads<C>::N, 0, st>>>(in, out, taps, g); break;
-        case 2: k_gauss3<2, C><<<grid, GaussThreads<C>::N, 0, st>>>(in, out, taps, g); break;
-        case 3: k_gauss3<3, C><<<grid, GaussThreads<C>::N, 0, st>>>(in, out, taps, g); break;
-        case 4: k_gauss3<4, C><<<grid, GaussThreads<C>::N, 0, st>>>(in, out, taps, g); break;
-        case 5: k_gauss3<5, C><<<grid, GaussThreads<C>::N, 0, st>>>(in, out, taps, g); break;
-        case 6: k_gauss3<6, C><<<grid, GaussThreads<C>::N, 0, st>>>(in, out, taps, g); break;
-        case 7: if (C == 1) { k_gauss3<7, 1><<<grid, 256, 0, st>>>(in, out, taps, g); break; } [[fallthrough]];
-        case 8: if (C == 1) { k_gauss3<8, 1><<<grid, 256, 0, st>>>(in, out, taps, g); break; } [[fallthrough]];
+        case 1: TH_G3(1, C); break;
+        case 2: TH_G3(2, C); break;
+        case 3: TH_G3(3, C); break;
+        case 4: TH_G3(4, C); break;
+        case 5: TH_G3(5, C); break;
+        case 6: TH_G3(6, C); break;
+        case 7: if (C == 1) { TH_G3(7, 1); break; } [[fallthrough]];
+        case 8: if (C == 1) { TH_G3(8, 1); break; } [[fallthrough]];
         default:
             // 6-component fields keep a (2R+1) x 6 register ring: R <= 6 (13 taps) stays spill-free
             set_error("Gaussian radius %d outside 1..%d for %d components", R, C == 1 ? 8 : 6, C);
             return TIRA_ERR_INVALID_ARG;
     }
+#undef TH_G3
     TH_HIP(hipGetLastError());
     return TIRA_OK;
 }

@@ -7,7 +7,7 @@
 
 namespace th {
 
-constexpr int kGT = 16;  // tile edge (x and y)
+constexpr int kGT = 16;  // tile width along x (tile height TY along y: 8, 16 or 32)
 
 // LDS row of a staged patch of edge pw: room for the x-pass's 16-byte reads
 // past the edge, rounded up to 16 floats past a multiple of 32.
@@ -54,32 +54,43 @@ __device__ __forceinline__ void tile_eig3(const float* s_raw, int pos, int cs, f
 // offsets for every z-plane, so their global offsets and LDS slots are
 // computed once, and the next plane is prefetched into registers while the
 // current one is filtered.
-// 6-component fields use 512 threads: the two 256-thread halves split the
+// A tile is 16 (x) by TY (y) outputs, one thread per output of a plane;
+// 6-component fields use two such thread sets: the halves split the
 // components (3 each) for the y- and z-passes, which keeps the register ring
 // at (2R+1) x 3 floats per thread.
-template <int C> struct GaussThreads { static constexpr int CG = C == 6 ? 2 : 1, N = 256 * CG; };
+template <int C, int TY = kGT> struct GaussThreads {
+    static constexpr int CG = C == 6 ? 2 : 1, N = kGT * TY * CG;
+};
 
-template <int R, int C, bool EIG>
+// LDS floats of a tile: the staged patch (component-major, padded rows), then
+// the x-filtered patch.
+template <int R, int C, int TY>
+__host__ __device__ constexpr int gauss_tile_lds_floats() {
+    return C * (TY + 2 * R) * gauss_row_stride(kGT + 2 * R) + C * (TY + 2 * R) * kGT;
+}
+
+template <int R, int C, bool EIG, int TY = kGT>
 __device__ __forceinline__ void gauss3_tile(const float* __restrict__ in, float* __restrict__ out,
                                             const float* __restrict__ taps, const Gauss3Geom& g, unsigned b,
                                             float* __restrict__ evals, float* s_raw, float* s_xf) {
     constexpr int K = 2 * R + 1;
-    constexpr int NT = GaussThreads<C>::N;
-    constexpr int CP = C / GaussThreads<C>::CG;         // components per thread in the y/z passes
-    constexpr int PW = kGT + 2 * R;                     // staged patch edge
+    constexpr int NT = GaussThreads<C, TY>::N;
+    constexpr int CP = C / GaussThreads<C, TY>::CG;     // components per thread in the y/z passes
+    constexpr int PW = kGT + 2 * R;                     // staged patch width (x)
+    constexpr int PH = TY + 2 * R;                      // staged patch height (y)
     constexpr int RS = gauss_row_stride(PW);            // padded LDS row
-    constexpr int NV = PW * PW;                         // staged voxels per plane
+    constexpr int NV = PW * PH;                         // staged voxels per plane
     constexpr int NS = (NV + NT - 1) / NT;              // staged voxels per thread
     constexpr int NSEG = kGT / 4;                       // 4-output segments per x-filtered row
-    constexpr int NXT = C * PW * NSEG;                  // x-pass tasks per plane
+    constexpr int NXT = C * PH * NSEG;                  // x-pass tasks per plane
     constexpr int NX = (NXT + NT - 1) / NT;
     constexpr int NWIN = 4 + 2 * R;                     // inputs per x-pass task
     constexpr int NWIN4 = (NWIN + 3) / 4;
 
     const int tid = threadIdx.x;
-    const int tx = tid % kGT, ty = (tid / kGT) % kGT, c0 = (tid / 256) * CP;
+    const int tx = tid % kGT, ty = (tid / kGT) % TY, c0 = (tid / (kGT * TY)) * CP;
     const int bx = (int)(b % g.nbx), by = (int)((b / g.nbx) % g.nby), bz = (int)(b / (g.nbx * g.nby));
-    const int ox0 = bx * kGT, oy0 = by * kGT, oz0 = bz * g.zchunk;
+    const int ox0 = bx * kGT, oy0 = by * TY, oz0 = bz * g.zchunk;
     const int nz = min(g.zchunk, g.o0 - oz0);
     const int nplanes = nz + 2 * R;
     const float* gx = taps;
@@ -89,7 +100,7 @@ __device__ __forceinline__ void gauss3_tile(const float* __restrict__ in, float*
     const size_t plane = (size_t)g.s1 * g.s2 * C;
 
     // staged voxels of this thread: voxel (yy, xx) of the patch, LDS slot
-    // [c][yy][xx] = lslot + c * PW * RS; goff < 0 marks voxels outside the volume
+    // [c][yy][xx] = lslot + c * PH * RS; goff < 0 marks voxels outside the volume
     int goff[NS], lslot[NS];
 #pragma unroll
     for (int s = 0; s < NS; s++) {
@@ -147,7 +158,7 @@ __device__ __forceinline__ void gauss3_tile(const float* __restrict__ in, float*
                 for (int s = 0; s < NS; s++)
                     if (lslot[s] >= 0) {
 #pragma unroll
-                        for (int c = 0; c < C; c++) s_raw[lslot[s] + c * PW * RS] = pre[s][c];
+                        for (int c = 0; c < C; c++) s_raw[lslot[s] + c * PH * RS] = pre[s][c];
                     }
                 __syncthreads();
                 prefetch(i + 1);  // in flight while this plane is filtered
@@ -156,7 +167,7 @@ __device__ __forceinline__ void gauss3_tile(const float* __restrict__ in, float*
                 for (int s = 0; s < NX; s++) {
                     const int tsk = tid + NT * s;
                     if (tsk < NXT) {
-                        const int seg = tsk % NSEG, row = tsk / NSEG;  // row = c * PW + yy
+                        const int seg = tsk % NSEG, row = tsk / NSEG;  // row = c * PH + yy
                         const float4* src = reinterpret_cast<const float4*>(s_raw + row * RS + seg * 4);
                         float win[NWIN4 * 4];
 #pragma unroll
@@ -179,7 +190,7 @@ __device__ __forceinline__ void gauss3_tile(const float* __restrict__ in, float*
                 // y-pass for this thread's column, then the z ring
 #pragma unroll
                 for (int c = 0; c < CP; c++) {
-                    const float* col = s_xf + ((c0 + c) * PW + ty) * kGT + tx;
+                    const float* col = s_xf + ((c0 + c) * PH + ty) * kGT + tx;
                     float v = 0.0f;
 #pragma unroll
                     for (int k = 0; k < K; k++) v = fmaf(gy[k], col[k * kGT], v);
@@ -207,9 +218,9 @@ __device__ __forceinline__ void gauss3_tile(const float* __restrict__ in, float*
         // output grids coincide), after the z loop so that the solver's
         // registers do not add to the ring's: the tile's input columns were
         // read moments ago and come from L2.
-        const int ntile = kGT * kGT * nz;
+        const int ntile = kGT * TY * nz;
         for (int k = tid; k < ntile; k += NT) {
-            const int t = k / (kGT * kGT), rem = k % (kGT * kGT);
+            const int t = k / (kGT * TY), rem = k % (kGT * TY);
             const int ex = ox0 + rem % kGT, ey = oy0 + rem / kGT;
             if (ex < g.o2 && ey < g.o1) {
                 const size_t v = ((size_t)(oz0 + t) * g.o1 + ey) * g.o2 + ex;
@@ -221,13 +232,14 @@ __device__ __forceinline__ void gauss3_tile(const float* __restrict__ in, float*
 
 // Tensor fields up to R = 5 fit 80 VGPRs: six waves per SIMD, three blocks
 // per CU (4% faster at R = 5); R = 6 would spill at that bound (30% slower).
+// LDS: gauss_tile_lds_floats<R, C, kGT>() floats, sized at launch.
 template <int R, int C>
 __global__ __launch_bounds__(GaussThreads<C>::N, (C == 6 && R <= 5) ? 6 : 1) void k_gauss3(const float* __restrict__ in, float* __restrict__ out,
                                                                const float* __restrict__ taps, Gauss3Geom g) {
     constexpr int PW = kGT + 2 * R;
-    constexpr int RS = gauss_row_stride(PW);
-    __shared__ __attribute__((aligned(16))) float s_raw[C * PW * RS];
-    __shared__ __attribute__((aligned(16))) float s_xf[C * PW * kGT];
+    extern __shared__ __attribute__((aligned(16))) float s_gauss[];
+    float* s_raw = s_gauss;
+    float* s_xf = s_gauss + C * PW * gauss_row_stride(PW);
     // XCD-major tile order: neighbouring tiles (x fastest) run on one XCD at
     // the same time, so the halo columns they share come from its L2
     gauss3_tile<R, C, false>(in, out, taps, g, xcd_remap(blockIdx.x, gridDim.x), nullptr, s_raw, s_xf);
