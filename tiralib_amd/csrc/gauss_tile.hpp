@@ -232,17 +232,18 @@ __device__ __forceinline__ void gauss3_tile(const float* __restrict__ in, float*
 
 // Tensor fields up to R = 5 fit 80 VGPRs: six waves per SIMD, three blocks
 // per CU (4% faster at R = 5); R = 6 would spill at that bound (30% slower).
-// LDS: gauss_tile_lds_floats<R, C, kGT>() floats, sized at launch.
-template <int R, int C>
-__global__ __launch_bounds__(GaussThreads<C>::N, (C == 6 && R <= 5) ? 6 : 1) void k_gauss3(const float* __restrict__ in, float* __restrict__ out,
-                                                               const float* __restrict__ taps, Gauss3Geom g) {
+// LDS: gauss_tile_lds_floats<R, C, TY>() floats, sized at launch.
+template <int R, int C, int TY> struct Gauss3Bound { static constexpr int W = (C == 6 && TY == kGT && R <= 5) ? 6 : 1; };
+template <int R, int C, int TY = kGT>
+__global__ __launch_bounds__((GaussThreads<C, TY>::N), (Gauss3Bound<R, C, TY>::W)) void k_gauss3(
+    const float* __restrict__ in, float* __restrict__ out, const float* __restrict__ taps, Gauss3Geom g) {
     constexpr int PW = kGT + 2 * R;
     extern __shared__ __attribute__((aligned(16))) float s_gauss[];
     float* s_raw = s_gauss;
-    float* s_xf = s_gauss + C * PW * gauss_row_stride(PW);
+    float* s_xf = s_gauss + C * (TY + 2 * R) * gauss_row_stride(PW);
     // XCD-major tile order: neighbouring tiles (x fastest) run on one XCD at
     // the same time, so the halo columns they share come from its L2
-    gauss3_tile<R, C, false>(in, out, taps, g, xcd_remap(blockIdx.x, gridDim.x), nullptr, s_raw, s_xf);
+    gauss3_tile<R, C, false, TY>(in, out, taps, g, xcd_remap(blockIdx.x, gridDim.x), nullptr, s_raw, s_xf);
 }
 
 }  // namespace th
