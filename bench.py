@@ -32,15 +32,16 @@ PEAK_FP32_TFLOPS = 157.3          # MI355X FP32 vector, published (needs packed 
 PEAK_HBM_GBS = 8000.0
 
 # Algorithmic FLOP per in-bounds (receiver, voter) pair of the fused kernel
-# (FMA = 2), counted from the formulation the kernel evaluates:
-STICK_PAIR_FLOP = 31         # t = q.d, eta, r = q - 2 t d, eta * s * r r^T
-PLATE_CLOSED_PAIR_FLOP = 65  # s = n.d, 4 polynomials in 1 - s^2, alpha I + beta nn^T + (h d^T + d h^T)
-# Reference formulation (SURVEY counting rule): stick 33, N-sample plate 41 N + 12.
+# (FMA = 2), counted from the formulation the kernel evaluates; per-voter
+# work (the voter's scaled normal w n) is left out:
+STICK_PAIR_FLOP = 30         # t = q.d (5), eta (3), scale (1), r = q - t (2 d) (6), eta s r r^T (15)
+PLATE_CLOSED_PAIR_FLOP = 63  # s = n.d, 4 polynomials in 1 - s^2, w (alpha I + beta nn^T) + (h d^T + d h^T)
+# The reference's own formulation counts stick 33 and N-sample plate 41 N + 12.
 ATV3_PAIR_FLOP = 106         # a = e2.v, b = e0.v, six coefficients, 3 symmetric rank-2 updates + trace term
-# Tabulated learned plate per pair: s = n.d (5), x = 1 - s^2 (2), cell index and
-# fraction (3), linear interpolation of (A, B, g, D/2) (8), voter scale (6), then
-# A I + B nn^T + (h d^T + d h^T) as the closed form's last step (43).
-MLP_TABLE_PLATE_PAIR_FLOP = 67
+# Tabulated learned plate per pair: s = n.d (5), x = 1 - s^2 (2), cell position
+# (2), linear interpolation of the staged c1/pi (A, B, g, D/2) (8), then
+# w (A I + B nn^T) + (h d^T + d h^T) with h = g s w n + w D d (46).
+MLP_TABLE_PLATE_PAIR_FLOP = 63
 MLP_NET_FLOP = 2 * (16 * 64 + 64 * 64 * 2 + 64 * 64 * 2 + 64 * 32 * 2)  # MFMA work of one split-activation evaluation
 
 

@@ -770,8 +770,7 @@ __device__ __forceinline__ void add_circle_average(const float4 N, float* __rest
 // take two or more groups: a second eigen-solve costs less than writing and
 // re-reading a 32-byte payload per voxel).
 // Four waves per SIMD (at most 128 VGPRs): unbounded, the eigen-solve and the
-// features take 136 and three waves; the bound costs a 16-VGPR spill and
-// gains 10% at 128^3.
+// features take 136 and three waves; bounded, they fit in 120 without a spill.
 template <bool FROM_TENSORS>
 __global__ __launch_bounds__(kFT, 4) void k_spec_xfwd(const float4* __restrict__ ps, const float4* __restrict__ pp,
                                                    const float* __restrict__ tensors, int layout, SpecGeom g, int B,
@@ -801,9 +800,12 @@ __global__ __launch_bounds__(kFT, 4) void k_spec_xfwd(const float4* __restrict__
         }
         for (int idx = (int)threadIdx.x; idx < Bv * g.n2; idx += kFT) {
             const int b = fdiv(idx, g.n2, rn2), e = idx - b * g.n2;
-            float A[kMono + 1];  // the 15 combined features and a zero (the last channel's imaginary part)
-#pragma unroll
-            for (int f = 0; f <= kMono; f++) A[f] = 0.0f;
+            // the 15 combined features and a zero (the last channel's imaginary
+            // part); the stick features are written first, zero when the vote
+            // has no stick part (zeroing A ahead of the eigen-solve kept 15
+            // registers live through it and spilled 80 bytes per lane)
+            float A[kMono + 1];
+            A[kMono] = 0.0f;
             {
                 const size_t vox = (size_t)(row0 + b) * g.n2 + e;
                 float4 S = make_float4(0.0f, 0.0f, 0.0f, 0.0f), N = S;
@@ -822,7 +824,7 @@ __global__ __launch_bounds__(kFT, 4) void k_spec_xfwd(const float4* __restrict__
                     if (g.stick) S = ps[vox];
                     if (g.plate) N = pp[vox];
                 }
-                if (g.stick) mono15(S, A);
+                mono15(g.stick ? S : make_float4(0.0f, 0.0f, 0.0f, 0.0f), A);
                 if (g.plate) add_circle_average(make_float4(N.x, N.y, N.z, N.w * g.alpha), A);
                 if (g.atv) {  // three times the sphere averages: 1/5 for x^4, 1/15 for x^2 y^2
                     A[0] = fmaf(0.6f, s3, A[0]);

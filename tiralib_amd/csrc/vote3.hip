@@ -238,34 +238,41 @@ __device__ __forceinline__ float horner(const float* c, float x) {
 }
 
 // Stick vote of voter S = (q, scale) at offset entry e (stickvote3_accumulate_kernel,
-// tensorvote.h:1183-1213, sticknorm folded into e[6..7]).
+// tensorvote.h:1183-1213, sticknorm folded into e[6..7]; e[3..5] = 2 d, so the
+// reflection q - 2 t d takes one FMA per component).
 template <int P>
 __device__ __forceinline__ void stick_pair(Acc& a, const float4 S, const float* __restrict__ e) {
     const float d0 = e[0], d1 = e[1], d2 = e[2];
     const float t = fmaf(S.x, d0, fmaf(S.y, d1, S.z * d2));
-    const float t2 = t * t;
     float eta;
-    if (P == 1) eta = fmaf(e[7], t2, e[6]);
-    else eta = fmaf(e[6], ipow<P>(1.0f - t2), e[7] * ipow<P>(t2));
+    if (P == 1) eta = fmaf(t, t * e[7], e[6]);
+    else {
+        const float t2 = t * t;
+        eta = fmaf(e[6], ipow<P>(1.0f - t2), e[7] * ipow<P>(t2));
+    }
     const float f = eta * S.w;
-    const float m2t = -2.0f * t;
-    const float r0 = fmaf(m2t, d0, S.x), r1 = fmaf(m2t, d1, S.y), r2 = fmaf(m2t, d2, S.z);
+    const float r0 = fmaf(-t, e[3], S.x), r1 = fmaf(-t, e[4], S.y), r2 = fmaf(-t, e[5], S.z);
     const float f0 = f * r0, f1 = f * r1, f2 = f * r2;
     a.xx = fmaf(f0, r0, a.xx); a.xy = fmaf(f0, r1, a.xy); a.xz = fmaf(f0, r2, a.xz);
     a.yy = fmaf(f1, r1, a.yy); a.yz = fmaf(f1, r2, a.yz); a.zz = fmaf(f2, r2, a.zz);
 }
 
-// Accumulates A I + B n n^T + G (n d^T + d n^T) + 2 D d d^T for the voter
-// normal n = N.xyz and the offset entry e (d = e[0..2], 2 d = e[3..5]):
-// G (n d^T + d n^T) + 2 D d d^T = h d^T + d h^T with h = G n + D d.
-__device__ __forceinline__ void plate_accumulate(Acc& a, const float4 N, const float* __restrict__ e, float A,
-                                                 float B, float G, float D) {
+// Accumulates w [A I + B n n^T + G (n d^T + d n^T) + 2 D d d^T] for the voter
+// normal n = N.xyz, scale w = N.w and the offset entry e (d = e[0..2],
+// 2 d = e[3..5]): w G (n d^T + d n^T) + 2 w D d d^T = h d^T + d h^T with
+// h = G nw + w D d. nw = w n is computed once per voter (voter_nw): the scale
+// then costs two products per pair instead of five.
+__device__ __forceinline__ float3 voter_nw(const float4 N) { return make_float3(N.w * N.x, N.w * N.y, N.w * N.z); }
+
+__device__ __forceinline__ void plate_accumulate(Acc& a, const float4 N, const float3 nw, const float* __restrict__ e,
+                                                 float A, float B, float G, float D) {
     const float d0 = e[0], d1 = e[1], d2 = e[2];
-    a.ii += A;
-    const float b0 = B * N.x, b1 = B * N.y, b2 = B * N.z;
+    a.ii = fmaf(N.w, A, a.ii);
+    const float b0 = B * nw.x, b1 = B * nw.y, b2 = B * nw.z;
     a.xx = fmaf(b0, N.x, a.xx); a.xy = fmaf(b0, N.y, a.xy); a.xz = fmaf(b0, N.z, a.xz);
     a.yy = fmaf(b1, N.y, a.yy); a.yz = fmaf(b1, N.z, a.yz); a.zz = fmaf(b2, N.z, a.zz);
-    const float h0 = fmaf(D, d0, G * N.x), h1 = fmaf(D, d1, G * N.y), h2 = fmaf(D, d2, G * N.z);
+    const float Dw = N.w * D;
+    const float h0 = fmaf(Dw, d0, G * nw.x), h1 = fmaf(Dw, d1, G * nw.y), h2 = fmaf(Dw, d2, G * nw.z);
     a.xx = fmaf(h0, e[3], a.xx); a.yy = fmaf(h1, e[4], a.yy); a.zz = fmaf(h2, e[5], a.zz);
     a.xy = fmaf(h0, d1, fmaf(h1, d0, a.xy));
     a.xz = fmaf(h0, d2, fmaf(h2, d0, a.xz));
@@ -274,30 +281,49 @@ __device__ __forceinline__ void plate_accumulate(Acc& a, const float4 N, const f
 
 // Plate vote of voter N = (n, scale) at offset entry e, closed form:
 //   scale * [alpha I + beta n n^T + gamma (n d^T + d n^T) + delta d d^T]
-// (the table holds delta / 2, see plate_accumulate).
+// (the table holds delta / 2, see plate_accumulate); nw = voter_nw(N).
 template <int P>
-__device__ __forceinline__ void plate_pair(Acc& a, const float4 N, const float* __restrict__ e) {
+__device__ __forceinline__ void plate_pair(Acc& a, const float4 N, const float3 nw, const float* __restrict__ e) {
     const float s = fmaf(N.x, e[0], fmaf(N.y, e[1], N.z * e[2]));
     const float x = fmaf(-s, s, 1.0f);
     const float* al = e + 8;
     const float* be = al + (P + 1);
     const float* ga = be + (P + 1);
     const float* de = ga + (P + 1);
-    const float sp = N.w;
-    plate_accumulate(a, N, e, sp * horner<P>(al, x), sp * horner<P>(be, x), (sp * s) * horner<P>(ga, x),
-                     sp * horner<P + 1>(de, x));
+    plate_accumulate(a, N, nw, e, horner<P>(al, x), horner<P>(be, x), s * horner<P>(ga, x), horner<P + 1>(de, x));
 }
 
-// Plate vote of voter N = (n, scale) from the tabulated network: the row of
-// the offset's distance (LDS), looked up at x = 1 - (n.d)^2, gives
-// (A, B, g, D/2) of the vote / c1; c1pi = c1 / pi of the offset.
-__device__ __forceinline__ void tab_plate_pair(Acc& a, const float4 N, const float* __restrict__ e,
-                                               const float4* row, float c1pi) {
+template <int P>
+__device__ __forceinline__ void plate_pair(Acc& a, const float4 N, const float* __restrict__ e) {
+    plate_pair<P>(a, N, voter_nw(N), e);
+}
+
+// Network-table row as the tabulated vote stages it in LDS: kMlpTabPad cells
+// of (value, difference to the next cell), cell p holding the stored cell
+// p - 1 scaled by c1 / pi of the offset; the first and the last cell continue
+// the end segments linearly, so that the lookup needs no clamp.
+constexpr int kMlpTabPad = kMlpTabM + 1;
+
+// Linear interpolation in a staged row at x = 1 - (n.d)^2 in [0, 1]:
+// t = 64 x + 1/2 lies in [1/2, 64 + 1/2], its cell in [0, 64] (|n.d| <= 1
+// within rounding keeps t above zero, where truncation is the floor).
+__device__ __forceinline__ float4 mlp_tab_lookup_staged(const float4* __restrict__ row, float x) {
+    const float t = fmaf(x, (float)kMlpTabM, 0.5f);
+    const int p = (int)t;
+    const float f = __builtin_amdgcn_fractf(t);
+    const float4 v = row[2 * p], dl = row[2 * p + 1];
+    return make_float4(fmaf(f, dl.x, v.x), fmaf(f, dl.y, v.y), fmaf(f, dl.z, v.z), fmaf(f, dl.w, v.w));
+}
+
+// Plate vote of voter N = (n, scale) from the tabulated network: the staged
+// row of the offset's distance, looked up at x = 1 - (n.d)^2, gives
+// c1 / pi (A, B, g, D/2) of the offset.
+__device__ __forceinline__ void tab_plate_pair(Acc& a, const float4 N, const float3 nw, const float* __restrict__ e,
+                                               const float4* row) {
     const float s = fmaf(N.x, e[0], fmaf(N.y, e[1], N.z * e[2]));
     const float x = fmaf(-s, s, 1.0f);
-    const float4 v = mlp_tab_lookup(row, x);
-    const float sp = N.w * c1pi;
-    plate_accumulate(a, N, e, sp * v.x, sp * v.y, (sp * s) * v.z, sp * v.w);
+    const float4 v = mlp_tab_lookup_staged(row, x);
+    plate_accumulate(a, N, nw, e, v.x, v.y, s * v.z, v.w);
 }
 
 // Literal N-sample plate sum (plate3_numerical_device, tensorvote.h:2457-2485):
@@ -421,6 +447,7 @@ __global__ __launch_bounds__(kThreads) void k_vote3_closed(const float4* __restr
                 float4 S, N;
                 if (STICK) S = pay_s[zrow + du];
                 if (PLATE) N = pay_p[zrow + du];
+                const float3 nw = PLATE ? voter_nw(N) : make_float3(0.0f, 0.0f, 0.0f);
                 const int ebase = (dv + r) * w + (du + r);
 #pragma unroll
                 for (int j = 0; j < BZ; j++) {
@@ -428,7 +455,7 @@ __global__ __launch_bounds__(kThreads) void k_vote3_closed(const float4* __restr
                     if (dw >= -r && dw <= r) {
                         const float* e = tab + (size_t)((dw + r) * w * w + ebase) * STRIDE;
                         if (STICK) stick_pair<P>(acc[j], S, e);
-                        if (PLATE) plate_pair<P>(acc[j], N, e);
+                        if (PLATE) plate_pair<P>(acc[j], N, nw, e);
                     }
                 }
             }
@@ -490,6 +517,7 @@ __global__ __launch_bounds__(kThreads) void k_vote3_closed_lds(const float4* __r
                 float4 S, N;
                 if (STICK) S = sS[lane + r + du];
                 if (PLATE) N = sN[lane + r + du];
+                const float3 nw = PLATE ? voter_nw(N) : make_float3(0.0f, 0.0f, 0.0f);
                 const int ebase = (dv + r) * w + (du + r);
 #pragma unroll
                 for (int j = 0; j < BZ; j++) {
@@ -497,7 +525,7 @@ __global__ __launch_bounds__(kThreads) void k_vote3_closed_lds(const float4* __r
                     if (dw >= -r && dw <= r) {
                         const float* e = tab + (size_t)((dw + r) * w * w + ebase) * STRIDE;
                         if (STICK) stick_pair<P>(acc[j], S, e);
-                        if (PLATE) plate_pair<P>(acc[j], N, e);
+                        if (PLATE) plate_pair<P>(acc[j], N, nw, e);
                     }
                 }
             }
@@ -924,10 +952,11 @@ __global__ __launch_bounds__(kThreads) void k_vote3_mlp(const float4* __restrict
 // Stick + tabulated-network plate gather. Same tiling, padding, z register
 // blocking and XCD order as k_vote3_closed. Per offset column (dv, du) the
 // block stages in LDS the network-table rows of the column's distances
-// (r + 1 rows: |dw| = 0..r, kMlpTabM cells of two float4 each), so every
-// pair reads its row from LDS. tab: kMlpTabStride floats per offset
-// (build_mlptab_table). Voters flagged by the error check carry plate scale
-// 0 here; their plates come from the integrator's pass.
+// (r + 1 rows: |dw| = 0..r, kMlpTabPad cells of two float4 each, scaled by
+// the offset's c1 / pi), so every pair reads its row from LDS. tab:
+// kMlpTabStride floats per offset (build_mlptab_table). Voters flagged by the
+// error check carry plate scale 0 here; their plates come from the
+// integrator's pass.
 template <int P, bool STICK, int BZ>
 __global__ __launch_bounds__(kThreads) void k_vote3_mlptab(const float4* __restrict__ pay_s,
                                                            const float4* __restrict__ pay_p,
@@ -936,7 +965,7 @@ __global__ __launch_bounds__(kThreads) void k_vote3_mlptab(const float4* __restr
                                                            VoteGeom g, float* __restrict__ out, int out_layout) {
     constexpr int STRIDE = kMlpTabStride;
     constexpr int C1PI = 8, ROW = 9;
-    constexpr int CELLS2 = 2 * kMlpTabM;  // float4 per row
+    constexpr int CELLS2 = 2 * kMlpTabPad;  // float4 per staged row
     extern __shared__ float4 s_rows[];
     unsigned bx, by, bz;
     tile_coords_zfast(g, bx, by, bz);
@@ -957,8 +986,21 @@ __global__ __launch_bounds__(kThreads) void k_vote3_mlptab(const float4* __restr
             __syncthreads();  // the previous column's rows are no longer read
             for (int i = threadIdx.x; i < nst; i += kThreads) {
                 const int slot = i / CELLS2;  // |dw|: offsets +-dw share the distance
-                const int row = min(max((int)tab[(size_t)((slot + r) * w * w + ebase) * STRIDE + ROW], 0), nrows - 1);
-                s_rows[i] = vtab[(size_t)row * CELLS2 + (i - slot * CELLS2)];
+                const int q = i - slot * CELLS2, p = q >> 1;
+                const float* e = tab + (size_t)((slot + r) * w * w + ebase) * STRIDE;
+                const int row = min(max((int)e[ROW], 0), nrows - 1);
+                // staged cell p: stored cell o = p - 1, the end cells extended by +-1 step
+                const int o = min(max(p - 1, 0), kMlpTabM - 2);
+                const float4* src = vtab + ((size_t)row * kMlpTabM + o) * 2;
+                const float4 dl = src[1];
+                float4 v = dl;
+                if (!(q & 1)) {
+                    const float sh = (float)(p - 1 - o);
+                    v = src[0];
+                    v = make_float4(fmaf(sh, dl.x, v.x), fmaf(sh, dl.y, v.y), fmaf(sh, dl.z, v.z), fmaf(sh, dl.w, v.w));
+                }
+                const float c = e[C1PI];
+                s_rows[i] = make_float4(c * v.x, c * v.y, c * v.z, c * v.w);
             }
             __syncthreads();
             const size_t col = ((size_t)z0 * g.P1 + (size_t)(y + r + dv)) * g.P2 + (size_t)(x + r + du);
@@ -966,13 +1008,14 @@ __global__ __launch_bounds__(kThreads) void k_vote3_mlptab(const float4* __restr
                 float4 S;
                 if (STICK) S = pay_s[col + (size_t)zz * plane];
                 const float4 N = pay_p[col + (size_t)zz * plane];
+                const float3 nw = voter_nw(N);
 #pragma unroll
                 for (int j = 0; j < BZ; j++) {
                     const int dw = zz - r - j;
                     if (dw >= -r && dw <= r) {
                         const float* e = tab + (size_t)((dw + r) * w * w + ebase) * STRIDE;
                         if (STICK) stick_pair<P>(acc[j], S, e);
-                        tab_plate_pair(acc[j], N, e, s_rows + (dw < 0 ? -dw : dw) * CELLS2, e[C1PI]);
+                        tab_plate_pair(acc[j], N, nw, e, s_rows + (dw < 0 ? -dw : dw) * CELLS2);
                     }
                 }
             }
@@ -1467,16 +1510,36 @@ static void launch_mlp(bool stick, unsigned grid, hipStream_t st, const float4* 
     else k_vote3_mlp<P, false><<<grid, kThreads, 0, st>>>(ps, pn, pu, ct, mt, wr, aff, g, out, layout);
 }
 
-template <int P>
-static void launch_mlptab(bool stick, unsigned grid, size_t lds, hipStream_t st, const float4* ps, const float4* pn,
-                          const float* t, const float4* vt, int nrows, const VoteGeom& g, float* out, int layout) {
-    if (stick) k_vote3_mlptab<P, true, kBZMlp><<<grid, kThreads, lds, st>>>(ps, pn, t, vt, nrows, g, out, layout);
-    else k_vote3_mlptab<P, false, kBZMlp><<<grid, kThreads, lds, st>>>(ps, pn, t, vt, nrows, g, out, layout);
+// Largest window of the tabulated-network vote: its LDS holds (w / 2 + 1)
+// staged distance rows of 2 kMlpTabPad float4 (65 KB at w = 63).
+constexpr unsigned kMlpTabMaxW = 63;
+
+// Rows above 64 KB (w = 63) need the kernel's dynamic-LDS limit raised, once
+// per process and kernel.
+template <int P, bool STICK>
+static int mlptab_lds_attr(size_t lds) {
+    static bool set = false;
+    if (lds > 65536 && !set) {
+        TH_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_vote3_mlptab<P, STICK, kBZMlp>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)((kMlpTabMaxW / 2 + 1) * 2 * kMlpTabPad * sizeof(float4))));
+        set = true;
+    }
+    return TIRA_OK;
 }
 
-// Largest window of the tabulated-network vote: its LDS holds (w / 2 + 1)
-// distance rows of 2 kMlpTabM float4 (64 KB at w = 63).
-constexpr unsigned kMlpTabMaxW = 63;
+template <int P>
+static int launch_mlptab(bool stick, unsigned grid, size_t lds, hipStream_t st, const float4* ps, const float4* pn,
+                         const float* t, const float4* vt, int nrows, const VoteGeom& g, float* out, int layout) {
+    if (stick) {
+        TH_TRY((mlptab_lds_attr<P, true>(lds)));
+        k_vote3_mlptab<P, true, kBZMlp><<<grid, kThreads, lds, st>>>(ps, pn, t, vt, nrows, g, out, layout);
+    } else {
+        TH_TRY((mlptab_lds_attr<P, false>(lds)));
+        k_vote3_mlptab<P, false, kBZMlp><<<grid, kThreads, lds, st>>>(ps, pn, t, vt, nrows, g, out, layout);
+    }
+    return TIRA_OK;
+}
 
 // Stick + learned plate vote on device buffers with the network `m` (trained
 // for its sigma1, sigma2 and power). Voters whose orientation bin's error
@@ -1524,16 +1587,16 @@ int vote3_mlp_run(PlateMlp* m, const float* tensors, int in_layout, float* out, 
     if (table) {
         float* mt = nullptr;
         TH_TRY(mlptab_table_for(P, (int)w, m->sigma1, m->sigma2, m->power, st, &mt));
-        const size_t lds = (size_t)(w / 2 + 1) * 2 * kMlpTabM * sizeof(float4);
+        const size_t lds = (size_t)(w / 2 + 1) * 2 * kMlpTabPad * sizeof(float4);
         const float4* vt = m->vtab;
         const int nr = m->vtab_rows;
         {
             ProfScope prof("vote3_mlptab", st);
             switch (P) {
-                case 1: launch_mlptab<1>(stick, grid, lds, st, ps, pn, mt, vt, nr, g, out, out_layout); break;
-                case 2: launch_mlptab<2>(stick, grid, lds, st, ps, pn, mt, vt, nr, g, out, out_layout); break;
-                case 3: launch_mlptab<3>(stick, grid, lds, st, ps, pn, mt, vt, nr, g, out, out_layout); break;
-                default: launch_mlptab<4>(stick, grid, lds, st, ps, pn, mt, vt, nr, g, out, out_layout); break;
+                case 1: TH_TRY(launch_mlptab<1>(stick, grid, lds, st, ps, pn, mt, vt, nr, g, out, out_layout)); break;
+                case 2: TH_TRY(launch_mlptab<2>(stick, grid, lds, st, ps, pn, mt, vt, nr, g, out, out_layout)); break;
+                case 3: TH_TRY(launch_mlptab<3>(stick, grid, lds, st, ps, pn, mt, vt, nr, g, out, out_layout)); break;
+                default: TH_TRY(launch_mlptab<4>(stick, grid, lds, st, ps, pn, mt, vt, nr, g, out, out_layout)); break;
             }
         }
         TH_HIP(hipGetLastError());
