@@ -244,7 +244,12 @@ TH_HD void eig3_vecs(T a, T b, T c, T d, T e, T f, T l0, T l1, T l2, T diag_eps,
 
 // Fast reciprocal / reciprocal square root: hardware approximations in FP32
 // device code (used only inside self-correcting or normalising steps), exact
-// division elsewhere.
+// division elsewhere. FP64 device code takes the hardware approximation and
+// two Newton steps (relative error at the level of the last bit instead of
+// the correctly rounded division and square-root sequences, a third of their
+// instructions); the operands are positive and normal wherever these are
+// called (a spread p > 0, 3 (b^2 - 1) >= 6, squared norms of nonzero
+// vectors).
 TH_HD float d_rcp(float x) {
 #ifdef __HIP_DEVICE_COMPILE__
     return __builtin_amdgcn_rcpf(x);
@@ -252,7 +257,15 @@ TH_HD float d_rcp(float x) {
     return 1.0f / x;
 #endif
 }
-TH_HD double d_rcp(double x) { return 1.0 / x; }
+TH_HD double d_rcp(double x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return fma(r, fma(-x, r, 1.0), r);
+#else
+    return 1.0 / x;
+#endif
+}
 TH_HD float d_rsqrt(float x) {
 #ifdef __HIP_DEVICE_COMPILE__
     return __builtin_amdgcn_rsqf(x);
@@ -260,7 +273,15 @@ TH_HD float d_rsqrt(float x) {
     return 1.0f / sqrtf(x);
 #endif
 }
-TH_HD double d_rsqrt(double x) { return 1.0 / sqrt(x); }
+TH_HD double d_rsqrt(double x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    double y = __builtin_amdgcn_rsq(x);
+    y = fma(0.5 * y, fma(-x * y, y, 1.0), y);
+    return fma(0.5 * y, fma(-x * y, y, 1.0), y);
+#else
+    return 1.0 / sqrt(x);
+#endif
+}
 // Square root without the correctly-rounded fix-up sequence (hardware
 // v_sqrt_f32, about 1 ulp) in FP32 device code; exact sqrt elsewhere.
 TH_HD float d_sqrt_fast(float x) {
@@ -270,7 +291,15 @@ TH_HD float d_sqrt_fast(float x) {
     return sqrtf(x);
 #endif
 }
-TH_HD double d_sqrt_fast(double x) { return sqrt(x); }
+TH_HD double d_sqrt_fast(double x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    // x rsqrt(x) with a final Newton correction of the root itself; 0 stays 0
+    const double y = d_rsqrt(x), s = x * y;
+    return x > 0.0 ? fma(0.5 * y, fma(-s, s, x), s) : 0.0;
+#else
+    return sqrt(x);
+#endif
+}
 
 // Largest root of beta^3 - 3 beta - 2 r = 0 for r in [0, 1], i.e.
 // beta = 2 cos(acos(r) / 3) in [sqrt 3, 2]: a cubic fit (max error 1.3e-4)
