@@ -700,7 +700,7 @@ def eig_errors(got, ref6, significant=1e-2):
     """Eigenvalue errors of `got` (n x 3, ascending) against LAPACK FP64
     eigenvalues of the FP64 packed-6 tensors ref6. frobenius_rel: the
     eigenvalue arrays as a whole, ||dl||_F / ||l||_F (as the vote tensors are
-    judged); normwise (max_i |dl_i| / max_j |l_j|) and per-eigenvalue
+    compared); normwise (max_i |dl_i| / max_j |l_j|) and per-eigenvalue
     (|dl_i| / |l_i|) errors over the tensors whose largest |l| is at least
     `significant` times the sample's (the vote of an empty region is FFT
     rounding of the whole field's norm, so its own eigenvalues are not a

@@ -1,6 +1,6 @@
 """GPU parity of the 3D stick + plate vote against the CPU oracle.
 
-Bar (north star): relative Frobenius error <= 1e-4 against the
+Bar: relative Frobenius error <= 1e-4 against the
 reference-faithful FP32 restatement of cpu::tensorvote3 on identical seeded
 inputs, compared on the flat slots {0,1,2,4,5,8} (the reference CPU path
 accumulates only the upper triangle of the plate, tensorvote.h:1747-1751, so
