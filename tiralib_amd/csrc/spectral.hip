@@ -57,6 +57,9 @@ constexpr int kFT = 256;       // threads per FFT block
 #define TH_FFT_BL 4096
 #endif
 constexpr int kFftBL = TH_FFT_BL;  // complex values staged per block (32 KB of LDS)
+// k_fft_cols staging + twiddles, bytes: four blocks per CU (B = 16 at L = 400
+// took 54 KB, two blocks per CU, and ran its pass 1.4x slower than B = 8)
+constexpr size_t kColsLds = 40960;
 
 constexpr int kMaxL = 1024;    // longest transform
 constexpr int kMaxStages = 16;
@@ -516,6 +519,7 @@ struct LineGeom {
     int na, nb, nchan, B, nab;
     int n_in, n_out;
     int lb;  // log2 B for k_fft_cols (B a power of two)
+    int Bh;  // k_fft_cols: lines per group of the middle stages (a power of two dividing B)
 };
 
 // ROWS: elements contiguous (estride 1), loads walk a line first; otherwise
@@ -566,30 +570,33 @@ __global__ __launch_bounds__(kFT) void k_fft_lines(float2* __restrict__ data, Li
 template <int R, bool INV, int BL>
 __device__ __forceinline__ void cols_first(const float2* __restrict__ base, lds_f2* __restrict__ s, int L, int B,
                                            int lb, int Bv, size_t estride, int n_in) {
-    // every load of the thread is issued before the first butterfly
+    // every load of a round of MAXQ butterflies per thread is issued before
+    // its first butterfly; batches above BL values (B L > BL) take more rounds
     constexpr int MAXQ = (BL / R + kFT - 1) / kFT;
     const int m = L / R;
     const int nb = B * m;
     float2 u[MAXQ][R];
+    for (int t0 = 0; t0 < nb; t0 += MAXQ * kFT) {
 #pragma unroll
-    for (int q = 0; q < MAXQ; q++) {
-        const int t = (int)threadIdx.x + q * kFT;
-        const int l = t & (B - 1), i = t >> lb;
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            const int e = i + r * m;
-            u[q][r] = (t < nb && l < Bv && e < n_in) ? base[(size_t)e * estride + l] : make_float2(0.0f, 0.0f);
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < MAXQ; q++) {
-        const int t = (int)threadIdx.x + q * kFT;
-        if (t < nb) {
+        for (int q = 0; q < MAXQ; q++) {
+            const int t = t0 + (int)threadIdx.x + q * kFT;
             const int l = t & (B - 1), i = t >> lb;
-            dft_small<R, INV>(u[q]);
-            lds_f2* d = s + l * L + i * R;
 #pragma unroll
-            for (int r = 0; r < R; r++) st2(d + r, u[q][r]);
+            for (int r = 0; r < R; r++) {
+                const int e = i + r * m;
+                u[q][r] = (t < nb && l < Bv && e < n_in) ? base[(size_t)e * estride + l] : make_float2(0.0f, 0.0f);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < MAXQ; q++) {
+            const int t = t0 + (int)threadIdx.x + q * kFT;
+            if (t < nb) {
+                const int l = t & (B - 1), i = t >> lb;
+                dft_small<R, INV>(u[q]);
+                lds_f2* d = s + l * L + i * R;
+#pragma unroll
+                for (int r = 0; r < R; r++) st2(d + r, u[q][r]);
+            }
         }
     }
 }
@@ -657,18 +664,18 @@ __device__ __forceinline__ void cols_single(float2* __restrict__ base, int B, in
         default: TH_RADIX_SWITCH(R, CALL);                 \
     }
 
-// MINW: waves per SIMD the kernel is compiled for. Batches of 16 columns take
-// 6: the bound reaches the stage functions that only this instantiation calls
-// (TAG), which then fit 73 VGPRs without spilling (101 unbounded, five waves),
-// 1% faster at L = 135; eight waves spill and run 25% slower. Narrower batches
-// (L > 256) take no bound: there the bounded code runs 8% slower and fetches
-// 1.3x the algorithmic bytes.
+// MINW: waves per SIMD the kernel is compiled for. Stage groups of 16 lines
+// take 6: the bound reaches the stage functions that only this instantiation
+// calls (TAG), which then fit 73 VGPRs without spilling (101 unbounded, five
+// waves), 1% faster at L = 135; eight waves spill and run 25% slower. Smaller
+// groups (L > 256) take no bound: there the bounded code ran 8% slower and
+// fetched 1.3x the algorithmic bytes.
 template <bool INV, int BL, int MINW>
 __global__ __launch_bounds__(kFT, MINW) void k_fft_cols(float2* __restrict__ data, LineGeom g, FftPlan pl,
                                                   const float2* __restrict__ twg) {
-    // B L staged values, then L twiddles: sized at launch (B L <= BL / 2),
-    // 18 KB at L = 135 where a static BL-value array took 40 KB and held the
-    // pass to four blocks per CU
+    // B L staged values, then L twiddles: sized at launch (at most kColsLds
+    // bytes), 18 KB at L = 135 where a static BL-value array took 40 KB and
+    // held the pass to four blocks per CU
     extern __shared__ float2 cols_dyn[];
     const int L = pl.L;
     float2* s_ = cols_dyn;
@@ -695,11 +702,16 @@ __global__ __launch_bounds__(kFT, MINW) void k_fft_cols(float2* __restrict__ dat
     const lds_f2* tw = to_lds(tw_);
     TH_RADIX_SWITCH(pl.rad[0], (cols_first<RR, INV, BL>(base, s, L, B, lb, Bv, g.estride, g.n_in)));
     __syncthreads();
-    int p = pl.rad[0];
-    for (int st = 1; st < pl.nst - 1; st++) {
-        const int R = pl.rad[st];
-        TH_RADIX_SWITCH(R, (fft_stage<RR, INV, BL, false, true, MINW>(s, L, B, p, tw)));
-        p *= R;
+    // the middle stages take the batch in groups of Bh lines (Bh L <= BL, the
+    // stages' register rounds), so that B can reach 16 (128-byte rows) for
+    // L above 256
+    for (int h0 = 0; h0 < B; h0 += g.Bh) {
+        int p = pl.rad[0];
+        for (int st = 1; st < pl.nst - 1; st++) {
+            const int R = pl.rad[st];
+            TH_RADIX_SWITCH(R, (fft_stage<RR, INV, BL, false, true, MINW>(s + h0 * L, L, g.Bh, p, tw)));
+            p *= R;
+        }
     }
     TH_RADIX_SWITCH_LAST(pl.rad[pl.nst - 1],
                          (cols_last<RR, INV>(s, base, tw, L, B, lb, Bv, g.estride, g.n_out)));
@@ -1455,14 +1467,15 @@ static int launch_lines(float2* data, const LineGeom& lg, const FftPlan& pl, con
     TH_ARG(nblk < (1ull << 31), "spectral vote: volume too large");
     const unsigned grid = (unsigned)nblk;
     const size_t clds = ((size_t)lg.B * pl.L + pl.L) * sizeof(float2);  // k_fft_cols staging + twiddles
-    TH_ARG(rows || (size_t)lg.B * pl.L <= (size_t)kFftBL, "spectral vote: column batch exceeds the staging");
+    TH_ARG(rows || (clds <= kColsLds && lg.Bh >= 1 && lg.B % lg.Bh == 0 && (size_t)lg.Bh * pl.L <= (size_t)kFftBL),
+           "spectral vote: column batch exceeds the staging");
     if (inv) {
         if (rows) k_fft_lines<true, true><<<grid, kFT, 0, st>>>(data, lg, pl, tw);
-        else if (lg.B >= 16) k_fft_cols<true, kFftBL, 6><<<grid, kFT, clds, st>>>(data, lg, pl, tw);
+        else if (lg.Bh >= 16) k_fft_cols<true, kFftBL, 6><<<grid, kFT, clds, st>>>(data, lg, pl, tw);
         else k_fft_cols<true, kFftBL, 1><<<grid, kFT, clds, st>>>(data, lg, pl, tw);
     } else {
         if (rows) k_fft_lines<false, true><<<grid, kFT, 0, st>>>(data, lg, pl, tw);
-        else if (lg.B >= 16) k_fft_cols<false, kFftBL, 6><<<grid, kFT, clds, st>>>(data, lg, pl, tw);
+        else if (lg.Bh >= 16) k_fft_cols<false, kFftBL, 6><<<grid, kFT, clds, st>>>(data, lg, pl, tw);
         else k_fft_cols<false, kFftBL, 1><<<grid, kFT, clds, st>>>(data, lg, pl, tw);
     }
     TH_HIP(hipGetLastError());
@@ -1481,10 +1494,16 @@ static LineGeom strided_geom(size_t cs, int nchan, int L, size_t estride, int na
     g.na = na;
     g.nb = nb;
     g.nchan = nchan;
+    // B: up to 16 adjacent lines (128-byte rows of complex values) within
+    // kColsLds of staging; Bh: the largest group of them the middle stages
+    // take at once (Bh L <= kFftBL)
     int B = 1, lb = 0;
-    while (2 * B * L <= kFftBL && 2 * B <= 32) { B *= 2; lb++; }
+    while (2 * B <= 16 && ((size_t)2 * B * L + L) * sizeof(float2) <= kColsLds) { B *= 2; lb++; }
+    int Bh = B;
+    while (Bh > 1 && Bh * L > kFftBL) Bh /= 2;
     g.B = B;
     g.lb = lb;
+    g.Bh = Bh;
     g.nab = (int)ceil_div((size_t)na, (size_t)g.B);
     g.n_in = n_in;
     g.n_out = n_out;
