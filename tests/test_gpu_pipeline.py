@@ -21,6 +21,22 @@ def test_structure_tensor_bit_exact(hip, oracle, spacing):
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("shape", [(5, 7, 66), (65537, 2, 3)])
+def test_structure_tensor_unaligned_output(hip, oracle, shape):
+    """An output that starts 4 bytes into a buffer, with the tiled kernel and
+    (65537 planes, beyond the grid's z limit) the grid-stride one: the same
+    bits as the restatement."""
+    rng = np.random.default_rng(12)
+    vol = rng.normal(size=shape).astype(np.float32)
+    n = int(np.prod(shape))
+    d_in = DeviceArray.from_host(vol)
+    d_out = DeviceArray((n * 6 + 1,))
+    _lib.call("tira_structure_tensor3_async", d_in.ptr, *shape, 1.0, 1.5, 0.75, d_out.ptr + 4)
+    _lib.call("tira_hip_synchronize")
+    got = d_out.to_host()[1:].reshape(shape + (6,))
+    assert np.array_equal(got, oracle.structure_tensor3(vol, (1.0, 1.5, 0.75)))
+
+
 def test_structure_tensor_rejects_thin_axes(hip):
     with pytest.raises(ValueError):
         api.structure_tensor3(DeviceArray((1, 8, 8)))

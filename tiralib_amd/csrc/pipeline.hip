@@ -14,38 +14,70 @@
 
 namespace th {
 
-// One voxel per thread (grid-stride). x = axis 0 (fastest, s2 voxels),
-// y = axis 1 (s1), z = axis 2 (s0). Output packed-6 (xx, xy, xz, yy, yz, zz)
-// with tensor axis 0 along x.
-__global__ __launch_bounds__(256) void k_structure_tensor(const float* __restrict__ I, unsigned s0, unsigned s1,
-                                                          unsigned s2, double dx, double dy, double dz,
-                                                          float* __restrict__ out) {
+// g g^T of voxel (x, y, z) at linear index i into o[0..5]. x = axis 0
+// (fastest, s2 voxels), y = axis 1 (s1), z = axis 2 (s0); output packed-6
+// (xx, xy, xz, yy, yz, zz) with tensor axis 0 along x.
+__device__ __forceinline__ void structure_tensor_voxel(const float* __restrict__ I, unsigned x, unsigned y, unsigned z,
+                                                       size_t i, unsigned s0, unsigned s1, unsigned s2, double dx,
+                                                       double dy, double dz, float* __restrict__ o) {
+    const size_t plane = (size_t)s1 * s2;
+    float g[3];
+    const unsigned pos[3] = {x, y, z}, len[3] = {s2, s1, s0};
+    const size_t stride[3] = {1, s2, plane};
+    const double sp[3] = {dx, dy, dz};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        if (pos[a] == 0) g[a] = (float)((double)(I[i + stride[a]] - I[i]) / sp[a]);
+        else if (pos[a] == len[a] - 1) g[a] = (float)((double)(I[i] - I[i - stride[a]]) / sp[a]);
+        else g[a] = (float)((double)(I[i + stride[a]] - I[i - stride[a]]) / (2.0 * sp[a]));
+    }
+    o[0] = g[0] * g[0]; o[1] = g[0] * g[1]; o[2] = g[0] * g[2];
+    o[3] = g[1] * g[1]; o[4] = g[1] * g[2]; o[5] = g[2] * g[2];
+}
+
+constexpr unsigned kStX = 64, kStY = 4;
+
+// One voxel per thread, blocks of 64 (x) x 4 (y) voxels of one plane: the
+// coordinates come from the block index (a grid-stride loop splits the linear
+// index with three 64-bit divisions per voxel: 0.92 -> 0.83 ms at
+// 512 x 512 x 384). Staging a wave's 64 x 6 outputs in LDS for 8-byte stores
+// of consecutive addresses measured 10% slower than the six 4-byte stores.
+__global__ __launch_bounds__(kStX * kStY) void k_structure_tensor(const float* __restrict__ I, unsigned s0,
+                                                                  unsigned s1, unsigned s2, double dx, double dy,
+                                                                  double dz, float* __restrict__ out) {
+    const unsigned lane = threadIdx.x % kStX, wave = threadIdx.x / kStX;
+    const unsigned x0 = blockIdx.x * kStX, x = x0 + lane, y = blockIdx.y * kStY + wave;
+    const unsigned z = blockIdx.z;
+    if (y >= s1) return;  // whole waves
+    const unsigned nv = min(kStX, s2 - x0);  // voxels of this wave's run
+    const size_t i0 = ((size_t)z * s1 + y) * s2 + x0;
+    if (lane < nv) structure_tensor_voxel(I, x, y, z, i0 + lane, s0, s1, s2, dx, dy, dz, out + (i0 + lane) * 6);
+}
+
+// Grid-stride form for shapes beyond the 3D grid's y and z limits (65535).
+__global__ __launch_bounds__(256) void k_structure_tensor_flat(const float* __restrict__ I, unsigned s0, unsigned s1,
+                                                               unsigned s2, double dx, double dy, double dz,
+                                                               float* __restrict__ out) {
     const size_t plane = (size_t)s1 * s2;
     const size_t n = plane * s0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const unsigned x = (unsigned)(i % s2), y = (unsigned)((i / s2) % s1), z = (unsigned)(i / plane);
-        float g[3];
-        const unsigned pos[3] = {x, y, z}, len[3] = {s2, s1, s0};
-        const size_t stride[3] = {1, s2, plane};
-        const double sp[3] = {dx, dy, dz};
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            if (pos[a] == 0) g[a] = (float)((double)(I[i + stride[a]] - I[i]) / sp[a]);
-            else if (pos[a] == len[a] - 1) g[a] = (float)((double)(I[i] - I[i - stride[a]]) / sp[a]);
-            else g[a] = (float)((double)(I[i + stride[a]] - I[i - stride[a]]) / (2.0 * sp[a]));
-        }
-        float* o = out + i * 6;
-        o[0] = g[0] * g[0]; o[1] = g[0] * g[1]; o[2] = g[0] * g[2];
-        o[3] = g[1] * g[1]; o[4] = g[1] * g[2]; o[5] = g[2] * g[2];
+        structure_tensor_voxel(I, x, y, z, i, s0, s1, s2, dx, dy, dz, out + i * 6);
     }
 }
 
 static int launch_structure_tensor(const float* I, unsigned s0, unsigned s1, unsigned s2, double dx, double dy,
                                    double dz, float* out, hipStream_t st) {
     const size_t n = (size_t)s0 * s1 * s2;
-    const unsigned grid = (unsigned)std::min<size_t>(ceil_div(n, 256), 65536);
     ProfScope prof("structure_tensor", st);
-    k_structure_tensor<<<grid, 256, 0, st>>>(I, s0, s1, s2, dx, dy, dz, out);
+    const unsigned gy = (unsigned)ceil_div(s1, kStY);
+    if (gy <= 65535 && s0 <= 65535) {
+        const dim3 grid((unsigned)ceil_div(s2, kStX), gy, s0);
+        k_structure_tensor<<<grid, kStX * kStY, 0, st>>>(I, s0, s1, s2, dx, dy, dz, out);
+    } else {
+        const unsigned grid = (unsigned)std::min<size_t>(ceil_div(n, 256), 65536);
+        k_structure_tensor_flat<<<grid, 256, 0, st>>>(I, s0, s1, s2, dx, dy, dz, out);
+    }
     TH_HIP(hipGetLastError());
     return TIRA_OK;
 }
