@@ -93,7 +93,8 @@ CASES = [
     ((3, 2, 40), 3.0, 0.0, 15, True, True),     # window larger than two axes
     ((12, 10, 14), 1.5, 0.0, 7, True, False),
     ((12, 10, 14), 1.5, 0.0, 7, False, True),
-    ((2, 3, 300), 1.5, 0.0, 7, True, True),     # x length 315: two channel groups, each eigen-solving its voxels
+    ((2, 3, 300), 1.5, 0.0, 7, True, True),     # x length 315
+    ((2, 3, 636), 1.5, 0.0, 7, True, True),     # x length 640: 8 staged lines transformed as two groups of 4
 ]
 
 

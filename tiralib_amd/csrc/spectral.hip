@@ -786,7 +786,7 @@ __device__ __forceinline__ void add_circle_average(const float4 N, float* __rest
 template <bool FROM_TENSORS>
 __global__ __launch_bounds__(kFT, 4) void k_spec_xfwd(const float4* __restrict__ ps, const float4* __restrict__ pp,
                                                    const float* __restrict__ tensors, int layout, SpecGeom g, int B,
-                                                   int C, FftPlan pl, const float2* __restrict__ twg,
+                                                   int C, int FG, FftPlan pl, const float2* __restrict__ twg,
                                                    float2* __restrict__ Z) {
     extern __shared__ float2 xs_dyn[];  // C B L staged values, then L twiddles
     const int L = pl.L;
@@ -852,7 +852,8 @@ __global__ __launch_bounds__(kFT, 4) void k_spec_xfwd(const float4* __restrict__
                 if (j >= j0 && j < j0 + Cv) s[((j - j0) * Bv + b) * L + e] = make_float2(A[2 * j], A[2 * j + 1]);
         }
         __syncthreads();
-        fft_lines<false>(s, pl, Cv * Bv, tw);
+        // the staged lines in groups of FG (each group fits the stages' register rounds)
+        for (int l0 = 0; l0 < Cv * Bv; l0 += FG) fft_lines<false>(s + l0 * L, pl, min(FG, Cv * Bv - l0), tw);
         // each wave stores whole lines (their bases are wave-uniform)
         for (int line = (int)threadIdx.x >> 6; line < Cv * Bv; line += kFT / 64) {
             const int ln = __builtin_amdgcn_readfirstlane(line);
@@ -1575,30 +1576,36 @@ int vote3_spectral_run(const float* tensors, int in_layout, const float* lam, co
     g.ocs = (size_t)g.h0 * g.h1 * g.h2;
 
     const size_t nvox = (size_t)s0 * s1 * s2;
-    // The fewest channel groups whose lines of one row fit every stage's
-    // register rounds and 64 KB of LDS, balanced (512-wide rows: 8 + 7
-    // channels instead of 7 + 7 + 1, one payload pass less).
+    // Channels per group: all of them when the lines of one row fit 64 KB of
+    // LDS, so that each voxel's eigen-solve and features are computed once;
+    // else the fewest balanced groups that fit. The staged lines are
+    // transformed in groups of FG that fit every stage's register rounds (at
+    // L = 540 two channel groups had solved every voxel twice).
     int C = g.nz;
     for (int ng = 1; ng <= g.nz; ng++) {
         C = (g.nz + ng - 1) / ng;
-        if (lines_fit(p2, C) && ((size_t)C * g.L2 + g.L2) * 8 <= 65536) break;
+        if (((size_t)C * g.L2 + g.L2) * 8 <= 65536) break;
     }
-    // Rows per block: among the counts whose lines fit every stage's rounds
-    // and 64 KB of dynamic LDS, the one whose voxels best fill whole passes
-    // of the block's threads in the eigen-solve (ties: more rows). 128-wide
-    // rows of 8 channels take 4 rows (512 voxels, 4320 values), not the 3
-    // that fit 4096 values (384 voxels: half the threads solving twice).
+    // Rows per block: among the counts that fit 64 KB of dynamic LDS (and,
+    // where one row's lines fit the stages' rounds, whose lines all do), the
+    // one whose voxels best fill whole passes of the block's threads in the
+    // eigen-solve (ties: more rows). 128-wide rows of 8 channels take 4 rows
+    // (512 voxels, 4320 values), not the 3 that fit 4096 values (384 voxels:
+    // half the threads solving twice).
+    const bool row_fits = lines_fit(p2, C);
     int B = 1;
     {
         double best = -1.0;
         for (int b = 1; b <= 64; b++) {
-            if (!lines_fit(p2, C * b) || ((size_t)C * b * g.L2 + g.L2) * 8 > 65536) break;
+            if (((size_t)C * b * g.L2 + g.L2) * 8 > 65536 || (row_fits && !lines_fit(p2, C * b))) break;
             const int vox = b * g.n2, passes = (vox + kFT - 1) / kFT;
             const double fill = (double)vox / ((double)passes * kFT);
             if (fill >= best) { best = fill; B = b; }
             if (vox >= 4 * kFT) break;
         }
     }
+    int FG = C * B;
+    for (int ng = 1; !lines_fit(p2, FG); ng++) FG = (C * B + ng) / (ng + 1);
     const size_t xlds = ((size_t)C * B * g.L2 + g.L2) * 8;
     // tensor input: the x pass decomposes the voxels itself, in every channel
     // group (no payload round trip); eigenpair input goes through the payload
@@ -1636,9 +1643,9 @@ int vote3_spectral_run(const float* tensors, int in_layout, const float* lam, co
         const size_t nblk = ceil_div(nrows, (size_t)B);
         TH_ARG(nblk < (1ull << 31), "spectral vote: volume too large");
         if (fused)
-            k_spec_xfwd<true><<<(unsigned)nblk, kFT, xlds, st>>>(ps, pp, tensors, in_layout, g, B, C, p2, tw2, Z);
+            k_spec_xfwd<true><<<(unsigned)nblk, kFT, xlds, st>>>(ps, pp, tensors, in_layout, g, B, C, FG, p2, tw2, Z);
         else
-            k_spec_xfwd<false><<<(unsigned)nblk, kFT, xlds, st>>>(ps, pp, tensors, in_layout, g, B, C, p2, tw2, Z);
+            k_spec_xfwd<false><<<(unsigned)nblk, kFT, xlds, st>>>(ps, pp, tensors, in_layout, g, B, C, FG, p2, tw2, Z);
         TH_HIP(hipGetLastError());
     }
     {
