@@ -773,14 +773,15 @@ __device__ __forceinline__ void add_circle_average(const float4 N, float* __rest
 }
 
 // Forward x pass fused with the features. A block takes B rows (z, y) of the
-// field and the complex channels in groups of C: every element computes all
-// features of its voxel from one payload read, channel j = features
-// (2j, 2j+1) as (real, imaginary), lines (channel, row) are transformed
-// together. Each group re-reads its voxels.
+// field and the complex channels in groups of C (all of them unless a row's
+// lines exceed 64 KB of LDS): every element computes all features of its
+// voxel from one payload read, channel j = features (2j, 2j+1) as (real,
+// imaginary), and the lines (channel, row) are transformed FG at a time.
+// Each channel group re-reads its voxels.
 // FROM_TENSORS: each element eigen-decomposes its voxel of the input field
-// (layout) instead of reading a payload, once per channel group (wide rows
-// take two or more groups: a second eigen-solve costs less than writing and
-// re-reading a 32-byte payload per voxel).
+// (layout) instead of reading a payload, once per channel group (a second
+// eigen-solve costs less than writing and re-reading a 32-byte payload per
+// voxel).
 // Four waves per SIMD (at most 128 VGPRs): unbounded, the eigen-solve and the
 // features take 136 and three waves; bounded, they fit in 120 without a spill.
 template <bool FROM_TENSORS>
