@@ -212,23 +212,27 @@ __global__ __launch_bounds__(256) void k_vote2_prep_tk(const float2* __restrict_
 // `per` entries: chunk z writes its partial sums to part[z * n + pixel] and
 // k_vote2_reduce adds the chunks in order (deterministic). With part == NULL
 // (one chunk) the sums go straight to out.
-__global__ __launch_bounds__(256) void k_vote2(const float4* __restrict__ pay, const Nb2* __restrict__ nb, int nbc,
-                                               int per, int s0, int s1, int power, int stick, int plate,
-                                               float* __restrict__ out, float4* __restrict__ part) {
-    const int x1 = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int x0 = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int k0 = (int)blockIdx.z * per, k1 = min(nbc, k0 + per);
-    float a = 0.0f, b = 0.0f, c = 0.0f;
+// The entries of one receiver, branch-free: a voter outside the field reads
+// voxel 0 with zero weights (adds exactly nothing), so the loop unrolls and
+// the table and voter reads of several entries are in flight together (a
+// loop that skipped them waited on each entry's reads in turn: 1.39 -> 1.27
+// ms per search-loop evaluation). Staging the table in LDS per block as well
+// measured 11% slower there (68 VGPRs and a barrier per 256 entries).
+template <bool STICK, bool PLATE, bool P1>
+__device__ __forceinline__ void vote2_entries(const float4* __restrict__ pay, const Nb2* __restrict__ nb, int k0,
+                                              int k1, int x0, int x1, int s0, int s1, int power, float& a, float& b,
+                                              float& c) {
+#pragma unroll 4
     for (int k = k0; k < k1; k++) {
         const Nb2 e = nb[k];
         const int r0 = x0 + e.dv, r1 = x1 + e.du;
-        if ((unsigned)r0 >= (unsigned)s0 || (unsigned)r1 >= (unsigned)s1) continue;
-        const float4 P = pay[(size_t)r0 * s1 + r1];
-        if (stick) {
+        const bool inb = (unsigned)r0 < (unsigned)s0 && (unsigned)r1 < (unsigned)s1;
+        const float4 P = pay[inb ? (size_t)r0 * s1 + r1 : 0];
+        if (STICK) {
             const float qTd = fmaf(P.x, e.dx, P.y * e.dy);
             const float q2 = qTd * qTd;
             float D;
-            if (power == 1) D = fmaf(e.c2 - e.c1, q2, e.c1);
+            if (P1) D = fmaf(e.c2 - e.c1, q2, e.c1);
             else {
                 const float t1 = 1.0f - q2;
                 float e1 = t1, e2 = q2;
@@ -236,13 +240,33 @@ __global__ __launch_bounds__(256) void k_vote2(const float4* __restrict__ pay, c
                 D = fmaf(e.c1, e1, e.c2 * e2);
             }
             const float Rqx = fmaf(e.Ra, P.x, e.Rb * P.y), Rqy = fmaf(e.Rb, P.x, e.Rc * P.y);
-            const float f = D * P.z;
+            const float f = D * (inb ? P.z : 0.0f);
             const float fx = f * Rqx;
             a = fmaf(fx, Rqx, a); b = fmaf(fx, Rqy, b); c = fmaf(f * Rqy, Rqy, c);
         }
-        if (plate) {
-            a = fmaf(P.w, e.Va, a); b = fmaf(P.w, e.Vb, b); c = fmaf(P.w, e.Vc, c);
+        if (PLATE) {
+            const float wp = inb ? P.w : 0.0f;
+            a = fmaf(wp, e.Va, a); b = fmaf(wp, e.Vb, b); c = fmaf(wp, e.Vc, c);
         }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_vote2(const float4* __restrict__ pay, const Nb2* __restrict__ nb, int nbc,
+                                               int per, int s0, int s1, int power, int stick, int plate,
+                                               float* __restrict__ out, float4* __restrict__ part) {
+    const int x1 = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int x0 = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int k0 = (int)blockIdx.z * per, k1 = min(nbc, k0 + per);
+    float a = 0.0f, b = 0.0f, c = 0.0f;
+    const bool p1 = power == 1;
+    if (stick && plate) {
+        if (p1) vote2_entries<true, true, true>(pay, nb, k0, k1, x0, x1, s0, s1, power, a, b, c);
+        else vote2_entries<true, true, false>(pay, nb, k0, k1, x0, x1, s0, s1, power, a, b, c);
+    } else if (stick) {
+        if (p1) vote2_entries<true, false, true>(pay, nb, k0, k1, x0, x1, s0, s1, power, a, b, c);
+        else vote2_entries<true, false, false>(pay, nb, k0, k1, x0, x1, s0, s1, power, a, b, c);
+    } else if (plate) {
+        vote2_entries<false, true, true>(pay, nb, k0, k1, x0, x1, s0, s1, power, a, b, c);
     }
     if (x0 < s0 && x1 < s1) {
         const size_t p = (size_t)x0 * s1 + x1;
@@ -344,11 +368,15 @@ __global__ __launch_bounds__(256) void k_vote2_atv(const float4* __restrict__ pa
     const int x0 = blockIdx.y * 4 + (threadIdx.x >> 6);
     const int k0 = (int)blockIdx.z * per, k1 = min(nt, k0 + per);
     float xx = 0.0f, xy2 = 0.0f, yy = 0.0f, ii = 0.0f;
+    // branch-free as vote2_entries: a voter outside the field weighs zero
+#pragma unroll 4
     for (int k = k0; k < k1; k++) {
         const Atv2 e = tab[k];
         const int r0 = x0 + e.dv, r1 = x1 + e.du;
-        if ((unsigned)r0 >= (unsigned)s0 || (unsigned)r1 >= (unsigned)s1) continue;
-        const float4 P = pay[(size_t)r0 * s1 + r1];
+        const bool inb = (unsigned)r0 < (unsigned)s0 && (unsigned)r1 < (unsigned)s1;
+        float4 P = pay[inb ? (size_t)r0 * s1 + r1 : 0];
+        P.z = inb ? P.z : 0.0f;
+        P.w = inb ? P.w : 0.0f;
         const float a = fmaf(P.x, e.vx, P.y * e.vy);
         const float al = a * a;
         const float K11 = P.z * fmaf(e.nc, al, e.c);
