@@ -1038,8 +1038,8 @@ def main(argv=None):
     ap.add_argument("--no-vote2d", action="store_true")
     a = ap.parse_args(argv)
     if a.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); the vote shards by z-slabs with an "
-              f"r-plane halo exchange, which this build does not run", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); the vote would shard by z-slabs with "
+              f"an r-plane halo exchange, which the library does not implement", file=sys.stderr)
         return 2
 
     from tiralib_amd import _lib

@@ -38,7 +38,7 @@ typedef enum tira_status {
     TIRA_ERR_INVALID_ARG = 1,   /* bad shape / sigma / window (pybind checks, tensorvote_tk.cu:73-83, 162-178) */
     TIRA_ERR_HIP = 2,           /* a HIP runtime call failed */
     TIRA_ERR_OUT_OF_MEMORY = 3,
-    TIRA_ERR_UNSUPPORTED = 4,   /* valid request this build does not serve */
+    TIRA_ERR_UNSUPPORTED = 4,   /* valid request the library does not serve */
     TIRA_ERR_NO_DEVICE = 5
 } tira_status;
 
