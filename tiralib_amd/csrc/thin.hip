@@ -125,6 +125,24 @@ __device__ __forceinline__ void wave_append(bool take, unsigned value, unsigned*
     if (take) list[base + __popcll(m & ((1ull << lane) - 1ull))] = value;
 }
 
+// As wave_append, with a second value per entry appended to `list2` at the
+// same slot.
+__device__ __forceinline__ void wave_append2(bool take, unsigned value, unsigned value2, unsigned* __restrict__ list,
+                                             unsigned* __restrict__ list2, unsigned* __restrict__ count) {
+    const unsigned long long m = __ballot(take);
+    if (!m) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(count, (unsigned)__popcll(m));
+    base = __shfl(base, leader);
+    if (take) {
+        const unsigned slot = base + __popcll(m & ((1ull << lane) - 1ull));
+        list[slot] = value;
+        list2[slot] = value2;
+    }
+}
+
 __device__ __forceinline__ void wave_count(bool hit, unsigned* __restrict__ count) {
     const unsigned long long m = __ballot(hit);
     if (m && (threadIdx.x & 63) == (unsigned)(__ffsll((long long)m) - 1)) atomicAdd(count, (unsigned)__popcll(m));
@@ -201,10 +219,14 @@ __global__ __launch_bounds__(256) void k_thin_objects(const unsigned char* __res
 
 // Candidates of border direction `dir` among the listed object voxels (the
 // same tests as k_thin_candidates, which scans the whole volume).
+// Each candidate's 26-neighbourhood code goes to codes[] beside its index:
+// the volume does not change until the direction's deletions are applied, so
+// the re-check sweeps read it instead of the 26 neighbours.
 __global__ __launch_bounds__(256) void k_thin_candidates_list(const unsigned char* __restrict__ v, int s0, int s1,
                                                               int s2, int dir, const unsigned* __restrict__ obj,
                                                               unsigned nobj, unsigned char* __restrict__ mark,
                                                               unsigned* __restrict__ list,
+                                                              unsigned* __restrict__ codes,
                                                               unsigned* __restrict__ count) {
     const int ddx = dir == 2 ? 1 : dir == 3 ? -1 : 0;
     const int ddy = dir == 0 ? -1 : dir == 1 ? 1 : 0;
@@ -214,7 +236,7 @@ __global__ __launch_bounds__(256) void k_thin_candidates_list(const unsigned cha
     for (unsigned it = 0; it < iters; it++) {
         const unsigned k = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
         bool take = false;
-        unsigned i = 0;
+        unsigned i = 0, code = 0;
         if (k < nobj) {
             i = obj[k];
             if (v[i] == 1) {
@@ -224,13 +246,13 @@ __global__ __launch_bounds__(256) void k_thin_candidates_list(const unsigned cha
                 const bool border = (unsigned)X >= (unsigned)s2 || (unsigned)Y >= (unsigned)s1 ||
                                     (unsigned)Z >= (unsigned)s0 || v[((size_t)Z * s1 + Y) * s2 + X] == 0;
                 if (border) {
-                    const unsigned code = thin_code(v, s0, s1, s2, x, y, z);
+                    code = thin_code(v, s0, s1, s2, x, y, z);
                     take = !thin_endpoint(code) && thin_euler_invariant(code) && thin_simple(code);
                 }
             }
         }
         if (take) mark[i] = 1;
-        wave_append(take, i, list, count);
+        wave_append2(take, i, code, list, codes, count);
     }
 }
 
@@ -240,8 +262,11 @@ __global__ __launch_bounds__(256) void k_thin_candidates_list(const unsigned cha
 // currently marked for deletion are removed; otherwise kept (mark 2).
 // The candidate count is read from device memory (the candidate pass wrote
 // it), so the host launches over an upper bound without reading it back.
-__global__ __launch_bounds__(256) void k_thin_resolve(const unsigned char* __restrict__ v, int s0, int s1, int s2,
-                                                      const unsigned* __restrict__ list,
+// codes[k]: candidate k's neighbourhood code (k_thin_candidates_list); a set
+// bit is an object neighbour, inside the volume, so its index is the
+// candidate's plus the bit's offset.
+__global__ __launch_bounds__(256) void k_thin_resolve(int s1, int s2, const unsigned* __restrict__ list,
+                                                      const unsigned* __restrict__ codes,
                                                       const unsigned* __restrict__ count,
                                                       const unsigned char* __restrict__ mark_in,
                                                       unsigned char* __restrict__ mark_out,
@@ -249,20 +274,19 @@ __global__ __launch_bounds__(256) void k_thin_resolve(const unsigned char* __res
     const unsigned cnt = *count;
     const unsigned stride = gridDim.x * blockDim.x;
     const unsigned iters = (cnt + stride - 1) / stride;
+    const long long plane = (long long)s1 * s2;
     for (unsigned it = 0; it < iters; it++) {
         const unsigned k = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
         bool changed = false;
         if (k < cnt) {
             const size_t i = list[k];
-            const int x = (int)(i % (size_t)s2), y = (int)((i / (size_t)s2) % (size_t)s1);
-            const int z = (int)(i / ((size_t)s1 * s2));
-            unsigned code = thin_code(v, s0, s1, s2, x, y, z);
+            unsigned code = codes[k];
             unsigned earlier = code & 0x1FFFu;
             while (earlier) {
                 const int b = __builtin_ctz(earlier);
                 earlier &= earlier - 1;
                 const int dx = b % 3 - 1, dy = (b / 3) % 3 - 1, dz = b / 9 - 1;
-                const size_t q = ((size_t)(z + dz) * s1 + (y + dy)) * s2 + (x + dx);
+                const size_t q = (size_t)((long long)i + dz * plane + (long long)dy * s2 + dx);
                 if (mark_in[q] == 1) code &= ~(1u << b);
             }
             const unsigned char m = thin_simple(code) ? 1 : 2;
@@ -307,6 +331,10 @@ __global__ __launch_bounds__(256) void k_thin_classify(unsigned first, unsigned 
         flags[k] = (unsigned char)((thin_euler_invariant(code) ? 1 : 0) | (thin_simple(code) ? 2 : 0) |
                                    (thin_endpoint(code) ? 4 : 0));
     }
+}
+
+__global__ __launch_bounds__(256) void k_thin_zero_n(unsigned* __restrict__ c, int n) {
+    for (int i = (int)threadIdx.x; i < n; i += (int)blockDim.x) c[i] = 0u;
 }
 
 __global__ void k_thin_zero(unsigned* __restrict__ c, int n) {
@@ -440,6 +468,8 @@ int otsu_run(const float* v, size_t n, int stride, float* thr, hipStream_t st) {
     return TIRA_OK;
 }
 
+constexpr int kThinSlots = 34;  // counters per direction in an outer pass (1 + 33 sweeps)
+
 // Lee thinning of the 0/1 byte volume `v` (device) in place; blocking. The
 // object voxels are compacted into a list once and the list is refiltered
 // after every outer pass, so each border direction examines the remaining
@@ -450,7 +480,7 @@ int lee_run(unsigned char* v, unsigned s0, unsigned s1, unsigned s2, unsigned* i
     unsigned *list = nullptr, *cnt = nullptr, *objA = nullptr, *objB = nullptr;
     TH_TRY(ws_get(WS_SCRATCH0, n, (void**)&ma));
     TH_TRY(ws_get(WS_SCRATCH1, n, (void**)&mb));
-    TH_TRY(ws_get(WS_COUNTER, 64, (void**)&cnt));
+    TH_TRY(ws_get(WS_COUNTER, (16 + 6 * kThinSlots) * 4, (void**)&cnt));
     k_fill_u8<<<grid_for(n), 256, 0, st>>>(ma, n);
     k_fill_u8<<<grid_for(n), 256, 0, st>>>(mb, n);
     TH_HIP(hipGetLastError());
@@ -473,20 +503,24 @@ int lee_run(unsigned char* v, unsigned s0, unsigned s1, unsigned s2, unsigned* i
     k_thin_zero<<<1, 64, 0, st>>>(cnt + 3, 1);
     k_thin_objects<<<grid_for(n), 256, 0, st>>>(v, n, nullptr, 0, objA, cnt + 3);
     TH_HIP(hipGetLastError());
-    // Counters: [0] candidates of the current direction, [1] [2] changes of the
-    // two sweeps of a batch, [3] object voxels, [8 + dir] deletions of this
-    // outer pass. The re-check and apply passes read the candidate count on
-    // the device (launched over the object count, an upper bound), so a
-    // direction needs one read-back per two sweeps and an outer pass one more.
+    // Counters: [1] [2] changes of the two sweeps of a batch beyond the
+    // per-direction slots, [3] object voxels, [8 + dir] deletions of this
+    // outer pass, then per direction kThinSlots slots: [0] its candidates,
+    // [1 + s] the changes of its re-check sweep s. One launch clears a pass's
+    // counters; the re-check and apply passes read the candidate count on the
+    // device (launched over the object count, an upper bound), so a direction
+    // needs one read-back per two sweeps and an outer pass one more. The
+    // candidates' neighbourhood codes go to objB, which is free until the
+    // pass's object list is rebuilt.
     int iters = 0;
-    unsigned host_pass[7];
+    unsigned host_pass[6];
     for (;;) {
         iters++;
-        k_thin_zero<<<1, 64, 0, st>>>(cnt + 8, 6);
+        k_thin_zero_n<<<1, 256, 0, st>>>(cnt + 8, 8 + 6 * kThinSlots);
         for (int dir = 0; dir < 6; dir++) {
-            k_thin_zero<<<1, 64, 0, st>>>(cnt, 3);
+            unsigned* dc = cnt + 16 + dir * kThinSlots;
             k_thin_candidates_list<<<grid_for(nobj), 256, 0, st>>>(v, (int)s0, (int)s1, (int)s2, dir, objA, nobj, ma,
-                                                                   list, cnt);
+                                                                   list, objB, dc);
             TH_HIP(hipGetLastError());
             unsigned char *in = ma, *out = mb;
             for (unsigned sweep = 0;; sweep += 2) {
@@ -494,18 +528,21 @@ int lee_run(unsigned char* v, unsigned s0, unsigned s1, unsigned s2, unsigned* i
                 // there are at most nobj candidates; a sweep that changes nothing is a fixed point,
                 // so the second sweep of a batch may repeat a settled state harmlessly
                 TH_ARG(sweep <= nobj + 2, "thinning re-check did not settle (internal error)");
-                k_thin_zero<<<1, 64, 0, st>>>(cnt + 1, 2);
+                unsigned* ch = dc + 1 + sweep;
+                if (sweep + 2 > (unsigned)kThinSlots - 1) {  // beyond the cleared slots
+                    ch = cnt + 1;
+                    k_thin_zero<<<1, 64, 0, st>>>(ch, 2);
+                }
                 for (int b = 0; b < 2; b++) {
-                    k_thin_resolve<<<grid_for(nobj), 256, 0, st>>>(v, (int)s0, (int)s1, (int)s2, list, cnt, in, out,
-                                                                   cnt + 1 + b);
+                    k_thin_resolve<<<grid_for(nobj), 256, 0, st>>>((int)s1, (int)s2, list, objB, dc, in, out, ch + b);
                     std::swap(in, out);
                 }
                 TH_HIP(hipGetLastError());
-                TH_HIP(hipMemcpyAsync(host + 2, cnt + 2, 4, hipMemcpyDeviceToHost, st));
+                TH_HIP(hipMemcpyAsync(host + 2, ch + 1, 4, hipMemcpyDeviceToHost, st));
                 TH_HIP(hipStreamSynchronize(st));
                 if (host[2] == 0) break;
             }
-            k_thin_apply<<<grid_for(nobj), 256, 0, st>>>(v, list, cnt, ma, mb, in == ma ? 1 : 0, cnt + 8 + dir);
+            k_thin_apply<<<grid_for(nobj), 256, 0, st>>>(v, list, dc, ma, mb, in == ma ? 1 : 0, cnt + 8 + dir);
             TH_HIP(hipGetLastError());
         }
         // drop deleted voxels from the object list; read it with this pass's deletions
