@@ -265,12 +265,17 @@ __global__ __launch_bounds__(256) void k_thin_candidates_list(const unsigned cha
 // codes[k]: candidate k's neighbourhood code (k_thin_candidates_list); a set
 // bit is an object neighbour, inside the volume, so its index is the
 // candidate's plus the bit's offset.
+// prev (optional): the previous sweep's change count; zero means the marks
+// are already a fixed point (mark_in equals mark_out on every candidate), and
+// the sweep returns at once.
 __global__ __launch_bounds__(256) void k_thin_resolve(int s1, int s2, const unsigned* __restrict__ list,
                                                       const unsigned* __restrict__ codes,
                                                       const unsigned* __restrict__ count,
                                                       const unsigned char* __restrict__ mark_in,
                                                       unsigned char* __restrict__ mark_out,
-                                                      unsigned* __restrict__ changes) {
+                                                      unsigned* __restrict__ changes,
+                                                      const unsigned* __restrict__ prev) {
+    if (prev && *prev == 0u) return;
     const unsigned cnt = *count;
     const unsigned stride = gridDim.x * blockDim.x;
     const unsigned iters = (cnt + stride - 1) / stride;
@@ -469,6 +474,7 @@ int otsu_run(const float* v, size_t n, int stride, float* thr, hipStream_t st) {
 }
 
 constexpr int kThinSlots = 34;  // counters per direction in an outer pass (1 + 33 sweeps)
+constexpr unsigned kThinFirst = 4;  // re-check sweeps of a direction before its first read-back
 
 // Lee thinning of the 0/1 byte volume `v` (device) in place; blocking. The
 // object voxels are compacted into a list once and the list is refiltered
@@ -523,24 +529,30 @@ int lee_run(unsigned char* v, unsigned s0, unsigned s1, unsigned s2, unsigned* i
                                                                    list, objB, dc);
             TH_HIP(hipGetLastError());
             unsigned char *in = ma, *out = mb;
-            for (unsigned sweep = 0;; sweep += 2) {
+            // the first batch runs kThinFirst sweeps (most directions settle
+            // within them; a sweep after a fixed point returns at once), later
+            // batches two, each batch ending with one read-back
+            for (unsigned sweep = 0;;) {
                 // bounded: each sweep settles at least one more level of the raster-order DAG, and
                 // there are at most nobj candidates; a sweep that changes nothing is a fixed point,
-                // so the second sweep of a batch may repeat a settled state harmlessly
-                TH_ARG(sweep <= nobj + 2, "thinning re-check did not settle (internal error)");
+                // so the later sweeps of a batch may repeat a settled state harmlessly
+                TH_ARG(sweep <= nobj + kThinFirst, "thinning re-check did not settle (internal error)");
+                const unsigned nb = sweep == 0 ? kThinFirst : 2;
                 unsigned* ch = dc + 1 + sweep;
-                if (sweep + 2 > (unsigned)kThinSlots - 1) {  // beyond the cleared slots
+                if (sweep + nb > (unsigned)kThinSlots - 1) {  // beyond the cleared slots
                     ch = cnt + 1;
                     k_thin_zero<<<1, 64, 0, st>>>(ch, 2);
                 }
-                for (int b = 0; b < 2; b++) {
-                    k_thin_resolve<<<grid_for(nobj), 256, 0, st>>>((int)s1, (int)s2, list, objB, dc, in, out, ch + b);
+                for (unsigned b = 0; b < nb; b++) {
+                    k_thin_resolve<<<grid_for(nobj), 256, 0, st>>>((int)s1, (int)s2, list, objB, dc, in, out, ch + b,
+                                                                   b ? ch + b - 1 : nullptr);
                     std::swap(in, out);
                 }
                 TH_HIP(hipGetLastError());
-                TH_HIP(hipMemcpyAsync(host + 2, ch + 1, 4, hipMemcpyDeviceToHost, st));
+                TH_HIP(hipMemcpyAsync(host + 2, ch + nb - 1, 4, hipMemcpyDeviceToHost, st));
                 TH_HIP(hipStreamSynchronize(st));
                 if (host[2] == 0) break;
+                sweep += nb;
             }
             k_thin_apply<<<grid_for(nobj), 256, 0, st>>>(v, list, dc, ma, mb, in == ma ? 1 : 0, cnt + 8 + dir);
             TH_HIP(hipGetLastError());
