@@ -163,39 +163,6 @@ __global__ __launch_bounds__(256) void k_thin_export(const unsigned char* __rest
         out[i] = (T)v[i];
 }
 
-// Candidates of border direction `dir` (0..5 = N y-1, S y+1, E x+1, W x-1,
-// U z+1, B z-1; lee_thinning.h:142-147, 623-628): set voxels whose neighbour
-// in that direction is 0 (or outside), that are not end points, are Euler
-// invariant and simple (lee_thinning.h:614-646).
-__global__ __launch_bounds__(256) void k_thin_candidates(const unsigned char* __restrict__ v, int s0, int s1,
-                                                         int s2, int dir, unsigned char* __restrict__ mark,
-                                                         unsigned* __restrict__ list, unsigned* __restrict__ count) {
-    const size_t n = (size_t)s0 * s1 * s2;
-    const int ddx = dir == 2 ? 1 : dir == 3 ? -1 : 0;
-    const int ddy = dir == 0 ? -1 : dir == 1 ? 1 : 0;
-    const int ddz = dir == 4 ? 1 : dir == 5 ? -1 : 0;
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    // every lane of a wave runs the same number of iterations (wave_append uses the whole wave)
-    const size_t iters = (n + stride - 1) / stride;
-    for (size_t it = 0; it < iters; it++) {
-        const size_t i = it * stride + (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-        bool take = false;
-        if (i < n && v[i] == 1) {
-            const int x = (int)(i % (size_t)s2), y = (int)((i / (size_t)s2) % (size_t)s1);
-            const int z = (int)(i / ((size_t)s1 * s2));
-            const int X = x + ddx, Y = y + ddy, Z = z + ddz;
-            const bool border = (unsigned)X >= (unsigned)s2 || (unsigned)Y >= (unsigned)s1 ||
-                                (unsigned)Z >= (unsigned)s0 || v[((size_t)Z * s1 + Y) * s2 + X] == 0;
-            if (border) {
-                const unsigned code = thin_code(v, s0, s1, s2, x, y, z);
-                take = !thin_endpoint(code) && thin_euler_invariant(code) && thin_simple(code);
-            }
-        }
-        if (take) mark[i] = 1;
-        wave_append(take, (unsigned)i, list, count);
-    }
-}
-
 // Object voxels (value 1) of the volume, or of a previous list, appended to
 // `list`; every lane of a wave runs the same number of iterations.
 __global__ __launch_bounds__(256) void k_thin_objects(const unsigned char* __restrict__ v, size_t n,
@@ -217,8 +184,10 @@ __global__ __launch_bounds__(256) void k_thin_objects(const unsigned char* __res
     }
 }
 
-// Candidates of border direction `dir` among the listed object voxels (the
-// same tests as k_thin_candidates, which scans the whole volume).
+// Candidates of border direction `dir` (0..5 = N y-1, S y+1, E x+1, W x-1,
+// U z+1, B z-1; lee_thinning.h:142-147, 623-628) among the listed object
+// voxels: voxels whose neighbour in that direction is 0 (or outside), that are
+// not end points, are Euler invariant and simple (lee_thinning.h:614-646).
 // Each candidate's 26-neighbourhood code goes to codes[] beside its index:
 // the volume does not change until the direction's deletions are applied, so
 // the re-check sweeps read it instead of the 26 neighbours.
@@ -515,7 +484,8 @@ int lee_run(unsigned char* v, unsigned s0, unsigned s1, unsigned s2, unsigned* i
     // [1 + s] the changes of its re-check sweep s. One launch clears a pass's
     // counters; the re-check and apply passes read the candidate count on the
     // device (launched over the object count, an upper bound), so a direction
-    // needs one read-back per two sweeps and an outer pass one more. The
+    // needs one read-back for its first kThinFirst sweeps and one per two
+    // after, and an outer pass one more. The
     // candidates' neighbourhood codes go to objB, which is free until the
     // pass's object list is rebuilt.
     int iters = 0;
