@@ -45,7 +45,11 @@ __global__ void k_fill_zero8(u8* __restrict__ x, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) x[i] = 0;
 }
 
-__global__ void k_clear_flag(unsigned* __restrict__ f) { *f = 0u; }
+__global__ void k_clear_flags(unsigned* __restrict__ f, int n) {
+    if ((int)threadIdx.x < n) f[threadIdx.x] = 0u;
+}
+
+constexpr int kFillBatch = 4;  // sweeps per read-back
 
 constexpr int kFillT = 32;  // tile edge; one thread per pixel
 
@@ -53,8 +57,12 @@ constexpr int kFillT = 32;  // tile edge; one thread per pixel
 // (mask 0) 4-connected to the outside through background. Cells outside the
 // image count as reached. *changed becomes nonzero when the sweep reached a
 // new pixel.
+// prev (optional): the previous sweep's flag; when it is zero the reachable
+// set is already complete and the sweep returns at once.
 __global__ __launch_bounds__(kFillT * kFillT) void k_fill_sweep(const u8* __restrict__ mask, u8* __restrict__ reach,
-                                                                 int h, int w, unsigned* __restrict__ changed) {
+                                                                 int h, int w, unsigned* __restrict__ changed,
+                                                                 const unsigned* __restrict__ prev) {
+    if (prev && *prev == 0u) return;
     __shared__ u8 s_r[kFillT + 2][kFillT + 2];
     const int tx = (int)threadIdx.x % kFillT, ty = (int)threadIdx.x / kFillT;
     const int x0 = (int)blockIdx.x * kFillT, y0 = (int)blockIdx.y * kFillT;
@@ -131,15 +139,23 @@ int tira_fill_holes2(const unsigned char* mask, unsigned h, unsigned w, unsigned
     }
     k_fill_zero8<<<grid, 256, 0, st>>>(reach, n);
     const dim3 tiles(ceil_div(w, kFillT), ceil_div(h, kFillT));
-    // every sweep but the last reaches a new pixel: at most n + 1 sweeps
-    unsigned host_flag = 1, count = 0;
-    while (host_flag && count <= n) {
-        k_clear_flag<<<1, 1, 0, st>>>(flag);
-        k_fill_sweep<<<tiles, kFillT * kFillT, 0, st>>>(boundary, reach, (int)h, (int)w, flag);
+    // every sweep but the last reaches a new pixel: at most n + 1 sweeps. They
+    // run kFillBatch to a read-back; a sweep after one that reached nothing
+    // returns at once, and `sweeps` counts those that ran.
+    unsigned host_flags[kFillBatch], count = 0;
+    bool done = false;
+    while (!done && count <= n) {
+        k_clear_flags<<<1, 64, 0, st>>>(flag, kFillBatch);
+        for (int b = 0; b < kFillBatch; b++)
+            k_fill_sweep<<<tiles, kFillT * kFillT, 0, st>>>(boundary, reach, (int)h, (int)w, flag + b,
+                                                            b ? flag + b - 1 : nullptr);
         TH_HIP(hipGetLastError());
-        TH_HIP(hipMemcpyAsync(&host_flag, flag, 4, hipMemcpyDeviceToHost, st));
+        TH_HIP(hipMemcpyAsync(host_flags, flag, sizeof(host_flags), hipMemcpyDeviceToHost, st));
         TH_HIP(hipStreamSynchronize(st));
-        count++;
+        for (int b = 0; b < kFillBatch && !done; b++) {
+            count++;
+            done = host_flags[b] == 0;
+        }
     }
     k_fill_out<<<grid, 256, 0, st>>>(boundary, reach, n, (u8*)dout);
     TH_HIP(hipGetLastError());
