@@ -563,37 +563,42 @@ def bench_skeleton(sal, shape, st):
     saliency (a PSD voted field, so ranking by value is ranking by magnitude),
     mask = saliency > threshold, Lee 3D thinning of the mask. The thinning loop
     reads counters back per border direction, so the stage is timed as a whole
-    with events and split by the library's profile scopes."""
+    with events and split by the library's profile scopes. The first run is
+    reported as the cold call (it includes the kernels' first loads); the
+    figures are those of the second."""
     from tiralib_amd import _lib
     from tiralib_amd._lib import DeviceArray, Event, call
     n = int(np.prod(shape))
     thr, mask = DeviceArray((1,)), DeviceArray(shape, np.uint8)
-    e0, e1, e2 = Event(), Event(), Event()
     it = _lib.C.c_uint()
+    runs = []
     try:
-        call("tira_hip_profile_begin")
-        e0.record(st)
-        call("tira_otsu_threshold_async", sal.ptr, n, 3, thr.ptr)
-        call("tira_threshold_async", sal.ptr, n, 3, thr.ptr, mask.ptr)
-        e1.record(st)
-        st.synchronize()
-        fg = int(mask.to_host().sum())
-        e1b = Event()
-        e1b.record(st)
-        call("tira_lee_thin3_u8", mask.ptr, *shape, _lib.C.byref(it))
-        e2.record(st)
-        st.synchronize()
-        call("tira_hip_profile_end")
-        t_mask, t_thin = e0.elapsed_ms(e1), e1b.elapsed_ms(e2)
-        otsu_ms, _ = _lib.profile_get("otsu")
+        for _ in range(2):
+            e0, e1, e1b, e2 = Event(), Event(), Event(), Event()
+            call("tira_hip_profile_begin")
+            e0.record(st)
+            call("tira_otsu_threshold_async", sal.ptr, n, 3, thr.ptr)
+            call("tira_threshold_async", sal.ptr, n, 3, thr.ptr, mask.ptr)
+            e1.record(st)
+            st.synchronize()
+            fg = int(mask.to_host().sum())
+            e1b.record(st)
+            call("tira_lee_thin3_u8", mask.ptr, *shape, _lib.C.byref(it))
+            e2.record(st)
+            st.synchronize()
+            call("tira_hip_profile_end")
+            otsu_ms, _ = _lib.profile_get("otsu")
+            runs.append((e0.elapsed_ms(e1), e1b.elapsed_ms(e2), otsu_ms, fg))
         sk = int(mask.to_host().sum())
         t = float(thr.to_host()[0])
     finally:
         thr.free()
         mask.free()
+    t_mask, t_thin, otsu_ms, fg = runs[1]
     ms = t_mask + t_thin
     return {"workload": "Otsu threshold of the stick saliency, mask, Lee 3D thinning (512x512x384)",
-            "ms": round(ms, 2), "otsu_threshold_mask_ms": round(t_mask, 3), "otsu_ms": round(otsu_ms, 3),
+            "ms": round(ms, 2), "cold_first_call_ms": round(runs[0][0] + runs[0][1], 2),
+            "otsu_threshold_mask_ms": round(t_mask, 3), "otsu_ms": round(otsu_ms, 3),
             "lee_thin_ms": round(t_thin, 2), "threshold": t, "mask_voxels": fg, "skeleton_voxels": sk,
             "outer_iterations": int(it.value), "mvoxel_s": round(n / (ms * 1e-3) / 1e6, 1)}
 
