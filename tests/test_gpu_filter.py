@@ -61,6 +61,30 @@ def test_tensor6_same_mode(hip, oracle, shape, sigma, K):
     np.testing.assert_allclose(v, ref[r:-r, r:-r, r:-r], atol=2e-6 * np.abs(ref).max())
 
 
+def test_tensor6_same_mode_deep_chunks(hip, oracle):
+    """200 x 256 x 256 runs in 128-plane z chunks (a 128- and a 72-plane chunk per
+    column): output blocks across the chunk seam (z = 128) and at two corners match the FP64
+    oracle of an input crop with a K/2 margin (clipped at the volume, where the crop's zero
+    padding is the volume's)."""
+    shape = (200, 256, 256)
+    t6 = np.random.default_rng(5).standard_normal(size=shape + (6,), dtype=np.float32)
+    taps = api.gaussian_taps(13, 2.0)
+    dev = DeviceArray.from_host(t6)
+    out = api.gaussian_filter3_tensor(dev, taps, same=True).to_host()
+    dev.free()
+    r, b = 6, 16
+    for corner in [(120, 100, 60), (0, 0, 0), (184, 240, 240)]:
+        lo = [max(c - r, 0) for c in corner]
+        hi = [min(c + b + r, s) for c, s in zip(corner, shape)]
+        crop = np.ascontiguousarray(t6[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]])
+        ref = oracle.gaussian3_same_packed6_f64(crop, taps, nthreads=8)
+        ref = ref[tuple(slice(c - l, c - l + b) for c, l in zip(corner, lo))]
+        got = out[corner[0]:corner[0] + b, corner[1]:corner[1] + b, corner[2]:corner[2] + b]
+        e = _rel_max(got, ref)
+        print(f"tensor6 same 200x256x256 block at {corner}: {e:.2e}")
+        assert e <= 2e-6
+
+
 def test_tensor_constant_field(hip):
     t = np.full((30, 31, 32, 6), 2.5, np.float32)
     taps = api.gaussian_taps(11, 1.5)
