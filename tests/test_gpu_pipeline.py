@@ -21,11 +21,12 @@ def test_structure_tensor_bit_exact(hip, oracle, spacing):
     assert np.array_equal(got, ref)
 
 
-@pytest.mark.parametrize("shape", [(5, 7, 66), (65537, 2, 3)])
+@pytest.mark.parametrize("shape", [(5, 7, 66), (65537, 2, 3), (131073, 2, 3)])
 def test_structure_tensor_unaligned_output(hip, oracle, shape):
-    """An output that starts 4 bytes into a buffer, with the tiled kernel and
-    (65537 planes, beyond the grid's z limit) the grid-stride one: the same
-    bits as the restatement."""
+    """An output that starts 4 bytes into a buffer, with the tiled kernel (two
+    planes per thread; 65537 planes end in a one-plane column) and, for 131073
+    planes (beyond the grid's z limit), the grid-stride one: the same bits as
+    the restatement."""
     rng = np.random.default_rng(12)
     vol = rng.normal(size=shape).astype(np.float32)
     n = int(np.prod(shape))
