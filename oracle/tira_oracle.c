@@ -30,9 +30,11 @@
  *                            sum exact).
  *   3D analytic plate        its Boost.Math terms (beta, 2F1 with -p) are
  *                            restated from their terminating series / Gamma
- *                            closed forms; pinned by the numerical sum only
- *                            (Boost is not in the image, so the reference's
- *                            own call is "parity unpinned").
+ *                            closed forms; pinned by the numerical sum and,
+ *                            term by term, against scipy.special's hyp2f1
+ *                            and beta (independent implementations of the
+ *                            same functions; Boost itself is not in the
+ *                            image).
  */
 #include "tira_oracle.h"
 
@@ -947,7 +949,7 @@ static void o_vote3_receiver_f32(const o_v3ctx* c, int x0, int x1, int x2, float
             if (c->samples > 0) oracle_plate3_numerical_f32(n->d, n->c1, n->c2, c->power, c->P + base * 3, c->samples, V);
             else {
                 /* analytic form restated in FP64 then rounded (the reference mixes FP32 geometry
-                 * with FP64 J/K terms; parity for samples == 0 is unpinned). */
+                 * with FP64 J/K terms, so the two agree to FP32 rounding, not bit for bit). */
                 const float* e = c->P + base * 3;
                 double dd[3] = {n->d[0], n->d[1], n->d[2]}, ee[3] = {e[0], e[1], e[2]}, V6[6];
                 oracle_plate3_analytic_f64(dd, n->c1, n->c2, c->power, ee, V6);

@@ -125,6 +125,32 @@ def test_plate_three_formulations(oracle, p):
     assert np.abs(oracle.plate3_sum_f64(d, 0.7, 0.2, p, n, p + 1) - oracle.plate3_sum_f64(d, 0.7, 0.2, p, n, 97)).max() > 1e-4
 
 
+@pytest.mark.parametrize("p", [1, 2, 3, 5, 8])
+def test_plate_analytic_special_functions_vs_scipy(oracle, p):
+    """The Boost.Math terms of the reference's analytic plate (tensorvote.h:1589-1592,
+    1722-1723): J0 = pi/2 2F1(-p, 3/2; 2; a^2), J1 = pi 2F1(-p, 1/2; 1; a^2),
+    K0 = B(1/2, p + 3/2), K1 = B(1/2, p + 1/2), here from scipy.special (hyp2f1 and
+    beta, implementations independent of the oracle's terminating series and
+    log-Gamma form). For a unit offset d and the normal n = z the reference's local
+    matrices are rank-one blocks plus the y term: c1 A has the eigenvalues
+    {J0, J1 - J0, 0} and c2 B {a^2p K0, a^2p (K1 - K0), 0}, since
+    (1 - 2 a^2)^2 + 4 a^2 d_z^2 = 1 when |d| = 1."""
+    from scipy import special
+    n = np.array([0.0, 0.0, 1.0])
+    rng = np.random.default_rng(40 + p)
+    for _ in range(20):
+        d = _unit(rng.normal(size=3))
+        a2 = 1.0 - d[2] ** 2
+        J0 = 0.5 * np.pi * special.hyp2f1(-p, 1.5, 2.0, a2)
+        J1 = np.pi * special.hyp2f1(-p, 0.5, 1.0, a2)
+        K0, K1 = special.beta(0.5, p + 1.5), special.beta(0.5, p + 0.5)
+        for c1, c2, want in ((1.0, 0.0, [J0, J1 - J0, 0.0]),
+                             (0.0, 1.0, [a2 ** p * K0, a2 ** p * (K1 - K0), 0.0])):
+            u = oracle.plate3_analytic_f64(d, c1, c2, p, n)
+            M = np.array([[u[0], u[1], u[2]], [u[1], u[3], u[4]], [u[2], u[4], u[5]]])
+            np.testing.assert_allclose(np.linalg.eigvalsh(M), np.sort(want), rtol=0, atol=1e-11)
+
+
 def test_plate_fp32_formulation_matches_fp64(oracle):
     rng = np.random.default_rng(9)
     for _ in range(200):
